@@ -1,0 +1,176 @@
+"""Benchmark of the FIT leapfrog hot path on one MI355X.
+
+Workload: the lossy pillbox cavity, 256 x 256 x 768 cells (5.03e7), FP32
+fields, conductive walls (J = sigma*E), 10-cell z PML, PEC x/y faces; every
+step is the wakesolve inner loop on the device: in-kernel Gaussian bunch
+current, E_z probe into the HBM history, H and E updates.
+
+Metric: Gcell-steps/s = Nx*Ny*Nz*steps / stepping time (HIP events on the
+solver stream around exactly `--steps` steps, after `--warmup` untimed steps,
+with device synchronisation on both sides).  The JSON line also carries the
+HBM roofline of the dominant kernel and the CPU oracle timed on a bounded
+sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(nz_sample=32, steps=40):
+    """The CPU oracle (scipy.sparse FP64, single-threaded csc/csr mat-vec, the
+    reference's arithmetic path) on a z-slab of the same pillbox grid."""
+    from oracle.fit_oracle import OGrid, OSolver
+    from wakis_amd.workloads import lossy_pillbox
+
+    w = lossy_pillbox(256, 256, nz_sample)
+    g = w["grid"]
+    og = OGrid(x=g["x"], y=g["y"], z=g["z"])
+    sv = w["solver"]
+    osv = OSolver(og, cfln=sv["cfln"], bc_low=sv["bc_low"], bc_high=sv["bc_high"],
+                  solids=g["solids"], bg=sv["bg"], n_pml=sv["n_pml"], dtype=np.float64)
+    osv.one_step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        osv.one_step()
+    dt = time.perf_counter() - t0
+    cells = osv.N * steps
+    return {
+        "value": cells / dt / 1e9,
+        "unit": "Gcell-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle FP64 scipy.sparse one_step, 256x256x{nz_sample} z-slab of the pillbox "
+                  f"grid, {steps} steps, {dt:.2f} s on 1 core of {cpu_model()}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--grid", default="256,256,768")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "split", "fused"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from wakis_amd.workloads import build, lossy_pillbox
+
+    nx, ny, nz = (int(v) for v in args.grid.split(","))
+    dtype = np.float32 if args.dtype == "f32" else np.float64
+    esz = np.dtype(dtype).itemsize
+    wl = lossy_pillbox(nx, ny, nz)
+    t0 = time.perf_counter()
+    grid, wake, solver = build(wl, dtype=dtype, kernel=args.kernel)
+    setup_s = time.perf_counter() - t0
+    ctx = solver.ctx
+    N = solver.N
+
+    # the wakesolve inner loop: device bunch current + E_z probe + step
+    from wakis_amd.sources import Beam
+
+    beam = Beam(q=wake.q, sigmaz=wake.sigmaz, beta=wake.beta)
+    ixt = int(np.abs(solver.x).argmin())
+    iyt = int(np.abs(solver.y).argmin())
+    ctx.wake_probe(ixt, iyt, 1, args.warmup + args.steps + 200)
+    ctx.set_step_index(0)
+    ctx.set_beam_gauss(*[int(v) for v in (np.abs(solver.x).argmin(), np.abs(solver.y).argmin())],
+                       beam.device_params(solver), solver.z)
+    ctx.zero_ghosts()
+    solver.step_0 = False
+    cond = solver.use_conductivity
+
+    print(f"[bench] {wl['name']} {args.dtype} N={N} setup {setup_s:.1f}s warmup {args.warmup}",
+          file=sys.stderr, flush=True)
+    ctx.step(args.warmup, cond)
+    ctx.sync()
+    wall0 = time.perf_counter()
+    total_ms, _, _ = ctx.step_timed(args.steps, cond, per_kernel=False)
+    wall = time.perf_counter() - wall0
+    ms_per_step = total_ms / args.steps
+    value = N * args.steps / (total_ms * 1e-3) / 1e9
+    print(f"[bench] timed {args.steps} steps: {total_ms:.1f} ms (wall {wall * 1e3:.1f} ms)",
+          file=sys.stderr, flush=True)
+
+    # per-kernel durations for the roofline of the dominant kernel
+    nk = min(args.steps, 100)
+    _, k1, k2 = ctx.step_timed(nk, cond, per_kernel=True)
+    fused = k2 == 0.0
+    if fused:
+        kname, kms, words = "fit_fused_sweep", k1 / nk, 12
+    elif k2 >= k1:
+        kname, kms, words = "e_update_split", k2 / nk, 9
+    else:
+        kname, kms, words = "h_update_split", k1 / nk, 9
+    alg_bytes = words * esz * N
+    achieved = alg_bytes / (kms * 1e-3) / 1e9
+    step_alg = 12 * esz * N / (ms_per_step * 1e-3) / 1e9
+
+    out = {
+        "metric": "Gcell-steps/s",
+        "value": round(value, 3),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {
+            "workload": f"lossy pillbox cavity {nx}x{ny}x{nz} ({N} cells), sigma=1e5 walls, "
+                        "10-cell z PML, wakesolve inner loop (beam + E_z probe + step)",
+            "grid": [nx, ny, nz],
+            "kernel": "fused" if fused else "split",
+            "step_hbm_frac": round(step_alg / HBM_PEAK_GBS, 4),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kname,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "alg_bytes_per_launch": alg_bytes,
+            "avg_launch_ms": round(kms, 5),
+        },
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline (oracle) ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,201 @@
+/*
+ * wakis_fit.h - C ABI of the MI355X (gfx950) FIT time-stepping library.
+ *
+ * Plain C types only (pointers, sizes, ints, doubles): any FFI (ctypes,
+ * cffi, cgo, JNI) can bind it.  Every entry point returns 0 on success and a
+ * negative FIT_E* code on failure; fit_last_error() then returns a message
+ * (per host thread).  A context owns all device memory of one solve on one
+ * HIP device and one HIP stream; host pointers are only borrowed for the
+ * duration of a call.  Calls on one context are not thread-safe.
+ *
+ * Field layout (identical to the reference's Field, wakis/field.py:19-91):
+ * a field is 3 blocks (x, y, z) of N = nx*ny*nz values each, every block in
+ * Fortran order: n = i + nx*j + nx*ny*k.
+ *
+ * The reference is pure Python; its drop-in slot for this path is the
+ * 0-argument callable SolverFIT3D.one_step bound at construction
+ * (wakis/solverFIT3D.py:143, 504-525) plus the host loops emsolve/wakesolve
+ * (wakis/routines.py:17-229, 231-485) and the WakeSolver post-processing
+ * (wakis/wakeSolver.py:423-551, 628-720).  Each entry point below names the
+ * reference code it replaces.
+ */
+#ifndef WAKIS_FIT_H
+#define WAKIS_FIT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fit_ctx fit_ctx;
+
+/* storage / compute precision of the field updates */
+enum { FIT_F64 = 0, FIT_F32 = 1 };
+/* field ids */
+enum { FIT_FIELD_E = 0, FIT_FIELD_H = 1, FIT_FIELD_J = 2 };
+/* source of the current density J used by the E update */
+enum {
+  FIT_J_ZERO = 0,   /* J == 0 everywhere (except the beam line) */
+  FIT_J_SIGMA_E = 1,/* J == sigma*E (state after a conductive step) */
+  FIT_J_BUFFER = 2  /* J read from the J field buffer */
+};
+/* beam line source of J_z at (ixs, iys, :) */
+enum { FIT_BEAM_OFF = 0, FIT_BEAM_VALUES = 1, FIT_BEAM_GAUSS = 2 };
+/* step kernels */
+enum { FIT_KERNEL_AUTO = 0, FIT_KERNEL_SPLIT = 1, FIT_KERNEL_FUSED = 2 };
+
+/* error codes */
+enum {
+  FIT_OK = 0,
+  FIT_EINVAL = -1,  /* bad argument */
+  FIT_EHIP = -2,    /* HIP runtime error */
+  FIT_ENOMEM = -3,  /* device allocation failed */
+  FIT_ESTATE = -4   /* call not valid in the current context state */
+};
+
+typedef struct fit_desc {
+  int32_t nx, ny, nz; /* cells per axis */
+  int32_t dtype;      /* FIT_F64 or FIT_F32 */
+  int32_t device;     /* HIP device ordinal */
+  int32_t reserved[3];
+} fit_desc;
+
+/* Message of the last failed call on this host thread ("" if none). */
+const char *fit_last_error(void);
+/* Library version string. */
+const char *fit_version(void);
+
+/* Allocate a context: E, H, J (3N each, zeroed), per-cell material ids.
+ * Replaces the Field allocations of SolverFIT3D.__init__
+ * (wakis/solverFIT3D.py:182-190). */
+int fit_create(const fit_desc *desc, fit_ctx **out);
+int fit_destroy(fit_ctx *ctx);
+
+/* FIT metric tables, FP64, replacing the diagonal matrices Ds, tDs, iDa, itDa
+ * (wakis/solverFIT3D.py:212-217) after the boundary fix-ups of
+ * wakis/boundaries.py:36-70, 153-181.  Packed layout, in this order:
+ *   L    [nx | ny | nz]     primal lengths   L_x(i), L_y(j), L_z(k)
+ *   tL   [nx | ny | nz]     dual lengths     tL_x(i), tL_y(j), tL_z(k)
+ *   iA_x [ny*nz] (j + ny*k), iA_y [nx*nz] (i + nx*k), iA_z [nx*ny] (i + nx*j)
+ *   itA_x, itA_y, itA_z     same 2-D layouts
+ * total 2*(nx+ny+nz) + 2*(ny*nz + nx*nz + nx*ny) doubles. */
+int fit_set_geometry(fit_ctx *ctx, const double *packed, int64_t count);
+
+/* Materials: ids[N] (uint8, Fortran order; NULL = all cells id 0) and a
+ * table of nmat rows x 9 FP64 values (1/eps x,y,z, 1/mu x,y,z, sigma x,y,z).
+ * Replaces iDeps/iDmu/Dsigma and the operator products
+ * (wakis/solverFIT3D.py:337-344, update_tensors 378-436). */
+int fit_set_materials(fit_ctx *ctx, const uint8_t *ids, int32_t nmat,
+                      const double *table);
+
+/* Boundary masks, bit f set = face f masked, f = xlo,xhi,ylo,yhi,zlo,zhi
+ * (bits 0..5).  pec_cols: tangential-E column mask of C (PEC/PML faces,
+ * wakis/boundaries.py:73-110); pmc_rows: tangential-H row mask of C
+ * (wakis/boundaries.py:113-150, 0 when no PMC block is active). */
+int fit_set_bc(fit_ctx *ctx, uint32_t pec_cols, uint32_t pmc_rows);
+
+/* Time step (already cast to the solver dtype by the caller,
+ * wakis/solverFIT3D.py:291-318). */
+int fit_set_dt(fit_ctx *ctx, double dt);
+
+/* How the E update obtains J (see FIT_J_*). */
+int fit_set_jmode(fit_ctx *ctx, int32_t jmode);
+
+/* Beam line: J_z(ixs, iys, k) for the next steps.
+ * mode FIT_BEAM_VALUES: values[nz] FP64 (already evaluated by the host, as
+ *   sources.Beam.update does, wakis/sources.py:76-110);
+ * mode FIT_BEAM_GAUSS: the Gaussian of wakis/sources.py:96-110 evaluated in
+ *   the kernel each step at t = step*dt; params (FP64) =
+ *   {q*v, 1/sqrt(2*pi*sigmaz^2), 2*sigmaz^2, s0 = zmin - v*ti, v, dx[ixs],
+ *    dy[iys], f32time} and zc[nz] the cell-centre z coordinates; f32time =
+ *   1.0 evaluates t = step*dt and v*t as float32 products (what numpy does
+ *   for a float32 dt), 0.0 in FP64;
+ * mode FIT_BEAM_OFF: no override. */
+int fit_set_beam(fit_ctx *ctx, int32_t mode, int32_t ixs, int32_t iys,
+                 const double *values, const double *params,
+                 const double *zc);
+
+/* Host <-> device copies of a box [i0,i1) x [j0,j1) x [k0,k1) of one
+ * component (0,1,2) of field `which`; host data is dense, Fortran order,
+ * in the context dtype.  Replaces Field.__getitem__/__setitem__ on device
+ * arrays (wakis/field.py:239-249, 313-335).  Reads synchronise. */
+int fit_field_read(fit_ctx *ctx, int32_t which, int32_t comp,
+                   const int32_t box[6], void *host);
+int fit_field_write(fit_ctx *ctx, int32_t which, int32_t comp,
+                    const int32_t box[6], const void *host);
+/* Fill a whole field with zeros. */
+int fit_field_zero(fit_ctx *ctx, int32_t which);
+/* Write the J the next step would use (per the J mode: 0, sigma*E or the
+ * buffer, then the beam line) into the J buffer, so that it can be read. */
+int fit_materialize_j(fit_ctx *ctx);
+/* Current J mode and beam mode (they change after a conductive step:
+ * J mode becomes FIT_J_SIGMA_E and a FIT_BEAM_VALUES line is consumed,
+ * because the reference overwrites all of J with sigma*E). */
+int fit_get_state(fit_ctx *ctx, int32_t *jmode, int32_t *beam_mode);
+
+/* First-step ghost zeroing (wakis/solverFIT3D.py:842-862). */
+int fit_zero_ghosts(fit_ctx *ctx);
+
+/* Absorbing boundary, CPU semantics of the reference (wakis/boundaries.py:
+ * 307-404 with the view aliasing of wakis/field.py:250-256): for every
+ * low face f (bit 0 = x, 1 = y, 2 = z) set in abc_low_mask, in x, y, z
+ * order, plane 0 := plane 1 for E and H, all three components. */
+int fit_apply_abc(fit_ctx *ctx, uint32_t abc_low_mask);
+
+/* Wake probe: record E_z on a (2h+1)x(2h+1)xnz window centred at (ixt, iyt)
+ * before every step (wakis/routines.py:438-448), into a device history of
+ * nt_cap columns (column = step counter).  h = 0 or 1.  nt_cap = 0 turns the
+ * probe off. */
+int fit_wake_probe(fit_ctx *ctx, int32_t ixt, int32_t iyt, int32_t h,
+                   int64_t nt_cap);
+/* Copy the probe history (window-major, then z, then time; ctx dtype). */
+int fit_wake_history(fit_ctx *ctx, void *host, int64_t count);
+
+/* Advance nsteps leapfrog steps (wakis/solverFIT3D.py:504-525):
+ *   H -= dt*M_H E;  E += dt*(M_E H - ieps*J);  J = sigma*E if conductive.
+ * conductive != 0 selects J := sigma*E after each step.  Asynchronous. */
+int fit_step(fit_ctx *ctx, int32_t nsteps, int32_t conductive);
+/* fit_step timed with HIP events on the context stream (synchronises).
+ * per_kernel = 0: ms[0] = elapsed time of the whole launch sequence.
+ * per_kernel = 1: an event pair around every launch; ms[1] = summed time of
+ * the first kernel of a step (H update, or the fused sweep), ms[2] = summed
+ * time of the second kernel (E update; 0 for the fused sweep); ms[0] = total.
+ * No counterpart in the reference (it has no timing facility beyond
+ * time.time() deltas, wakis/routines.py:437, 484). */
+int fit_step_timed(fit_ctx *ctx, int32_t nsteps, int32_t conductive, int32_t per_kernel,
+                   double *ms);
+/* Step counter used by the beam time and the probe column. */
+int fit_set_step_index(fit_ctx *ctx, int64_t step);
+int fit_get_step_index(fit_ctx *ctx, int64_t *step);
+/* Select the step kernel (FIT_KERNEL_*). */
+int fit_set_kernel(fit_ctx *ctx, int32_t kernel);
+int fit_sync(fit_ctx *ctx);
+
+/* Longitudinal wake potential on the device (wakis/wakeSolver.py:423-551,
+ * 311-421).  For every window w < nwin (taken in order from window_ids) and
+ * every sample n < ns:
+ *   acc = carry ? acc : 0;  for k < nzw: c = cols[k*ns + n];
+ *     if c >= 0: acc = acc + hist[w][k0 + k][c] * dz[k];
+ *   acc = acc / qnorm;  wp[w*ns + n] = acc.
+ * cols[nzw*ns] holds the time column of (n, k) (-1 = no contribution),
+ * computed by the host exactly as the reference's int((...)/dt). */
+int fit_wake_potential(fit_ctx *ctx, int32_t nwin, const int32_t *window_ids,
+                       int32_t carry, int32_t ns, int32_t nzw, int32_t k0,
+                       const int32_t *cols, const double *dz, double qnorm,
+                       double *wp);
+
+/* Impedance DFT (wakis/wakeSolver.py:676-703): bins j < K of the N-point
+ * zero-padded (or truncated) DFT of x[len] and of lam[len], both FP64,
+ * then Z_j = -(X_j*ds)/(Lam_j*ds).  Outputs re/im of Z, X*ds and Lam*ds.
+ * Runs on the context's device and stream. */
+int fit_impedance(fit_ctx *ctx, const double *x, const double *lam,
+                  int64_t len, int64_t N, int32_t K, double ds,
+                  double *z_re, double *z_im, double *x_re, double *x_im,
+                  double *l_re, double *l_im);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WAKIS_FIT_H */

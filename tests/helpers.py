@@ -1,0 +1,62 @@
+"""Shared builders: the same configuration as an oracle solver (CPU, the
+checker) and as a wakis_amd solver (HIP), with identical initial fields."""
+
+import numpy as np
+
+from oracle.fit_oracle import OBeam, OGrid, OPlaneWave, OSolver, OWake  # noqa: F401
+
+
+def axes(Nx, Ny, Nz, nonuniform=False, seed=0, ext=(0.1, 0.08, 0.2)):
+    if not nonuniform:
+        return (np.linspace(-ext[0] / 2, ext[0] / 2, Nx + 1),
+                np.linspace(-ext[1] / 2, ext[1] / 2, Ny + 1),
+                np.linspace(-ext[2] / 2, ext[2] / 2, Nz + 1))
+    rng = np.random.default_rng(seed)
+    out = []
+    for n, e in zip((Nx, Ny, Nz), ext):
+        d = rng.uniform(0.6, 1.4, n)
+        a = np.concatenate([[0.0], np.cumsum(d)])
+        out.append(a / a[-1] * e - e / 2)
+    return tuple(out)
+
+
+def block_mask(shape, lo, hi):
+    m = np.zeros(shape, bool)
+    m[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = True
+    return m
+
+
+def make_pair(Nx, Ny, Nz, bc_low, bc_high, dtype=np.float64, bg=(1.0, 1.0), solids=(),
+              n_pml=3, cfln=0.5, nonuniform=False, seed=0, init=True, wake_kw=None,
+              kernel="auto"):
+    import wakis_amd as wa
+
+    x, y, z = axes(Nx, Ny, Nz, nonuniform, seed)
+    owake = OWake(**wake_kw) if wake_kw else None
+    pwake = wa.WakeSolver(save=False, **wake_kw) if wake_kw else None
+    og = OGrid(x=x, y=y, z=z)
+    osv = OSolver(og, owake, cfln=cfln, bc_low=bc_low, bc_high=bc_high,
+                  solids=list(solids), bg=bg, dtype=dtype, n_pml=n_pml)
+    g = wa.GridFIT3D(x=x, y=y, z=z, solids=list(solids))
+    psv = wa.SolverFIT3D(g, pwake, cfln=cfln, bc_low=bc_low, bc_high=bc_high,
+                         use_stl=bool(solids), bg=bg, dtype=dtype, n_pml=n_pml, kernel=kernel)
+    if init:
+        rng = np.random.default_rng(seed + 100)
+        for name in ("E", "H", "J"):
+            a = rng.standard_normal(3 * osv.N).astype(dtype)
+            if name == "J":
+                a *= 1e-3
+            getattr(osv, name).fromarray(a)
+            getattr(psv, name).fromarray(a)
+    return osv, psv
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def compare_fields(osv, psv):
+    return {n: rel(getattr(psv, n).toarray(), getattr(osv, n).toarray()) for n in "EHJ"}

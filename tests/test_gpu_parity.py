@@ -1,0 +1,167 @@
+"""HIP step kernels vs the CPU oracle (FP64: relative L2 <= 1e-10; FP32:
+<= 1e-4 against the FP32 oracle), on small grids with every boundary kind,
+materials, non-uniform spacing, the flat-index wrap, J modes and the beam."""
+
+import numpy as np
+import pytest
+
+from helpers import OBeam, OPlaneWave, block_mask, compare_fields, make_pair, rel
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+TOL32 = 1e-4
+
+BC_CASES = [
+    ("pec", ["pec"] * 3, ["pec"] * 3),
+    ("periodic", ["periodic"] * 3, ["periodic"] * 3),
+    ("mixed", ["periodic", "pec", "pec"], ["periodic", "pmc", "pml"]),
+    ("pmc", ["pmc", "pec", "magnetic"], ["pec", "pmc", "pec"]),
+    ("abc", ["pec", "abc", "pec"], ["abc", "pec", "abc"]),
+    ("pml_z", ["pec", "pec", "pml"], ["pec", "pec", "pml"]),
+    ("open_x", ["pec", "periodic", "pec"], ["periodic", "periodic", "pec"]),
+]
+
+
+def _solids(shape):
+    Nx, Ny, Nz = shape
+    return [
+        (block_mask(shape, (1, 1, 2), (Nx - 1, Ny - 1, Nz - 2)), "vacuum"),
+        (block_mask(shape, (2, 1, 3), (4, 3, 6)), [4.0, 1.5]),
+        (block_mask(shape, (3, 2, 4), (5, 4, 7)), [2.0, 1.0, 50.0]),
+        (block_mask(shape, (1, 3, 5), (3, 5, 7)), "pec"),
+    ]
+
+
+@pytest.mark.parametrize("name,bl,bh", BC_CASES, ids=[c[0] for c in BC_CASES])
+@pytest.mark.parametrize("nonuniform", [False, True], ids=["uniform", "nonuniform"])
+def test_step_fp64(name, bl, bh, nonuniform):
+    shape = (7, 6, 11)
+    print(f"[case] step_fp64 {name} nonuniform={nonuniform} shape={shape}", flush=True)
+    osv, psv = make_pair(*shape, bl, bh, solids=_solids(shape), nonuniform=nonuniform,
+                         seed=3, n_pml=3, kernel="split")
+    for _ in range(25):
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    print(name, err)
+    assert max(err.values()) <= TOL64, err
+
+
+@pytest.mark.parametrize("name,bl,bh", BC_CASES[:4], ids=[c[0] for c in BC_CASES[:4]])
+def test_step_fp32(name, bl, bh):
+    shape = (8, 7, 10)
+    print(f"[case] step_fp32 {name} shape={shape}", flush=True)
+    osv, psv = make_pair(*shape, bl, bh, dtype=np.float32, solids=_solids(shape), seed=5,
+                         kernel="split")
+    for _ in range(25):
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    print(name, err)
+    assert max(err.values()) <= TOL32, err
+
+
+def test_conductive_background_and_update_tensors():
+    shape = (6, 7, 9)
+    print("[case] conductive bg + update_tensors", flush=True)
+    osv, psv = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, bg=[2.0, 1.0, 3.0], seed=9)
+    rng = np.random.default_rng(1)
+    # a few distinct values per component: the device table holds 256 rows
+    ieps = rng.choice([0.5, 0.7, 0.9, 1.0], 3 * osv.N) / 8.854e-12
+    for s in (osv, psv):
+        f = s.ieps
+        f.fromarray(ieps)
+        s.ieps = f
+        s.update_tensors()
+    for _ in range(15):
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    assert max(err.values()) <= TOL64, err
+
+
+def test_beam_user_loop_fp64():
+    """Beam.update + one_step in a user loop (wakis/sources.py:76-110)."""
+    import wakis_amd as wa
+
+    shape = (9, 9, 16)
+    print("[case] beam user loop", flush=True)
+    osv, psv = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, init=False,
+                         solids=[(block_mask(shape, (1, 1, 0), (8, 8, 16)), "vacuum")], bg="pec")
+    ob = OBeam(q=1e-9, sigmaz=0.01, beta=1.0)
+    pb = wa.Beam(q=1e-9, sigmaz=0.01, beta=1.0)
+    for n in range(40):
+        ob.update(osv, n * osv.dt)
+        pb.update(psv, n * psv.dt)
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    assert max(err.values()) <= TOL64, err
+
+
+def test_planewave_pml_fp32():
+    """PlaneWave source writing E/H planes each step into a PML-terminated
+    column (tests/test_005_pml_boundaries.py:19-137 shape, shorter)."""
+    import wakis_amd as wa
+
+    print("[case] planewave pml fp32", flush=True)
+    Nx, Ny, Nz = 6, 6, 60
+    osv, psv = make_pair(Nx, Ny, Nz, ["periodic", "periodic", "pec"],
+                         ["periodic", "periodic", "pml"], dtype=np.float32, bg=[1.0, 1.0, 0.0],
+                         n_pml=10, init=False)
+    f = 15 / ((osv.z.max() - osv.z.min()) / 299792458.0)
+    for s in (osv, psv):
+        s.dt = 1 / f / 200
+    ow = OPlaneWave(xs=slice(0, Nx), ys=slice(0, Ny), zs=0, amplitude=100.0, f=f, nodes=7,
+                    phase=np.pi / 2)
+    pw = wa.PlaneWave(xs=slice(0, Nx), ys=slice(0, Ny), zs=0, amplitude=100.0, f=f, nodes=7,
+                      phase=np.pi / 2)
+    for n in range(300):
+        ow.update(osv, n * osv.dt)
+        pw.update(psv, n * psv.dt)
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    assert max(err.values()) <= TOL32, err
+
+
+def test_emsolve_abc_fp64():
+    """emsolve with low-face ABC planes (CPU aliasing semantics)."""
+    print("[case] emsolve abc", flush=True)
+    shape = (7, 8, 12)
+    osv, psv = make_pair(*shape, ["abc", "pec", "abc"], ["pec", "abc", "pec"], seed=11)
+    osv.emsolve(20)
+    psv.emsolve(20)
+    err = compare_fields(osv, psv)
+    assert max(err.values()) <= TOL64, err
+
+
+def test_wrap_large_nx_fp64():
+    """A grid whose rows exceed one workgroup, periodic (helical wrap)."""
+    print("[case] wrap large nx", flush=True)
+    osv, psv = make_pair(300, 3, 4, ["periodic"] * 3, ["periodic"] * 3, seed=13)
+    for _ in range(10):
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    assert max(err.values()) <= TOL64, err
+
+
+def test_field_indexing_roundtrip():
+    """DeviceField tuple indexing agrees with the host Field."""
+    from wakis_amd.field import Field
+
+    print("[case] field indexing", flush=True)
+    osv, psv = make_pair(5, 6, 7, ["pec"] * 3, ["pec"] * 3, seed=2)
+    host = Field(5, 6, 7)
+    host.fromarray(psv.E.toarray())
+    keys = [(1, 2, 3, "x"), (slice(None), 2, slice(1, 5), "y"), (-1, slice(0, 3), 4, "z"),
+            (slice(1, 4), slice(2, 6), slice(0, 7), "z"), (np.int64(2), 3, slice(None), "x")]
+    for k in keys:
+        assert np.array_equal(np.asarray(psv.E[k]), np.asarray(host[k])), k
+    psv.E[1:3, :, 2, "y"] = 7.0
+    host[1:3, :, 2, "y"] = 7.0
+    psv.H[:, 0, :, "z"] = np.arange(35.0).reshape(5, 7)
+    assert np.array_equal(psv.E.toarray(), host.toarray())
+    assert np.array_equal(psv.H[:, 0, :, "z"], np.arange(35.0).reshape(5, 7))

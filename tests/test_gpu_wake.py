@@ -1,0 +1,145 @@
+"""Wake path on the device vs the CPU oracle: in-kernel bunch current, E_z
+probe, wake potential (bit-exact given the same history), impedance DFT,
+transverse wake, and the reference's test_001 golden vectors."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from helpers import OGrid, OSolver, OWake, block_mask, make_pair, rel
+from oracle.fit_oracle import _wp_accumulate
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _cavity(shape):
+    Nx, Ny, Nz = shape
+    cav = block_mask(shape, (2, 2, Nz // 3), (Nx - 2, Ny - 2, 2 * Nz // 3))
+    pipe = block_mask(shape, (Nx // 2 - 2, Ny // 2 - 2, 0), (Nx // 2 + 2, Ny // 2 + 2, Nz))
+    return [(cav | pipe, "vacuum")]
+
+
+WAKE_KW = dict(wakelength=0.15, q=1e-9, sigmaz=0.012, beta=1.0, skip_cells=2)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("plane", ["both", "longitudinal"])
+def test_wakesolve(dtype, plane):
+    shape = (14, 13, 36)
+    print(f"[case] wakesolve {np.dtype(dtype).name} {plane} shape={shape}", flush=True)
+    osv, psv = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, dtype=dtype, bg="pec",
+                         solids=_cavity(shape), init=False, cfln=1.0, wake_kw=WAKE_KW)
+    osv.wakesolve(compute_plane=plane)
+    psv.wakesolve(compute_plane=plane)
+    ow, pw = osv.wake, psv.wake
+    assert psv.Nt == osv.Nt and len(pw.s) == len(ow.s) and len(pw.Z) == len(ow.Z)
+    tol = 1e-10 if dtype == np.float64 else 2e-4
+    errs = {"E": rel(psv.E.toarray(), osv.E.toarray()), "H": rel(psv.H.toarray(), osv.H.toarray()),
+            "WP": rel(pw.WP, ow.WP), "Z": rel(pw.Z, ow.Z)}
+    if plane == "both":
+        errs["WP3d"] = rel(pw.WP_3d, ow.WP_3d)
+        errs["WPx"] = rel(pw.WPx, ow.WPx)
+        errs["WPy"] = rel(pw.WPy, ow.WPy)
+        errs["Zx"] = rel(pw.Zx, ow.Zx)
+    print(errs)
+    assert max(errs.values()) <= tol, errs
+    # J after the run holds the last beam line (non-conductive solver)
+    assert rel(psv.J.toarray(), osv.J.toarray()) <= tol
+
+
+def test_wake_potential_bit_exact():
+    """Given the same E_z history, the device WP equals the oracle's
+    accumulation bit for bit (same order of additions, no contraction)."""
+    shape = (12, 12, 30)
+    print("[case] WP bit-exact", flush=True)
+    osv, psv = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, bg="pec", solids=_cavity(shape),
+                         init=False, cfln=1.0, wake_kw=WAKE_KW)
+    psv.wakesolve(compute_plane="both")
+    pw = psv.wake
+    hist = psv.ctx.wake_history()  # (9, nz, nt)
+    # feed the device history to the oracle's restated WP loop
+    ow = OWake(**WAKE_KW)
+    hf = {"x": pw.xf, "y": pw.yf, "z": pw.zf, "dx": pw.dx, "dy": pw.dy, "dz": pw.dz, "t": pw.t}
+    for n in range(psv.Nt):
+        hf["#" + str(n).zfill(5)] = hist[:, :, n].reshape(3, 3, -1)
+    ow.Ez_hf = hf
+    ow.calc_long_WP_3d()
+    assert np.array_equal(ow.WP_3d, pw.WP_3d)
+
+
+def test_wakesolve_callback_path_matches_fast_path():
+    shape = (10, 10, 24)
+    print("[case] callback vs fast path", flush=True)
+    _, a = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, bg="pec", solids=_cavity(shape),
+                     init=False, cfln=1.0, wake_kw=WAKE_KW)
+    _, b = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, bg="pec", solids=_cavity(shape),
+                     init=False, cfln=1.0, wake_kw=WAKE_KW)
+    a.wakesolve(compute_plane="longitudinal")
+    seen = []
+    b.wakesolve(compute_plane="longitudinal", callback=lambda s, t: seen.append(t))
+    assert len(seen) == b.Nt
+    assert rel(b.wake.WP, a.wake.WP) <= 1e-12
+    assert rel(b.E.toarray(), a.E.toarray()) <= 1e-12
+
+
+def test_conductive_wakesolve_pml_fp64():
+    """Lossy walls + z PML (config-2 class, small): sigma*E J mode, PML
+    grading and the relaxation dt clamp."""
+    shape = (12, 12, 40)
+    print("[case] conductive pml wakesolve", flush=True)
+    Nx, Ny, Nz = shape
+    wall = ~block_mask(shape, (2, 2, 0), (Nx - 2, Ny - 2, Nz))
+    solids = [(wall, [1.0, 1.0, 1e3])]
+    osv, psv = make_pair(*shape, ["pec", "pec", "pml"], ["pec", "pec", "pml"], bg=[1.0, 1.0],
+                         solids=solids, init=False, cfln=0.9, n_pml=5, wake_kw=WAKE_KW)
+    assert float(psv.dt) == float(osv.dt)
+    osv.wakesolve(compute_plane="longitudinal")
+    psv.wakesolve(compute_plane="longitudinal")
+    errs = {"E": rel(psv.E.toarray(), osv.E.toarray()), "WP": rel(psv.wake.WP, osv.wake.WP),
+            "Z": rel(psv.wake.Z, osv.wake.Z), "J": rel(psv.J.toarray(), osv.J.toarray())}
+    print(errs)
+    assert max(errs.values()) <= 1e-10, errs
+
+
+def test_golden_001_fp32():
+    """tests/test_001_pec_cubic_cavity.py goldens on the device, FP32."""
+    import wakis_amd as wa
+
+    print("[case] golden 001 fp32", flush=True)
+    g = json.load(open(os.path.join(HERE, "golden", "goldens.json")))["test_001"]
+    stl = os.path.join(HERE, "golden", "001_cubic_cavity_box.json")
+    geo = json.load(open(stl))
+    lo, hi = geo["bounds_lo"], geo["bounds_hi"]
+    grid = wa.GridFIT3D(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], 50, 50, 150,
+                        solids=[(_cubic_cavity_mask(geo), "vacuum")])
+    wake = wa.WakeSolver(wakelength=1.0, q=1e-9, sigmaz=18.5e-3, beta=1.0, skip_cells=12,
+                         save=False)
+    solver = wa.SolverFIT3D(grid, wake, bc_low=["pec"] * 3, bc_high=["pec"] * 3, use_stl=True,
+                            bg="pec", dtype=np.float32)
+    solver.wakesolve(wakelength=1.0)
+    WP, Z = wake.WP, wake.Z
+    gz = np.array(g["Z_every20_re"]) + 1j * np.array(g["Z_every20_im"])
+    assert np.allclose(WP[::50], g["WP_every50"], rtol=g["tol_wp"], atol=g["tol_wp"])
+    assert np.cumsum(np.abs(WP))[-1] == pytest.approx(g["WP_abs_cumsum"], g["cumsum_rel"])
+    for part in (np.abs, np.real, np.imag):
+        assert np.allclose(part(Z)[::20], part(gz), rtol=g["tol_z"], atol=g["tol_z"])
+    assert np.cumsum(np.abs(Z))[-1] == pytest.approx(g["Z_abs_cumsum"], g["cumsum_rel"])
+
+
+def _cubic_cavity_mask(geo):
+    from wakis_amd.geometry import voxelize_nodes
+
+    x = np.linspace(geo["bounds_lo"][0], geo["bounds_hi"][0], 51)
+    y = np.linspace(geo["bounds_lo"][1], geo["bounds_hi"][1], 51)
+    z = np.linspace(geo["bounds_lo"][2], geo["bounds_hi"][2], 151)
+    tri = np.asarray(geo["triangles"], dtype=np.float64)
+    inside = voxelize_nodes(tri, x, y, z)
+    c = np.zeros((50, 50, 150), bool)
+    for a in (0, 1):
+        for b in (0, 1):
+            for d in (0, 1):
+                c |= inside[a:a + 50, b:b + 50, d:d + 150]
+    return c

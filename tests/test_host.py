@@ -1,0 +1,190 @@
+"""Host-side logic without a GPU: the C-ABI library exports every declared
+symbol, and the solver's compact setup (metric tables, face masks, material
+table, dt) reproduces the oracle's Fields and operators exactly."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import axes, block_mask
+from oracle.fit_oracle import OGrid, OSolver
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "wakis_fit.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(fit_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from wakis_amd import _lib
+
+    names = _declared_symbols()
+    assert len(names) >= 20
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES)
+
+
+def test_product_fails_loudly_without_device():
+    import wakis_amd as wa
+    from wakis_amd import _lib
+
+    g = wa.GridFIT3D(0, 1, 0, 1, 0, 1, 3, 3, 3)
+    s = wa.SolverFIT3D(g, bc_low=["pec"] * 3, bc_high=["pec"] * 3, device=None)
+    with pytest.raises(_lib.FitError):
+        s.one_step()
+
+
+BCS = [
+    (["pec"] * 3, ["pec"] * 3),
+    (["periodic"] * 3, ["periodic"] * 3),
+    (["periodic", "pec", "pec"], ["periodic", "pmc", "pml"]),
+    (["pmc", "pec", "magnetic"], ["pec", "PMC", "pec"]),
+    (["pec", "abc", "periodic"], ["abc", "pec", "periodic"]),
+    (["abc", "periodic", "pml"], ["abc", "periodic", "pml"]),
+]
+
+
+def _pair(bl, bh, dtype=np.float64, nonuniform=True, solids=(), bg=(1.0, 1.0), n_pml=2):
+    import wakis_amd as wa
+
+    x, y, z = axes(5, 4, 7, nonuniform=nonuniform, seed=7)
+    osv = OSolver(OGrid(x=x, y=y, z=z), bc_low=bl, bc_high=bh, dtype=dtype, solids=list(solids),
+                  bg=bg, n_pml=n_pml)
+    psv = wa.SolverFIT3D(wa.GridFIT3D(x=x, y=y, z=z, solids=list(solids)), bc_low=bl,
+                         bc_high=bh, dtype=dtype, use_stl=bool(solids), bg=bg, n_pml=n_pml,
+                         device=None)
+    return osv, psv
+
+
+def _expand(psv):
+    """Packed metric tables -> 3N arrays (L, tL, iA, itA) like the Fields."""
+    Nx, Ny, Nz = psv.Nx, psv.Ny, psv.Nz
+    p = psv.geometry
+    o = 0
+    one = []
+    for _ in range(2):
+        one.append((p[o:o + Nx], p[o + Nx:o + Nx + Ny], p[o + Nx + Ny:o + Nx + Ny + Nz]))
+        o += Nx + Ny + Nz
+    two = []
+    for _ in range(2):
+        ax = p[o:o + Ny * Nz].reshape((Ny, Nz), order="F")
+        o += Ny * Nz
+        ay = p[o:o + Nx * Nz].reshape((Nx, Nz), order="F")
+        o += Nx * Nz
+        az = p[o:o + Nx * Ny].reshape((Nx, Ny), order="F")
+        o += Nx * Ny
+        two.append((ax, ay, az))
+    assert o == p.size
+    sh = (Nx, Ny, Nz)
+
+    def f3(parts):
+        return np.concatenate([np.reshape(np.broadcast_to(a, sh), -1, order="F") for a in parts])
+
+    L = f3([one[0][0][:, None, None], one[0][1][None, :, None], one[0][2][None, None, :]])
+    tL = f3([one[1][0][:, None, None], one[1][1][None, :, None], one[1][2][None, None, :]])
+    iA = f3([two[0][0][None], two[0][1][:, None, :], two[0][2][:, :, None]])
+    itA = f3([two[1][0][None], two[1][1][:, None, :], two[1][2][:, :, None]])
+    return L, tL, iA, itA
+
+
+@pytest.mark.parametrize("bl,bh", BCS)
+def test_metric_tables_equal_oracle_fields(bl, bh):
+    osv, psv = _pair(bl, bh)
+    L, tL, iA, itA = _expand(psv)
+    assert np.array_equal(L, osv.L.toarray())
+    assert np.array_equal(tL, osv.tL.toarray())
+    assert np.array_equal(iA, osv.iA.toarray())
+    assert np.array_equal(itA, osv.itA.toarray())
+
+
+@pytest.mark.parametrize("bl,bh", BCS)
+def test_face_masks_reproduce_the_operator(bl, bh):
+    """Rebuild C from the face bits and compare with the oracle's C after
+    _apply_bc_to_C (column PEC mask, row PMC mask with carried flags)."""
+    from scipy.sparse import diags
+
+    osv, psv = _pair(bl, bh)
+    Nx, Ny, Nz = psv.Nx, psv.Ny, psv.Nz
+
+    def mask(bits):
+        # the kernel's rule: faces in order xlo, xhi, ylo, yhi, zlo, zhi, the
+        # last face containing the cell decides (masked iff its bit is set)
+        m = np.ones((3, Nx, Ny, Nz), dtype=np.int8)
+        for c in range(3):
+            if c != 0:
+                m[c, 0] = 0 if bits & 1 else 1
+                m[c, -1] = 0 if bits & 2 else 1
+            if c != 1:
+                m[c, :, 0] = 0 if bits & 4 else 1
+                m[c, :, -1] = 0 if bits & 8 else 1
+            if c != 2:
+                m[c, :, :, 0] = 0 if bits & 16 else 1
+                m[c, :, :, -1] = 0 if bits & 32 else 1
+        return np.concatenate([np.reshape(m[c], -1, order="F") for c in range(3)])
+
+    g = OGrid(x=psv.grid.x, y=psv.grid.y, z=psv.grid.z)
+    raw = OSolver(g, bc_low=["periodic"] * 3, bc_high=["periodic"] * 3)
+    C = raw.C
+    if psv.pec_bits:
+        C = C @ diags(mask(psv.pec_bits))
+    if psv.pmc_bits:
+        C = diags(mask(psv.pmc_bits)) @ C
+    assert abs(C - osv.C).sum() == 0
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_materials_and_dt_equal_oracle(dtype):
+    shape = (5, 4, 7)
+    solids = [
+        (block_mask(shape, (0, 0, 1), (5, 4, 6)), "vacuum"),
+        (block_mask(shape, (1, 1, 2), (3, 3, 4)), [4.0, 1.5]),
+        (block_mask(shape, (2, 1, 3), (4, 4, 5)), [2.0, 1.0, 50.0]),
+        (block_mask(shape, (0, 2, 3), (2, 4, 6)), "pec"),
+        (block_mask(shape, (3, 0, 0), (5, 2, 3)), [3.0, 2.0]),
+    ]
+    for bl, bh, bg in [(["pec"] * 3, ["pec", "pec", "pml"], "pec"),
+                       (["pml"] * 3, ["pml"] * 3, [1.0, 1.0, 2.0]),
+                       (["pec"] * 3, ["pec"] * 3, [2.0, 1.0])]:
+        osv, psv = _pair(bl, bh, dtype=dtype, solids=solids, bg=bg, n_pml=2)
+        for name in ("ieps", "imu", "sigma"):
+            assert np.array_equal(getattr(psv, name).toarray(),
+                                  getattr(osv, name).toarray().astype(np.float64)), name
+        assert type(psv.dt) is type(osv.dt) and psv.dt == osv.dt
+        assert psv.use_conductivity == osv.use_conductivity
+
+
+def test_empty_conductive_background_raises_like_reference():
+    """bg with sigma=0 and no PML: the reference's relaxation clamp takes the
+    min of an empty array (wakis/solverFIT3D.py:309-318)."""
+    with pytest.raises(ValueError):
+        _pair(["pec"] * 3, ["pec"] * 3, bg=[1.0, 1.0, 0.0])
+    with pytest.raises(ValueError):
+        OSolver(OGrid(0, 1, 0, 1, 0, 1, 3, 3, 3), bc_low=["pec"] * 3, bc_high=["pec"] * 3,
+                bg=[1.0, 1.0, 0.0])
+
+
+def test_bucket_columns_match_reference_expression():
+    """The host bucket table is int(ts/dt) of wakeSolver.py:517-529."""
+    import wakis_amd as wa
+
+    w = wa.WakeSolver(wakelength=0.05, sigmaz=0.005, save=False, skip_cells=1)
+    z = np.linspace(-0.04, 0.04, 17)
+    zc = (z[1:] + z[:-1]) / 2
+    dt = np.float32(2.7e-12)
+    nt = 300
+    w.attach_device_history(None, {"x": np.arange(3.0), "y": np.arange(3.0), "z": zc,
+                                   "dx": 0, "dy": 0, "dz": np.diff(z),
+                                   "t": np.arange(0, nt * dt, dt), "nsteps": nt, "half": 1})
+    cols, s, zs = w.bucket_columns(nt)
+    for n in range(0, len(s), 7):
+        for k in range(len(zs)):
+            ts = (zs[k] + s[n]) / w.v - zc.min() / w.v - w.t[0] + w.ti
+            it = int(ts / (w.t[2] - w.t[1]))
+            assert cols[k, n] == (it if it < nt else -1)

@@ -1,0 +1,303 @@
+"""ctypes binding of the HIP FIT library (``libwakis_fit.so``).
+
+The C ABI is declared in ``include/wakis_fit.h``.  There is no fallback: if
+the library is missing or a call fails, a ``FitError`` is raised.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwakis_fit.so")
+
+FIT_F64, FIT_F32 = 0, 1
+FIELD_E, FIELD_H, FIELD_J = 0, 1, 2
+J_ZERO, J_SIGMA_E, J_BUFFER = 0, 1, 2
+BEAM_OFF, BEAM_VALUES, BEAM_GAUSS = 0, 1, 2
+KERNEL_AUTO, KERNEL_SPLIT, KERNEL_FUSED = 0, 1, 2
+
+
+class FitError(RuntimeError):
+    """A call into the HIP FIT library failed."""
+
+
+class fit_desc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("dtype", C.c_int32),
+        ("device", C.c_int32),
+        ("reserved", C.c_int32 * 3),
+    ]
+
+
+_P = C.c_void_p
+_I32 = C.c_int32
+_I64 = C.c_int64
+_U32 = C.c_uint32
+_D = C.c_double
+_PD = C.POINTER(C.c_double)
+_PI32 = C.POINTER(C.c_int32)
+
+# name -> argtypes (restype is int unless listed in _STR)
+SIGNATURES = {
+    "fit_last_error": [],
+    "fit_version": [],
+    "fit_create": [C.POINTER(fit_desc), C.POINTER(_P)],
+    "fit_destroy": [_P],
+    "fit_set_geometry": [_P, _PD, _I64],
+    "fit_set_materials": [_P, _P, _I32, _PD],
+    "fit_set_bc": [_P, _U32, _U32],
+    "fit_set_dt": [_P, _D],
+    "fit_set_jmode": [_P, _I32],
+    "fit_set_beam": [_P, _I32, _I32, _I32, _PD, _PD, _PD],
+    "fit_field_read": [_P, _I32, _I32, _PI32, _P],
+    "fit_field_write": [_P, _I32, _I32, _PI32, _P],
+    "fit_field_zero": [_P, _I32],
+    "fit_materialize_j": [_P],
+    "fit_get_state": [_P, _PI32, _PI32],
+    "fit_zero_ghosts": [_P],
+    "fit_apply_abc": [_P, _U32],
+    "fit_wake_probe": [_P, _I32, _I32, _I32, _I64],
+    "fit_wake_history": [_P, _P, _I64],
+    "fit_step": [_P, _I32, _I32],
+    "fit_step_timed": [_P, _I32, _I32, _I32, _PD],
+    "fit_set_step_index": [_P, _I64],
+    "fit_get_step_index": [_P, C.POINTER(_I64)],
+    "fit_set_kernel": [_P, _I32],
+    "fit_sync": [_P],
+    "fit_wake_potential": [_P, _I32, _PI32, _I32, _I32, _I32, _I32, _PI32, _PD, _D, _PD],
+    "fit_impedance": [_P, _PD, _PD, _I64, _I64, _I32, _D, _PD, _PD, _PD, _PD, _PD, _PD],
+}
+_STR = {"fit_last_error", "fit_version"}
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load (once) and return the ctypes library; raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise FitError(
+            f"HIP FIT library not built: {path} is missing "
+            "(run __graft_entry__.build() or `make -C wakis_amd/csrc`)"
+        )
+    lib = C.CDLL(path)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name in _STR else C.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = load().fit_last_error().decode(errors="replace")
+        raise FitError(f"{what} failed ({rc}): {msg}")
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_PD)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_PI32)
+
+
+def vptr(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return C.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """Owns one device context (fields, materials, metrics) of one solve."""
+
+    def __init__(self, nx: int, ny: int, nz: int, dtype=np.float64, device: int = 0):
+        lib = load()
+        self.lib = lib
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.N = self.nx * self.ny * self.nz
+        self.np_dtype = np.dtype(dtype)
+        if self.np_dtype == np.float64:
+            code = FIT_F64
+        elif self.np_dtype == np.float32:
+            code = FIT_F32
+        else:
+            raise ValueError(f"unsupported dtype {dtype}: float64 or float32")
+        d = fit_desc(self.nx, self.ny, self.nz, code, int(device))
+        h = C.c_void_p()
+        check(lib.fit_create(C.byref(d), C.byref(h)), "fit_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fit_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- setup -------------------------------------------------------------
+    def set_geometry(self, packed: np.ndarray):
+        packed = np.ascontiguousarray(packed, dtype=np.float64)
+        check(self.lib.fit_set_geometry(self.h, dptr(packed), packed.size), "fit_set_geometry")
+
+    def set_materials(self, ids, table: np.ndarray):
+        table = np.ascontiguousarray(table, dtype=np.float64)
+        nmat = table.shape[0]
+        if ids is None:
+            p = None
+        else:
+            ids = np.ascontiguousarray(ids, dtype=np.uint8)
+            assert ids.size == self.N
+            p = C.c_void_p(ids.ctypes.data)
+        check(self.lib.fit_set_materials(self.h, p, nmat, dptr(table)), "fit_set_materials")
+
+    def set_bc(self, pec: int, pmc: int):
+        check(self.lib.fit_set_bc(self.h, pec, pmc), "fit_set_bc")
+
+    def set_dt(self, dt: float):
+        check(self.lib.fit_set_dt(self.h, float(dt)), "fit_set_dt")
+
+    def set_jmode(self, mode: int):
+        check(self.lib.fit_set_jmode(self.h, mode), "fit_set_jmode")
+
+    def get_state(self):
+        j, b = C.c_int32(), C.c_int32()
+        check(self.lib.fit_get_state(self.h, C.byref(j), C.byref(b)), "fit_get_state")
+        return j.value, b.value
+
+    def set_beam_values(self, ixs: int, iys: int, values: np.ndarray):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        assert v.size == self.nz
+        check(self.lib.fit_set_beam(self.h, BEAM_VALUES, ixs, iys, dptr(v), None, None),
+              "fit_set_beam")
+
+    def set_beam_gauss(self, ixs: int, iys: int, params, zc: np.ndarray):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        z = np.ascontiguousarray(zc, dtype=np.float64)
+        assert p.size == 8 and z.size == self.nz
+        check(self.lib.fit_set_beam(self.h, BEAM_GAUSS, ixs, iys, None, dptr(p), dptr(z)),
+              "fit_set_beam")
+
+    def beam_off(self):
+        check(self.lib.fit_set_beam(self.h, BEAM_OFF, 0, 0, None, None, None), "fit_set_beam")
+
+    # -- field transfer ----------------------------------------------------
+    def read_box(self, which: int, comp: int, box) -> np.ndarray:
+        b = np.ascontiguousarray(box, dtype=np.int32)
+        shape = (b[1] - b[0], b[3] - b[2], b[5] - b[4])
+        out = np.empty(int(np.prod(shape)), dtype=self.np_dtype)
+        check(self.lib.fit_field_read(self.h, which, comp, iptr(b), vptr(out)), "fit_field_read")
+        return out.reshape(shape, order="F")
+
+    def write_box(self, which: int, comp: int, box, data: np.ndarray):
+        b = np.ascontiguousarray(box, dtype=np.int32)
+        shape = (int(b[1] - b[0]), int(b[3] - b[2]), int(b[5] - b[4]))
+        d = np.asarray(data, dtype=self.np_dtype)
+        d = np.broadcast_to(d, shape)
+        flat = np.ascontiguousarray(np.reshape(d, -1, order="F"))
+        check(self.lib.fit_field_write(self.h, which, comp, iptr(b), vptr(flat)), "fit_field_write")
+
+    def read_field(self, which: int) -> np.ndarray:
+        out = np.empty(3 * self.N, dtype=self.np_dtype)
+        full = [0, self.nx, 0, self.ny, 0, self.nz]
+        for c in range(3):
+            out[c * self.N:(c + 1) * self.N] = np.reshape(
+                self.read_box(which, c, full), -1, order="F")
+        return out
+
+    def write_field(self, which: int, array: np.ndarray):
+        a = np.asarray(array, dtype=self.np_dtype)
+        assert a.size == 3 * self.N
+        full = [0, self.nx, 0, self.ny, 0, self.nz]
+        for c in range(3):
+            self.write_box(which, c, full, np.reshape(a[c * self.N:(c + 1) * self.N],
+                                                       (self.nx, self.ny, self.nz), order="F"))
+
+    def zero_field(self, which: int):
+        check(self.lib.fit_field_zero(self.h, which), "fit_field_zero")
+
+    def materialize_j(self):
+        check(self.lib.fit_materialize_j(self.h), "fit_materialize_j")
+
+    # -- stepping ----------------------------------------------------------
+    def zero_ghosts(self):
+        check(self.lib.fit_zero_ghosts(self.h), "fit_zero_ghosts")
+
+    def apply_abc(self, mask: int):
+        check(self.lib.fit_apply_abc(self.h, mask), "fit_apply_abc")
+
+    def step(self, nsteps: int = 1, conductive: bool = False):
+        check(self.lib.fit_step(self.h, int(nsteps), int(bool(conductive))), "fit_step")
+
+    def step_timed(self, nsteps: int, conductive: bool, per_kernel: bool):
+        """(total_ms, first_kernel_ms, second_kernel_ms) over nsteps steps."""
+        out = np.zeros(3, dtype=np.float64)
+        check(self.lib.fit_step_timed(self.h, int(nsteps), int(bool(conductive)),
+                                      int(bool(per_kernel)), dptr(out)), "fit_step_timed")
+        return float(out[0]), float(out[1]), float(out[2])
+
+    def set_step_index(self, n: int):
+        check(self.lib.fit_set_step_index(self.h, int(n)), "fit_set_step_index")
+
+    def get_step_index(self) -> int:
+        v = C.c_int64()
+        check(self.lib.fit_get_step_index(self.h, C.byref(v)), "fit_get_step_index")
+        return v.value
+
+    def set_kernel(self, k: int):
+        check(self.lib.fit_set_kernel(self.h, k), "fit_set_kernel")
+
+    def sync(self):
+        check(self.lib.fit_sync(self.h), "fit_sync")
+
+    # -- wake ----------------------------------------------------------------
+    def wake_probe(self, ixt: int, iyt: int, h: int, nt_cap: int):
+        check(self.lib.fit_wake_probe(self.h, ixt, iyt, h, int(nt_cap)), "fit_wake_probe")
+        self.probe = (ixt, iyt, h, int(nt_cap))
+
+    def wake_history(self) -> np.ndarray:
+        ixt, iyt, h, nt = self.probe
+        nw = (2 * h + 1) ** 2
+        out = np.empty(nw * self.nz * nt, dtype=self.np_dtype)
+        check(self.lib.fit_wake_history(self.h, vptr(out), out.size), "fit_wake_history")
+        return out.reshape(nw, self.nz, nt)
+
+    def wake_potential(self, window_ids, carry: bool, cols: np.ndarray, k0: int,
+                       dz: np.ndarray, qnorm: float) -> np.ndarray:
+        w = np.ascontiguousarray(window_ids, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        nzw, ns = cols.shape
+        dz = np.ascontiguousarray(dz[:nzw], dtype=np.float64)
+        out = np.empty(len(w) * ns, dtype=np.float64)
+        check(self.lib.fit_wake_potential(self.h, len(w), iptr(w), int(bool(carry)), ns, nzw,
+                                          int(k0), iptr(cols), dptr(dz), float(qnorm),
+                                          dptr(out)), "fit_wake_potential")
+        return out.reshape(len(w), ns)
+
+    def impedance(self, x: np.ndarray, lam: np.ndarray, N: int, K: int, ds: float):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        assert x.size == lam.size
+        bufs = [np.empty(K, dtype=np.float64) for _ in range(6)]
+        check(self.lib.fit_impedance(self.h, dptr(x), dptr(lam), x.size, int(N), int(K),
+                                     float(ds), *[dptr(b) for b in bufs]), "fit_impedance")
+        z = bufs[0] + 1j * bufs[1]
+        X = bufs[2] + 1j * bufs[3]
+        L = bufs[4] + 1j * bufs[5]
+        return z, X, L

@@ -1,0 +1,727 @@
+// C ABI of the FIT time-stepping library (declarations: include/wakis_fit.h).
+//
+// One context = one solve on one HIP device with one HIP stream (default
+// priority).  The context owns E, H, J (3N each), the per-cell material ids,
+// the material table, the metric tables and the wake-probe history.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/wakis_fit.h"
+#include "fit_aux.hpp"
+#include "fit_fused.hpp"
+#include "fit_split.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(FIT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+}  // namespace
+
+struct fit_ctx {
+  int nx = 0, ny = 0, nz = 0;
+  int64_t N = 0;
+  int dtype = FIT_F64;
+  size_t esz = 8;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void *E = nullptr, *H = nullptr, *J = nullptr;
+  uint8_t *mat = nullptr;
+  bool mat_uniform = true;
+  void *mtab = nullptr;
+  int nmat = 0;
+  std::vector<double> mtab_host;  // nmat * 9 FP64 (1/eps, 1/mu, sigma)
+  void *geo = nullptr;
+  int64_t geo_count = 0;
+  bool geo_set = false, mat_set = false, dt_set = false;
+  uint32_t pec = 0, pmc = 0;
+  double dt = 0;
+  int jmode = FIT_J_ZERO;
+  int beam_mode = FIT_BEAM_OFF, ixs = 0, iys = 0;
+  void *jline = nullptr;
+  double *zc = nullptr;
+  double bp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t step = 0;
+  void *hist = nullptr;
+  int64_t hist_nt = 0;
+  int ixt = 0, iyt = 0, hw = 0;
+  void *staging = nullptr;
+  size_t staging_bytes = 0;
+  int kernel = FIT_KERNEL_AUTO;
+};
+
+namespace {
+
+int set_device(fit_ctx *c) {
+  HIPCHK(hipSetDevice(c->device));
+  return 0;
+}
+
+int ensure_staging(fit_ctx *c, size_t bytes) {
+  if (c->staging_bytes >= bytes) return 0;
+  if (c->staging) HIPCHK(hipFree(c->staging));
+  c->staging = nullptr;
+  c->staging_bytes = 0;
+  if (hipMalloc(&c->staging, bytes) != hipSuccess)
+    return fail(FIT_ENOMEM, "staging allocation failed");
+  c->staging_bytes = bytes;
+  return 0;
+}
+
+// Material table in the compute type: dt/mu, dt/eps, sigma.
+int upload_mtab(fit_ctx *c) {
+  if (!c->mat_set) return 0;
+  const int nm = c->nmat;
+  std::vector<double> t(nm * fit::MAT_STRIDE);
+  for (int m = 0; m < nm; ++m) {
+    const double *r = &c->mtab_host[m * 9];
+    for (int d = 0; d < 3; ++d) {
+      t[m * 9 + d] = c->dt * r[3 + d];      // dt/mu
+      t[m * 9 + 3 + d] = c->dt * r[d];      // dt/eps
+      t[m * 9 + 6 + d] = r[6 + d];          // sigma
+    }
+  }
+  if (c->mtab) HIPCHK(hipFree(c->mtab));
+  c->mtab = nullptr;
+  if (hipMalloc(&c->mtab, t.size() * c->esz) != hipSuccess)
+    return fail(FIT_ENOMEM, "material table allocation failed");
+  if (c->dtype == FIT_F64) {
+    HIPCHK(hipMemcpy(c->mtab, t.data(), t.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(t.begin(), t.end());
+    HIPCHK(hipMemcpy(c->mtab, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+template <typename T>
+fit::StepArgs<T> make_args(fit_ctx *c) {
+  fit::StepArgs<T> a{};
+  a.E = (T *)c->E;
+  a.H = (T *)c->H;
+  a.J = (const T *)c->J;
+  a.mat = c->mat_uniform ? nullptr : c->mat;
+  a.mtab = (const T *)c->mtab;
+  const T *g = (const T *)c->geo;
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  a.g.Lx = g;
+  a.g.Ly = g + nx;
+  a.g.Lz = g + nx + ny;
+  const T *t = g + nx + ny + nz;
+  a.g.tLx = t;
+  a.g.tLy = t + nx;
+  a.g.tLz = t + nx + ny;
+  const T *ia = t + nx + ny + nz;
+  a.g.iAx = ia;
+  a.g.iAy = ia + (int64_t)ny * nz;
+  a.g.iAz = ia + (int64_t)ny * nz + (int64_t)nx * nz;
+  const T *ita = ia + (int64_t)ny * nz + (int64_t)nx * nz + (int64_t)nx * ny;
+  a.g.itAx = ita;
+  a.g.itAy = ita + (int64_t)ny * nz;
+  a.g.itAz = ita + (int64_t)ny * nz + (int64_t)nx * nz;
+  a.nx = nx;
+  a.ny = ny;
+  a.nz = nz;
+  a.N = c->N;
+  a.pec = c->pec;
+  a.pmc = c->pmc;
+  a.jmode = c->jmode;
+  a.beam_mode = c->beam_mode;
+  a.ixs = c->ixs;
+  a.iys = c->iys;
+  a.jline = (const T *)c->jline;
+  a.zc = c->zc;
+  a.b_qv = c->bp[0];
+  a.b_A = c->bp[1];
+  a.b_B = c->bp[2];
+  a.b_s0 = c->bp[3];
+  a.b_v = c->bp[4];
+  a.b_dx = c->bp[5];
+  a.b_dy = c->bp[6];
+  a.b_f32time = c->bp[7] != 0.0;
+  a.dt64 = c->dt;
+  a.dt32 = (float)c->dt;
+  a.step = c->step;
+  a.hist = (T *)c->hist;
+  a.hist_nt = c->hist_nt;
+  a.ixt = c->ixt;
+  a.iyt = c->iyt;
+  a.hw = c->hw;
+  return a;
+}
+
+inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ev (optional): 3 events per step [before first kernel, between, after].
+template <typename T>
+int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) {
+  const bool fused = fit::fused_supported(c->kernel, c->nx, c->ny, c->nz);
+  for (int s = 0; s < nsteps; ++s) {
+    fit::StepArgs<T> a = make_args<T>(c);
+    if (ev) HIPCHK(hipEventRecord(ev[3 * s], c->stream));
+    if (fused) {
+      fit::launch_fused<T>(a, c->stream);
+      if (ev) HIPCHK(hipEventRecord(ev[3 * s + 1], c->stream));
+    } else {
+      hipLaunchKernelGGL(fit::h_update_split<T>, dim3(nblocks(c->N, 256)), dim3(256), 0,
+                         c->stream, a);
+      if (ev) HIPCHK(hipEventRecord(ev[3 * s + 1], c->stream));
+      hipLaunchKernelGGL(fit::e_update_split<T>, dim3(nblocks(c->N, 256)), dim3(256), 0,
+                         c->stream, a);
+    }
+    if (ev) HIPCHK(hipEventRecord(ev[3 * s + 2], c->stream));
+    HIPCHK(hipGetLastError());
+    c->step += 1;
+    if (conductive) {
+      c->jmode = FIT_J_SIGMA_E;
+      if (c->beam_mode == FIT_BEAM_VALUES) c->beam_mode = FIT_BEAM_OFF;
+    }
+  }
+  return 0;
+}
+
+int check_ready(fit_ctx *c) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (!c->geo_set) return fail(FIT_ESTATE, "geometry not set");
+  if (!c->mat_set) return fail(FIT_ESTATE, "materials not set");
+  if (!c->dt_set) return fail(FIT_ESTATE, "dt not set");
+  return 0;
+}
+
+void *field_ptr(fit_ctx *c, int which) {
+  return which == FIT_FIELD_E ? c->E : (which == FIT_FIELD_H ? c->H : c->J);
+}
+
+int check_box(fit_ctx *c, int which, int comp, const int32_t *b) {
+  if (which < 0 || which > 2) return fail(FIT_EINVAL, "bad field id");
+  if (comp < 0 || comp > 2) return fail(FIT_EINVAL, "bad component");
+  if (!b) return fail(FIT_EINVAL, "null box");
+  if (b[0] < 0 || b[1] > c->nx || b[0] >= b[1] || b[2] < 0 || b[3] > c->ny ||
+      b[2] >= b[3] || b[4] < 0 || b[5] > c->nz || b[4] >= b[5])
+    return fail(FIT_EINVAL, "box out of range or empty");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fit_last_error(void) { return g_err.c_str(); }
+const char *fit_version(void) { return "wakis_amd-fit 0.1 gfx950"; }
+
+int fit_create(const fit_desc *d, fit_ctx **out) {
+  if (!d || !out) return fail(FIT_EINVAL, "null argument");
+  *out = nullptr;
+  if (d->nx < 2 || d->ny < 2 || d->nz < 2) return fail(FIT_EINVAL, "need nx, ny, nz >= 2");
+  if (d->dtype != FIT_F64 && d->dtype != FIT_F32) return fail(FIT_EINVAL, "bad dtype");
+  const int64_t N = (int64_t)d->nx * d->ny * d->nz;
+  if (N > (int64_t)1 << 31) return fail(FIT_EINVAL, "grid larger than 2^31 cells");
+  fit_ctx *c = new fit_ctx();
+  c->nx = d->nx;
+  c->ny = d->ny;
+  c->nz = d->nz;
+  c->N = N;
+  c->dtype = d->dtype;
+  c->esz = d->dtype == FIT_F64 ? 8 : 4;
+  c->device = d->device;
+  int rc = set_device(c);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(FIT_EHIP, "stream creation failed");
+  }
+  const size_t fb = (size_t)3 * N * c->esz;
+  if (hipMalloc(&c->E, fb) != hipSuccess || hipMalloc(&c->H, fb) != hipSuccess ||
+      hipMalloc(&c->J, fb) != hipSuccess || hipMalloc((void **)&c->mat, (size_t)N) != hipSuccess ||
+      hipMalloc(&c->jline, (size_t)c->nz * c->esz) != hipSuccess ||
+      hipMalloc((void **)&c->zc, (size_t)c->nz * 8) != hipSuccess) {
+    fit_destroy(c);
+    return fail(FIT_ENOMEM, "field allocation failed (" + std::to_string(3 * fb) + " bytes)");
+  }
+  if (hipMemsetAsync(c->E, 0, fb, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->H, 0, fb, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->J, 0, fb, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->mat, 0, (size_t)N, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    fit_destroy(c);
+    return fail(FIT_EHIP, "field initialisation failed");
+  }
+  *out = c;
+  return 0;
+}
+
+int fit_destroy(fit_ctx *c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void *ps[] = {c->E, c->H, c->J, c->mat, c->mtab, c->geo, c->jline, c->zc, c->hist, c->staging};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int fit_set_geometry(fit_ctx *c, const double *packed, int64_t count) {
+  if (!c || !packed) return fail(FIT_EINVAL, "null argument");
+  const int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+  const int64_t want = 2 * (nx + ny + nz) + 2 * (ny * nz + nx * nz + nx * ny);
+  if (count != want)
+    return fail(FIT_EINVAL, "geometry count " + std::to_string(count) + " != " + std::to_string(want));
+  if (int rc = set_device(c)) return rc;
+  if (c->geo) HIPCHK(hipFree(c->geo));
+  c->geo = nullptr;
+  if (hipMalloc(&c->geo, (size_t)count * c->esz) != hipSuccess)
+    return fail(FIT_ENOMEM, "geometry allocation failed");
+  if (c->dtype == FIT_F64) {
+    HIPCHK(hipMemcpy(c->geo, packed, (size_t)count * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(packed, packed + count);
+    HIPCHK(hipMemcpy(c->geo, f.data(), (size_t)count * 4, hipMemcpyHostToDevice));
+  }
+  c->geo_count = count;
+  c->geo_set = true;
+  return 0;
+}
+
+int fit_set_materials(fit_ctx *c, const uint8_t *ids, int32_t nmat, const double *table) {
+  if (!c || !table) return fail(FIT_EINVAL, "null argument");
+  if (nmat < 1 || nmat > 256) return fail(FIT_EINVAL, "nmat must be in [1, 256]");
+  if (int rc = set_device(c)) return rc;
+  c->mtab_host.assign(table, table + (size_t)nmat * 9);
+  c->nmat = nmat;
+  if (ids) {
+    for (int64_t n = 0; n < c->N; ++n)
+      if (ids[n] >= nmat) return fail(FIT_EINVAL, "material id out of range");
+    HIPCHK(hipMemcpy(c->mat, ids, (size_t)c->N, hipMemcpyHostToDevice));
+    c->mat_uniform = false;
+  } else {
+    HIPCHK(hipMemset(c->mat, 0, (size_t)c->N));
+    c->mat_uniform = true;
+  }
+  c->mat_set = true;
+  return upload_mtab(c);
+}
+
+int fit_set_bc(fit_ctx *c, uint32_t pec_cols, uint32_t pmc_rows) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if ((pec_cols | pmc_rows) & ~63u) return fail(FIT_EINVAL, "face mask uses bits above 5");
+  c->pec = pec_cols;
+  c->pmc = pmc_rows;
+  return 0;
+}
+
+int fit_set_dt(fit_ctx *c, double dt) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (!(dt > 0)) return fail(FIT_EINVAL, "dt must be > 0");
+  if (int rc = set_device(c)) return rc;
+  c->dt = dt;
+  c->dt_set = true;
+  return upload_mtab(c);
+}
+
+int fit_set_jmode(fit_ctx *c, int32_t jmode) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (jmode < FIT_J_ZERO || jmode > FIT_J_BUFFER) return fail(FIT_EINVAL, "bad J mode");
+  c->jmode = jmode;
+  return 0;
+}
+
+int fit_get_state(fit_ctx *c, int32_t *jmode, int32_t *beam_mode) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (jmode) *jmode = c->jmode;
+  if (beam_mode) *beam_mode = c->beam_mode;
+  return 0;
+}
+
+int fit_set_beam(fit_ctx *c, int32_t mode, int32_t ixs, int32_t iys, const double *values,
+                 const double *params, const double *zc) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (mode == FIT_BEAM_OFF) {
+    c->beam_mode = mode;
+    return 0;
+  }
+  if (ixs < 0 || ixs >= c->nx || iys < 0 || iys >= c->ny)
+    return fail(FIT_EINVAL, "beam line outside the grid");
+  if (int rc = set_device(c)) return rc;
+  if (mode == FIT_BEAM_VALUES) {
+    if (!values) return fail(FIT_EINVAL, "null beam values");
+    if (c->dtype == FIT_F64) {
+      HIPCHK(hipMemcpyAsync(c->jline, values, (size_t)c->nz * 8, hipMemcpyHostToDevice, c->stream));
+    } else {
+      std::vector<float> f(values, values + c->nz);
+      HIPCHK(hipMemcpyAsync(c->jline, f.data(), (size_t)c->nz * 4, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+  } else if (mode == FIT_BEAM_GAUSS) {
+    if (!params || !zc) return fail(FIT_EINVAL, "null beam parameters");
+    std::memcpy(c->bp, params, sizeof(c->bp));
+    HIPCHK(hipMemcpyAsync(c->zc, zc, (size_t)c->nz * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    return fail(FIT_EINVAL, "bad beam mode");
+  }
+  c->beam_mode = mode;
+  c->ixs = ixs;
+  c->iys = iys;
+  return 0;
+}
+
+int fit_field_read(fit_ctx *c, int32_t which, int32_t comp, const int32_t box[6], void *host) {
+  if (!c || !host) return fail(FIT_EINVAL, "null argument");
+  if (int rc = check_box(c, which, comp, box)) return rc;
+  if (int rc = set_device(c)) return rc;
+  const int bi = box[1] - box[0], bj = box[3] - box[2], bk = box[5] - box[4];
+  const int64_t cnt = (int64_t)bi * bj * bk;
+  char *src = (char *)field_ptr(c, which) + (size_t)comp * c->N * c->esz;
+  if (bi == c->nx && bj == c->ny) {
+    // whole planes: contiguous
+    HIPCHK(hipMemcpyAsync(host, src + (size_t)box[4] * c->nx * c->ny * c->esz, (size_t)cnt * c->esz,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+  }
+  if (int rc = ensure_staging(c, (size_t)cnt * c->esz)) return rc;
+  if (c->dtype == FIT_F64)
+    hipLaunchKernelGGL(fit::box_pack<double>, dim3(nblocks(cnt, 256)), dim3(256), 0, c->stream,
+                       (const double *)src, (double *)c->staging, c->nx, c->ny, box[0], bi, box[2],
+                       bj, box[4], cnt);
+  else
+    hipLaunchKernelGGL(fit::box_pack<float>, dim3(nblocks(cnt, 256)), dim3(256), 0, c->stream,
+                       (const float *)src, (float *)c->staging, c->nx, c->ny, box[0], bi, box[2], bj,
+                       box[4], cnt);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(host, c->staging, (size_t)cnt * c->esz, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fit_field_write(fit_ctx *c, int32_t which, int32_t comp, const int32_t box[6],
+                    const void *host) {
+  if (!c || !host) return fail(FIT_EINVAL, "null argument");
+  if (int rc = check_box(c, which, comp, box)) return rc;
+  if (int rc = set_device(c)) return rc;
+  const int bi = box[1] - box[0], bj = box[3] - box[2], bk = box[5] - box[4];
+  const int64_t cnt = (int64_t)bi * bj * bk;
+  char *dst = (char *)field_ptr(c, which) + (size_t)comp * c->N * c->esz;
+  if (bi == c->nx && bj == c->ny) {
+    HIPCHK(hipMemcpyAsync(dst + (size_t)box[4] * c->nx * c->ny * c->esz, host, (size_t)cnt * c->esz,
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+  }
+  if (int rc = ensure_staging(c, (size_t)cnt * c->esz)) return rc;
+  HIPCHK(hipMemcpyAsync(c->staging, host, (size_t)cnt * c->esz, hipMemcpyHostToDevice, c->stream));
+  if (c->dtype == FIT_F64)
+    hipLaunchKernelGGL(fit::box_unpack<double>, dim3(nblocks(cnt, 256)), dim3(256), 0, c->stream,
+                       (const double *)c->staging, (double *)dst, c->nx, c->ny, box[0], bi, box[2],
+                       bj, box[4], cnt);
+  else
+    hipLaunchKernelGGL(fit::box_unpack<float>, dim3(nblocks(cnt, 256)), dim3(256), 0, c->stream,
+                       (const float *)c->staging, (float *)dst, c->nx, c->ny, box[0], bi, box[2], bj,
+                       box[4], cnt);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fit_field_zero(fit_ctx *c, int32_t which) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (which < 0 || which > 2) return fail(FIT_EINVAL, "bad field id");
+  if (int rc = set_device(c)) return rc;
+  const int64_t cnt = 3 * c->N;
+  if (c->dtype == FIT_F64)
+    hipLaunchKernelGGL(fit::fill_zero<double>, dim3(nblocks(cnt, 256)), dim3(256), 0, c->stream,
+                       (double *)field_ptr(c, which), cnt);
+  else
+    hipLaunchKernelGGL(fit::fill_zero<float>, dim3(nblocks(cnt, 256)), dim3(256), 0, c->stream,
+                       (float *)field_ptr(c, which), cnt);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int fit_materialize_j(fit_ctx *c) {
+  if (int rc = check_ready(c)) return rc;
+  if (int rc = set_device(c)) return rc;
+  if (c->jmode == FIT_J_BUFFER) {
+    // buffer already holds J; only the beam line may need applying
+    if (c->beam_mode == FIT_BEAM_OFF) return 0;
+  }
+  if (c->dtype == FIT_F64) {
+    auto a = make_args<double>(c);
+    if (c->jmode == FIT_J_BUFFER) a.jmode = -1;
+    hipLaunchKernelGGL(fit::materialize_j<double>, dim3(nblocks(c->N, 256)), dim3(256), 0,
+                       c->stream, a);
+  } else {
+    auto a = make_args<float>(c);
+    if (c->jmode == FIT_J_BUFFER) a.jmode = -1;
+    hipLaunchKernelGGL(fit::materialize_j<float>, dim3(nblocks(c->N, 256)), dim3(256), 0,
+                       c->stream, a);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fit_zero_ghosts(fit_ctx *c) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (int rc = set_device(c)) return rc;
+  if (c->dtype == FIT_F64)
+    hipLaunchKernelGGL(fit::zero_ghosts<double>, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream,
+                       (double *)c->E, (double *)c->H, c->nx, c->ny, c->nz, c->N);
+  else
+    hipLaunchKernelGGL(fit::zero_ghosts<float>, dim3(nblocks(c->N, 256)), dim3(256), 0, c->stream,
+                       (float *)c->E, (float *)c->H, c->nx, c->ny, c->nz, c->N);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int fit_apply_abc(fit_ctx *c, uint32_t mask) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (mask & ~7u) return fail(FIT_EINVAL, "abc mask uses bits above 2");
+  if (int rc = set_device(c)) return rc;
+  for (int ax = 0; ax < 3; ++ax) {
+    if (!(mask & (1u << ax))) continue;
+    const int64_t plane = ax == 0 ? (int64_t)c->ny * c->nz
+                                  : (ax == 1 ? (int64_t)c->nx * c->nz : (int64_t)c->nx * c->ny);
+    if (c->dtype == FIT_F64)
+      hipLaunchKernelGGL(fit::abc_low<double>, dim3(nblocks(plane, 256)), dim3(256), 0, c->stream,
+                         (double *)c->E, (double *)c->H, ax, c->nx, c->ny, c->nz, c->N);
+    else
+      hipLaunchKernelGGL(fit::abc_low<float>, dim3(nblocks(plane, 256)), dim3(256), 0, c->stream,
+                         (float *)c->E, (float *)c->H, ax, c->nx, c->ny, c->nz, c->N);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+int fit_wake_probe(fit_ctx *c, int32_t ixt, int32_t iyt, int32_t h, int64_t nt_cap) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (int rc = set_device(c)) return rc;
+  if (c->hist) HIPCHK(hipFree(c->hist));
+  c->hist = nullptr;
+  c->hist_nt = 0;
+  if (nt_cap <= 0) return 0;
+  if (h < 0 || h > 1) return fail(FIT_EINVAL, "window half-width must be 0 or 1");
+  if (ixt - h < 0 || ixt + h >= c->nx || iyt - h < 0 || iyt + h >= c->ny)
+    return fail(FIT_EINVAL, "probe window outside the grid");
+  const int nw = (2 * h + 1) * (2 * h + 1);
+  const size_t bytes = (size_t)nw * c->nz * nt_cap * c->esz;
+  if (hipMalloc(&c->hist, bytes) != hipSuccess)
+    return fail(FIT_ENOMEM, "probe history allocation failed");
+  HIPCHK(hipMemsetAsync(c->hist, 0, bytes, c->stream));
+  c->hist_nt = nt_cap;
+  c->ixt = ixt;
+  c->iyt = iyt;
+  c->hw = h;
+  return 0;
+}
+
+int fit_wake_history(fit_ctx *c, void *host, int64_t count) {
+  if (!c || !host) return fail(FIT_EINVAL, "null argument");
+  if (!c->hist) return fail(FIT_ESTATE, "no probe history");
+  const int nw = (2 * c->hw + 1) * (2 * c->hw + 1);
+  if (count != (int64_t)nw * c->nz * c->hist_nt) return fail(FIT_EINVAL, "history count mismatch");
+  if (int rc = set_device(c)) return rc;
+  HIPCHK(hipMemcpyAsync(host, c->hist, (size_t)count * c->esz, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fit_step(fit_ctx *c, int32_t nsteps, int32_t conductive) {
+  if (int rc = check_ready(c)) return rc;
+  if (nsteps < 0) return fail(FIT_EINVAL, "nsteps < 0");
+  if (c->jmode == FIT_J_BUFFER && !c->J) return fail(FIT_ESTATE, "no J buffer");
+  if (int rc = set_device(c)) return rc;
+  return c->dtype == FIT_F64 ? step_impl<double>(c, nsteps, conductive)
+                             : step_impl<float>(c, nsteps, conductive);
+}
+
+int fit_step_timed(fit_ctx *c, int32_t nsteps, int32_t conductive, int32_t per_kernel,
+                   double *ms) {
+  if (int rc = check_ready(c)) return rc;
+  if (!ms) return fail(FIT_EINVAL, "null output");
+  if (nsteps < 1) return fail(FIT_EINVAL, "nsteps < 1");
+  if (int rc = set_device(c)) return rc;
+  ms[0] = ms[1] = ms[2] = 0.0;
+  if (!per_kernel) {
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = 0;
+    if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(FIT_EHIP, "event record");
+    if (!rc) rc = c->dtype == FIT_F64 ? step_impl<double>(c, nsteps, conductive)
+                                      : step_impl<float>(c, nsteps, conductive);
+    if (!rc && hipEventRecord(e1, c->stream) != hipSuccess) rc = fail(FIT_EHIP, "event record");
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(FIT_EHIP, "event sync");
+    float t = 0;
+    if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = fail(FIT_EHIP, "elapsed");
+    ms[0] = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+  }
+  std::vector<hipEvent_t> ev((size_t)3 * nsteps);
+  for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+  int rc = c->dtype == FIT_F64 ? step_impl<double>(c, nsteps, conductive, ev.data())
+                               : step_impl<float>(c, nsteps, conductive, ev.data());
+  if (!rc && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(FIT_EHIP, "event sync");
+  for (int s = 0; !rc && s < nsteps; ++s) {
+    float a = 0, b = 0;
+    if (hipEventElapsedTime(&a, ev[3 * s], ev[3 * s + 1]) != hipSuccess ||
+        hipEventElapsedTime(&b, ev[3 * s + 1], ev[3 * s + 2]) != hipSuccess) {
+      rc = fail(FIT_EHIP, "elapsed");
+      break;
+    }
+    ms[1] += a;
+    ms[2] += b;
+  }
+  ms[0] = ms[1] + ms[2];
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
+int fit_set_step_index(fit_ctx *c, int64_t step) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (step < 0) return fail(FIT_EINVAL, "step < 0");
+  c->step = step;
+  return 0;
+}
+
+int fit_get_step_index(fit_ctx *c, int64_t *step) {
+  if (!c || !step) return fail(FIT_EINVAL, "null argument");
+  *step = c->step;
+  return 0;
+}
+
+int fit_set_kernel(fit_ctx *c, int32_t kernel) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (kernel < FIT_KERNEL_AUTO || kernel > FIT_KERNEL_FUSED) return fail(FIT_EINVAL, "bad kernel id");
+  if (kernel == FIT_KERNEL_FUSED && !fit::fused_supported(kernel, c->nx, c->ny, c->nz))
+    return fail(FIT_EINVAL, "fused kernel does not support this grid");
+  c->kernel = kernel;
+  return 0;
+}
+
+int fit_sync(fit_ctx *c) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (int rc = set_device(c)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fit_wake_potential(fit_ctx *c, int32_t nwin, const int32_t *wids, int32_t carry, int32_t ns,
+                       int32_t nzw, int32_t k0, const int32_t *cols, const double *dz,
+                       double qnorm, double *wp) {
+  if (!c || !wids || !cols || !dz || !wp) return fail(FIT_EINVAL, "null argument");
+  if (!c->hist) return fail(FIT_ESTATE, "no probe history");
+  const int nwh = (2 * c->hw + 1) * (2 * c->hw + 1);
+  if (nwin < 1 || ns < 1 || nzw < 0 || k0 < 0 || k0 + nzw > c->nz)
+    return fail(FIT_EINVAL, "bad wake potential sizes");
+  for (int w = 0; w < nwin; ++w)
+    if (wids[w] < 0 || wids[w] >= nwh) return fail(FIT_EINVAL, "bad window id");
+  for (int64_t q = 0; q < (int64_t)nzw * ns; ++q)
+    if (cols[q] >= c->hist_nt) return fail(FIT_EINVAL, "time column beyond the probe history");
+  if (int rc = set_device(c)) return rc;
+  int32_t *dw = nullptr, *dcols = nullptr;
+  double *ddz = nullptr, *dwp = nullptr;
+  const size_t cb = (size_t)std::max<int64_t>((int64_t)nzw * ns, 1) * 4;
+  if (hipMalloc(&dw, nwin * 4) != hipSuccess || hipMalloc(&dcols, cb) != hipSuccess ||
+      hipMalloc(&ddz, std::max(nzw, 1) * 8) != hipSuccess ||
+      hipMalloc(&dwp, (size_t)nwin * ns * 8) != hipSuccess) {
+    (void)hipFree(dw);
+    (void)hipFree(dcols);
+    (void)hipFree(ddz);
+    (void)hipFree(dwp);
+    return fail(FIT_ENOMEM, "wake potential scratch allocation failed");
+  }
+  int rc = 0;
+  hipError_t e = hipMemcpyAsync(dw, wids, nwin * 4, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && nzw > 0)
+    e = hipMemcpyAsync(dcols, cols, (size_t)nzw * ns * 4, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && nzw > 0)
+    e = hipMemcpyAsync(ddz, dz, (size_t)nzw * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    if (c->dtype == FIT_F64)
+      hipLaunchKernelGGL(fit::wake_potential<double>, dim3(nblocks(ns, 256)), dim3(256), 0,
+                         c->stream, (const double *)c->hist, c->hist_nt, c->nz, dw, nwin, carry,
+                         ns, nzw, k0, dcols, ddz, qnorm, dwp);
+    else
+      hipLaunchKernelGGL(fit::wake_potential<float>, dim3(nblocks(ns, 256)), dim3(256), 0,
+                         c->stream, (const float *)c->hist, c->hist_nt, c->nz, dw, nwin, carry,
+                         ns, nzw, k0, dcols, ddz, qnorm, dwp);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(wp, dwp, (size_t)nwin * ns * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) rc = fail(FIT_EHIP, std::string("wake potential: ") + hipGetErrorString(e));
+  (void)hipFree(dw);
+  (void)hipFree(dcols);
+  (void)hipFree(ddz);
+  (void)hipFree(dwp);
+  return rc;
+}
+
+int fit_impedance(fit_ctx *c, const double *x, const double *lam, int64_t len, int64_t N,
+                  int32_t K, double ds, double *z_re, double *z_im, double *x_re, double *x_im,
+                  double *l_re, double *l_im) {
+  if (!c || !x || !lam || !z_re || !z_im || !x_re || !x_im || !l_re || !l_im)
+    return fail(FIT_EINVAL, "null argument");
+  if (len < 1 || N < 1 || K < 1 || K > N) return fail(FIT_EINVAL, "bad DFT sizes");
+  if ((int64_t)K * N > ((int64_t)1 << 62)) return fail(FIT_EINVAL, "DFT index overflow");
+  if (int rc = set_device(c)) return rc;
+  const int64_t M = std::min(len, N);
+  double *dx = nullptr, *dl = nullptr, *dout = nullptr;
+  if (hipMalloc(&dx, M * 8) != hipSuccess || hipMalloc(&dl, M * 8) != hipSuccess ||
+      hipMalloc(&dout, (size_t)K * 6 * 8) != hipSuccess) {
+    (void)hipFree(dx);
+    (void)hipFree(dl);
+    (void)hipFree(dout);
+    return fail(FIT_ENOMEM, "impedance scratch allocation failed");
+  }
+  std::vector<double> out((size_t)K * 6);
+  hipError_t e = hipMemcpyAsync(dx, x, M * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dl, lam, M * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(fit::impedance_dft, dim3(K), dim3(256), 0, c->stream, dx, dl, M, N, ds, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out.data(), dout, (size_t)K * 6 * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dx);
+  (void)hipFree(dl);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(FIT_EHIP, std::string("impedance: ") + hipGetErrorString(e));
+  for (int j = 0; j < K; ++j) {
+    z_re[j] = out[j * 6 + 0];
+    z_im[j] = out[j * 6 + 1];
+    x_re[j] = out[j * 6 + 2];
+    x_im[j] = out[j * 6 + 3];
+    l_re[j] = out[j * 6 + 4];
+    l_im[j] = out[j * 6 + 5];
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+// Shared device-side definitions of the FIT step kernels (gfx950).
+//
+// Field layout: three component blocks of N = nx*ny*nz values, Fortran order
+// inside a block (n = i + nx*j + nx*ny*k), as wakis/field.py:19-91.
+//
+// The discrete curl of the reference is a 3N x 3N sparse matrix built from
+// flat-index shift matrices P_d = diags([-1, 1], [0, s_d]) with
+// s = (1, nx, nx*ny) (wakis/solverFIT3D.py:207-227).  Its neighbour n + s_d
+// exists iff n + s_d < N, so an x or y neighbour past the end of a row wraps
+// "helically" into the next row/plane (i = nx-1 -> (0, j+1, k)), and a z
+// neighbour past the last plane is dropped.  The kernels reproduce that rule
+// exactly: nb_plus/nb_minus step (i, j, k) with carries/borrows.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fit {
+
+// Face bits: xlo, xhi, ylo, yhi, zlo, zhi.
+constexpr uint32_t XLO = 1u, XHI = 2u, YLO = 4u, YHI = 8u, ZLO = 16u, ZHI = 32u;
+
+enum { J_ZERO = 0, J_SIGMA_E = 1, J_BUFFER = 2 };
+enum { BEAM_OFF = 0, BEAM_VALUES = 1, BEAM_GAUSS = 2 };
+
+// Material table row: dt/mu x,y,z | dt/eps x,y,z | sigma x,y,z.
+constexpr int MAT_STRIDE = 9;
+
+template <typename T>
+struct Geo {
+  const T *Lx, *Ly, *Lz;        // primal lengths, 1-D per axis
+  const T *tLx, *tLy, *tLz;     // dual lengths, 1-D per axis
+  const T *iAx, *iAy, *iAz;     // inverse primal areas, 2-D (j,k) (i,k) (i,j)
+  const T *itAx, *itAy, *itAz;  // inverse dual areas, same 2-D layouts
+};
+
+template <typename T>
+struct StepArgs {
+  T *E;            // 3N
+  T *H;            // 3N
+  const T *J;      // 3N (J_BUFFER only)
+  const uint8_t *mat;  // N material ids (nullptr: all 0)
+  const T *mtab;   // nmat * MAT_STRIDE
+  Geo<T> g;
+  int nx, ny, nz;
+  int64_t N;
+  uint32_t pec;    // tangential-E column mask faces
+  uint32_t pmc;    // tangential-H row mask faces
+  int jmode;
+  // beam line
+  int beam_mode, ixs, iys;
+  const T *jline;        // BEAM_VALUES: nz values
+  const double *zc;      // BEAM_GAUSS: cell-centre z
+  double b_qv, b_A, b_B, b_s0, b_v, b_dx, b_dy;
+  bool b_f32time;  // t = step*dt and v*t as float32 products
+  double dt64;
+  float dt32;
+  int64_t step;
+  // wake probe
+  T *hist;
+  int64_t hist_nt;
+  int ixt, iyt, hw;
+};
+
+// Boundary mask of component `comp` at (i, j, k).  The reference builds it as
+// a Field of ones and assigns each face's flag to the tangential components on
+// that face, in the order xlo, xhi, ylo, yhi, zlo, zhi
+// (wakis/boundaries.py:90-100, 130-140): on an edge the LAST face written
+// wins, so an unmasked face (flag 1) written after a masked one unmasks the
+// edge.  m holds the masked faces (flag 0).
+__device__ __forceinline__ bool masked(uint32_t m, int comp, int i, int j, int k,
+                                       int nx, int ny, int nz) {
+  bool r = false;
+  if (comp != 0) {
+    if (i == 0) r = (m & XLO) != 0;
+    if (i == nx - 1) r = (m & XHI) != 0;
+  }
+  if (comp != 1) {
+    if (j == 0) r = (m & YLO) != 0;
+    if (j == ny - 1) r = (m & YHI) != 0;
+  }
+  if (comp != 2) {
+    if (k == 0) r = (m & ZLO) != 0;
+    if (k == nz - 1) r = (m & ZHI) != 0;
+  }
+  return r;
+}
+
+// Flat-index neighbour n + s_axis, as (i, j, k) with carries; false if >= N.
+__device__ __forceinline__ bool nb_plus(int axis, int &i, int &j, int &k, int nx,
+                                        int ny, int nz) {
+  if (axis == 0) {
+    if (++i < nx) return true;
+    i = 0;
+    if (++j < ny) return true;
+    j = 0;
+    return ++k < nz;
+  }
+  if (axis == 1) {
+    if (++j < ny) return true;
+    j = 0;
+    return ++k < nz;
+  }
+  return ++k < nz;
+}
+
+// Flat-index neighbour n - s_axis, as (i, j, k) with borrows; false if < 0.
+__device__ __forceinline__ bool nb_minus(int axis, int &i, int &j, int &k, int nx,
+                                         int ny, int nz) {
+  if (axis == 0) {
+    if (--i >= 0) return true;
+    i = nx - 1;
+    if (--j >= 0) return true;
+    j = ny - 1;
+    return --k >= 0;
+  }
+  if (axis == 1) {
+    if (--j >= 0) return true;
+    j = ny - 1;
+    return --k >= 0;
+  }
+  return --k >= 0;
+}
+
+__device__ __forceinline__ int64_t flat(int i, int j, int k, int nx, int ny) {
+  return (int64_t)i + (int64_t)nx * ((int64_t)j + (int64_t)ny * k);
+}
+
+template <typename T>
+__device__ __forceinline__ T Lof(const Geo<T> &g, int comp, int i, int j, int k) {
+  return comp == 0 ? g.Lx[i] : (comp == 1 ? g.Ly[j] : g.Lz[k]);
+}
+template <typename T>
+__device__ __forceinline__ T tLof(const Geo<T> &g, int comp, int i, int j, int k) {
+  return comp == 0 ? g.tLx[i] : (comp == 1 ? g.tLy[j] : g.tLz[k]);
+}
+
+// Column-scaled E of the H update: u_d = L_d * pecmask_d * E_d  (0 if absent).
+template <typename T>
+__device__ __forceinline__ T u_at(const StepArgs<T> &a, int comp, int i, int j, int k) {
+  if (masked(a.pec, comp, i, j, k, a.nx, a.ny, a.nz)) return T(0);
+  return Lof(a.g, comp, i, j, k) * a.E[comp * a.N + flat(i, j, k, a.nx, a.ny)];
+}
+
+// Column-scaled H of the E update: w_d = tL_d * pmcmask_d * H_d.
+template <typename T>
+__device__ __forceinline__ T w_at(const StepArgs<T> &a, int comp, int i, int j, int k) {
+  if (masked(a.pmc, comp, i, j, k, a.nx, a.ny, a.nz)) return T(0);
+  return tLof(a.g, comp, i, j, k) * a.H[comp * a.N + flat(i, j, k, a.nx, a.ny)];
+}
+
+// Beam current J_z on the beam line at plane k (wakis/sources.py:96-110).
+// With a float32 dt, numpy evaluates t = n*dt and v*t as float32 products
+// (python scalars are cast to float32 first); the rest is FP64; the value is
+// then stored in the solver dtype.
+template <typename T>
+__device__ __forceinline__ T beam_gauss(const StepArgs<T> &a, int k) {
+  double vt;
+  if (a.b_f32time) {
+    const float t = (float)a.step * a.dt32;
+    vt = (double)((float)a.b_v * t);
+  } else {
+    const double t = (double)a.step * a.dt64;
+    vt = a.b_v * t;
+  }
+  const double s = a.zc[k] - vt;
+  const double d = s - a.b_s0;
+  const double prof = a.b_A * exp(-(d * d) / a.b_B);
+  return (T)(a.b_qv * prof / a.b_dx / a.b_dy);
+}
+
+}  // namespace fit

@@ -1,0 +1,145 @@
+// Split step: one H-update kernel, then one E-update kernel, in place.
+//
+// H update (rows of M_H = iDa*iDmu*C*Ds, wakis/solverFIT3D.py:343, 510-512):
+//   H_x -= dt/mu_x * iA_x * pmc_x * ( u_y[n] - u_y[n+sz] - u_z[n] + u_z[n+sy] )
+//   H_y -= dt/mu_y * iA_y * pmc_y * ( u_z[n] - u_z[n+1]  - u_x[n] + u_x[n+sz] )
+//   H_z -= dt/mu_z * iA_z * pmc_z * ( u_x[n] - u_x[n+sy] - u_y[n] + u_y[n+1]  )
+//   with u_d = L_d * pec_d * E_d.
+// E update (rows of M_E = iDeps*itDa*C^T*tDs, wakis/solverFIT3D.py:344, 514-521):
+//   E_x += dt/eps_x * ( pec_x * itA_x * ( w_z[n] - w_z[n-sy] - w_y[n] + w_y[n-sz] ) - J_x )
+//   E_y += dt/eps_y * ( pec_y * itA_y * ( w_x[n] - w_x[n-sz] - w_z[n] + w_z[n-1]  ) - J_y )
+//   E_z += dt/eps_z * ( pec_z * itA_z * ( w_y[n] - w_y[n-1]  - w_x[n] + w_x[n-sy] ) - J_z )
+//   with w_d = tL_d * pmc_d * H_d and J per the J mode (J_SIGMA_E: J = sigma*E
+//   of the state before this update, which is what the reference's
+//   J = sigma*E (solverFIT3D.py:524-525) holds at that point).
+// These kernels handle every cell with the generic flat-index neighbour rule;
+// they are the reference implementation of the operator on the device and
+// serve any grid size.
+#pragma once
+
+#include "fit_common.hpp"
+
+namespace fit {
+
+template <typename T>
+__global__ void __launch_bounds__(256) h_update_split(StepArgs<T> a) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.N) return;
+  const int64_t nxy = (int64_t)a.nx * a.ny;
+  const int k = (int)(n / nxy);
+  const int64_t r = n - (int64_t)k * nxy;
+  const int j = (int)(r / a.nx);
+  const int i = (int)(r - (int64_t)j * a.nx);
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+
+  const T ux0 = u_at(a, 0, i, j, k), uy0 = u_at(a, 1, i, j, k), uz0 = u_at(a, 2, i, j, k);
+  T uy_px = 0, uz_px = 0, ux_py = 0, uz_py = 0, ux_pz = 0, uy_pz = 0;
+  {
+    int ii = i, jj = j, kk = k;
+    if (nb_plus(0, ii, jj, kk, nx, ny, nz)) {
+      uy_px = u_at(a, 1, ii, jj, kk);
+      uz_px = u_at(a, 2, ii, jj, kk);
+    }
+  }
+  {
+    int ii = i, jj = j, kk = k;
+    if (nb_plus(1, ii, jj, kk, nx, ny, nz)) {
+      ux_py = u_at(a, 0, ii, jj, kk);
+      uz_py = u_at(a, 2, ii, jj, kk);
+    }
+  }
+  {
+    int ii = i, jj = j, kk = k;
+    if (nb_plus(2, ii, jj, kk, nx, ny, nz)) {
+      ux_pz = u_at(a, 0, ii, jj, kk);
+      uy_pz = u_at(a, 1, ii, jj, kk);
+    }
+  }
+  const int m = a.mat ? a.mat[n] : 0;
+  const T *mt = a.mtab + m * MAT_STRIDE;
+  const T cx = masked(a.pmc, 0, i, j, k, nx, ny, nz) ? T(0) : mt[0] * a.g.iAx[j + (int64_t)ny * k];
+  const T cy = masked(a.pmc, 1, i, j, k, nx, ny, nz) ? T(0) : mt[1] * a.g.iAy[i + (int64_t)nx * k];
+  const T cz = masked(a.pmc, 2, i, j, k, nx, ny, nz) ? T(0) : mt[2] * a.g.iAz[i + (int64_t)nx * j];
+  const T curl_x = ((uy0 - uy_pz) - uz0) + uz_py;
+  const T curl_y = ((uz0 - uz_px) - ux0) + ux_pz;
+  const T curl_z = ((ux0 - ux_py) - uy0) + uy_px;
+  T *H = a.H;
+  H[n] = H[n] - cx * curl_x;
+  H[a.N + n] = H[a.N + n] - cy * curl_y;
+  H[2 * a.N + n] = H[2 * a.N + n] - cz * curl_z;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) e_update_split(StepArgs<T> a) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.N) return;
+  const int64_t nxy = (int64_t)a.nx * a.ny;
+  const int k = (int)(n / nxy);
+  const int64_t r = n - (int64_t)k * nxy;
+  const int j = (int)(r / a.nx);
+  const int i = (int)(r - (int64_t)j * a.nx);
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+
+  const T wx0 = w_at(a, 0, i, j, k), wy0 = w_at(a, 1, i, j, k), wz0 = w_at(a, 2, i, j, k);
+  T wy_mx = 0, wz_mx = 0, wx_my = 0, wz_my = 0, wx_mz = 0, wy_mz = 0;
+  {
+    int ii = i, jj = j, kk = k;
+    if (nb_minus(0, ii, jj, kk, nx, ny, nz)) {
+      wy_mx = w_at(a, 1, ii, jj, kk);
+      wz_mx = w_at(a, 2, ii, jj, kk);
+    }
+  }
+  {
+    int ii = i, jj = j, kk = k;
+    if (nb_minus(1, ii, jj, kk, nx, ny, nz)) {
+      wx_my = w_at(a, 0, ii, jj, kk);
+      wz_my = w_at(a, 2, ii, jj, kk);
+    }
+  }
+  {
+    int ii = i, jj = j, kk = k;
+    if (nb_minus(2, ii, jj, kk, nx, ny, nz)) {
+      wx_mz = w_at(a, 0, ii, jj, kk);
+      wy_mz = w_at(a, 1, ii, jj, kk);
+    }
+  }
+  const int m = a.mat ? a.mat[n] : 0;
+  const T *mt = a.mtab + m * MAT_STRIDE;
+  T *E = a.E;
+  const T ex = E[n], ey = E[a.N + n], ez = E[2 * a.N + n];
+
+  // wake probe: E_z of the state before this update (wakis/routines.py:443)
+  if (a.hist && a.step < a.hist_nt) {
+    const int di = i - a.ixt, dj = j - a.iyt;
+    if (di >= -a.hw && di <= a.hw && dj >= -a.hw && dj <= a.hw) {
+      const int w = (di + a.hw) * (2 * a.hw + 1) + (dj + a.hw);
+      a.hist[((int64_t)w * nz + k) * a.hist_nt + a.step] = ez;
+    }
+  }
+
+  T jx = 0, jy = 0, jz = 0;
+  if (a.jmode == J_SIGMA_E) {
+    jx = mt[6] * ex;
+    jy = mt[7] * ey;
+    jz = mt[8] * ez;
+  } else if (a.jmode == J_BUFFER) {
+    jx = a.J[n];
+    jy = a.J[a.N + n];
+    jz = a.J[2 * a.N + n];
+  }
+  if (a.beam_mode != BEAM_OFF && i == a.ixs && j == a.iys) {
+    jz = (a.beam_mode == BEAM_VALUES) ? a.jline[k] : beam_gauss<T>(a, k);
+  }
+
+  const T ax = masked(a.pec, 0, i, j, k, nx, ny, nz) ? T(0) : a.g.itAx[j + (int64_t)ny * k];
+  const T ay = masked(a.pec, 1, i, j, k, nx, ny, nz) ? T(0) : a.g.itAy[i + (int64_t)nx * k];
+  const T az = masked(a.pec, 2, i, j, k, nx, ny, nz) ? T(0) : a.g.itAz[i + (int64_t)nx * j];
+  const T cx = ((wz0 - wz_my) - wy0) + wy_mz;
+  const T cy = ((wx0 - wx_mz) - wz0) + wz_mx;
+  const T cz = ((wy0 - wy_mx) - wx0) + wx_my;
+  E[n] = ex + mt[3] * (ax * cx - jx);
+  E[a.N + n] = ey + mt[4] * (ay * cy - jy);
+  E[2 * a.N + n] = ez + mt[5] * (az * cz - jz);
+}
+
+}  // namespace fit

@@ -1,0 +1,421 @@
+"""``SolverFIT3D``: the reference's solver API on top of the HIP FIT library.
+
+Construction restates the reference's one-time FP64 setup
+(wakis/solverFIT3D.py:40-376): boundary masks (wakis/boundaries.py:14-188),
+background and solid materials (solverFIT3D.py:240-279, 438-502), PML
+grading (wakis/boundaries.py:190-305), dt with the relaxation clamp
+(solverFIT3D.py:291-318).  Instead of the 3N x 3N sparse operators, the
+device gets the ingredients of the matrix-free stencil: 1-D/2-D metric
+tables, a per-cell material id with a small table of (1/eps, 1/mu, sigma)
+per component, and face masks.  The drop-in slot is the instance attribute
+``one_step`` (a 0-argument callable, solverFIT3D.py:143) which launches the
+HIP step kernels; fields stay in HBM and are reachable through ``E``, ``H``
+and ``J`` (``DeviceField``).
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+from scipy.constants import c as c_light
+from scipy.constants import epsilon_0 as eps_0
+from scipy.constants import mu_0
+
+from . import _lib
+from .field import DeviceField, Field
+from .grid import metric_tables
+from .materials import material_lib
+from .routines import RoutinesMixin
+
+_PEC_LIKE = ("electric", "pec", "pml")
+_FACES = ("xlo", "xhi", "ylo", "yhi", "zlo", "zhi")
+
+
+def face_bits(bc_low, bc_high):
+    """(pec_cols, pmc_rows) face masks with the reference's flag carry-over:
+    the PMC block reuses the PEC block's xlo..zhi flags
+    (wakis/boundaries.py:24-25, 73-150)."""
+    bl, bh = list(bc_low), list(bc_high)
+    flags = [1, 1, 1, 1, 1, 1]  # xlo xhi ylo yhi zlo zhi
+    pec = 0
+    if any(b.lower() in _PEC_LIKE for b in bl) or any(b.lower() in _PEC_LIKE for b in bh):
+        for d in range(3):
+            if bl[d].lower() in _PEC_LIKE:
+                flags[2 * d] = 0
+            if bh[d].lower() in _PEC_LIKE:
+                flags[2 * d + 1] = 0
+        pec = sum(1 << f for f in range(6) if flags[f] == 0)
+    pmc = 0
+    if any(b.lower() in ("magnetic", "pmc") for b in bl) or any(
+            b.lower() in ("magnetic", "pmc") for b in bh):
+        for d in range(3):
+            if bl[d].lower() == "magnetic" or bl[d] == "pmc":
+                flags[2 * d] = 0
+            if bh[d].lower() == "magnetic" or bh[d] == "pmc":
+                flags[2 * d + 1] = 0
+        pmc = sum(1 << f for f in range(6) if flags[f] == 0)
+    return pec, pmc
+
+
+class _MaterialTable:
+    """Per-cell material ids + rows of 9 values (1/eps, 1/mu, sigma per
+    component), each value already rounded to the solver dtype, exactly the
+    values the reference's ieps/imu/sigma Fields would hold."""
+
+    def __init__(self, shape, row0, dtype):
+        self.dtype = dtype
+        self.rows = [tuple(row0)]
+        self.index = {tuple(row0): 0}
+        self.ids = np.zeros(shape, dtype=np.uint16)
+
+    def intern(self, row):
+        row = tuple(float(v) for v in row)
+        if row not in self.index:
+            self.index[row] = len(self.rows)
+            self.rows.append(row)
+        return self.index[row]
+
+    def apply(self, region, fn):
+        """ids[region] := id(fn(rows[old id])) for every old id in region."""
+        sub = self.ids[region]
+        old = np.unique(sub)
+        lut = np.arange(len(self.rows), dtype=np.uint16)
+        for o in old:
+            nid = self.intern(fn(self.rows[int(o)]))
+            if nid > np.iinfo(np.uint16).max:
+                raise ValueError("too many distinct materials")
+            if nid >= len(lut):
+                lut = np.concatenate([lut, np.arange(len(lut), nid + 1, dtype=np.uint16)])
+            lut[o] = nid
+        self.ids[region] = lut[sub]
+
+    def table(self):
+        return np.asarray(self.rows, dtype=np.float64)
+
+
+def _cast(dtype, value):
+    """A python/numpy scalar stored into a `dtype` array (Field semantics)."""
+    return float(np.asarray(value, dtype=np.float64).reshape(-1)[0].astype(dtype))
+
+
+class SolverFIT3D(RoutinesMixin):
+    def __init__(self, grid, wake=None, cfln=0.5, dt=None,
+                 bc_low=("Periodic", "Periodic", "Periodic"),
+                 bc_high=("Periodic", "Periodic", "Periodic"),
+                 use_stl=False, use_conductors=False, use_gpu=True, use_mpi=False,
+                 use_sibc=True, fmax=1e9, dtype=np.float64, n_pml=10, bg=(1.0, 1.0),
+                 verbose=1, device=0, kernel="auto"):
+        t0 = time.time()
+        if use_mpi:
+            raise NotImplementedError(
+                "use_mpi: multi-GPU z-slab decomposition is not part of this build")
+        if use_conductors:
+            raise NotImplementedError("use_conductors: legacy conductor masks are not supported")
+        self.verbose = verbose
+        self.step_0 = True
+        self.nstep = 0
+        self.use_stl = use_stl
+        self.use_gpu = True
+        self.use_mpi = False
+        self.use_sibc = use_sibc
+        self.fmax = fmax
+        self.activate_abc = False
+        self.activate_pml = False
+        self.use_conductivity = False
+        self.one_step = self._one_step
+
+        self.grid = grid
+        self.background = bg
+        self.Nx, self.Ny, self.Nz = grid.Nx, grid.Ny, grid.Nz
+        self.N = self.Nx * self.Ny * self.Nz
+        self.dx, self.dy, self.dz = grid.dx, grid.dy, grid.dz
+        self.x = grid.x[:-1] + self.dx / 2
+        self.y = grid.y[:-1] + self.dy / 2
+        self.z = grid.z[:-1] + self.dz / 2
+
+        self.wake = wake
+        if wake is not None and fmax == 1e9:
+            self.fmax = wake.fmax
+        self.dtype = dtype
+        self.bc_low, self.bc_high = list(bc_low), list(bc_high)
+
+        # boundaries (wakis/boundaries.py:14-188)
+        self.pec_bits, self.pmc_bits = face_bits(self.bc_low, self.bc_high)
+        bl = [b.lower() for b in self.bc_low]
+        bh = [b.lower() for b in self.bc_high]
+        if "abc" in bl or "abc" in bh:
+            self.activate_abc = True
+        if "pml" in bl or "pml" in bh:
+            self.activate_pml = True
+            self.use_conductivity = True
+
+        # materials (solverFIT3D.py:240-279)
+        if type(bg) is str:
+            bg = material_lib[bg.lower()]
+        bg = list(bg)
+        if len(bg) == 3:
+            self.eps_bg, self.mu_bg, self.sigma_bg = bg[0] * eps_0, bg[1] * mu_0, bg[2]
+            self.use_conductivity = True
+        else:
+            self.eps_bg, self.mu_bg, self.sigma_bg = bg[0] * eps_0, bg[1] * mu_0, 0.0
+        dt_np = np.dtype(dtype).type
+        ie = _cast(dtype, np.ones(1, dtype=dtype) * (1.0 / self.eps_bg))
+        im = _cast(dtype, np.ones(1, dtype=dtype) * (1.0 / self.mu_bg))
+        sg = _cast(dtype, np.ones(1, dtype=dtype) * self.sigma_bg)
+        self._mt = _MaterialTable((self.Nx, self.Ny, self.Nz), [ie] * 3 + [im] * 3 + [sg] * 3,
+                                  dtype)
+        if use_stl:
+            self._apply_stl_materials()
+        if self.activate_pml:
+            self.n_pml = n_pml
+            self._initialize_PML()
+        self._mat_fields = None
+
+        # dt (solverFIT3D.py:291-318)
+        self.cfln = cfln
+        if dt is None:
+            self.dt = cfln / (c_light * np.sqrt(1 / np.min(grid.dx) ** 2
+                                                 + 1 / np.min(grid.dy) ** 2
+                                                 + 1 / np.min(grid.dz) ** 2))
+        else:
+            self.dt = dt
+        self.dt = dt_np(self.dt)
+        if self.use_conductivity:
+            self._relaxation_clamp()
+
+        self.geometry = metric_tables(grid.x, grid.y, grid.z, self.bc_low, self.bc_high)
+
+        # device context (device=None: host-side setup only, no stepping)
+        self.ctx = None
+        if device is not None:
+            self.ctx = _lib.Context(self.Nx, self.Ny, self.Nz, dtype=dtype, device=device)
+            self.ctx.set_geometry(self.geometry)
+            self._upload_materials()
+            self.ctx.set_bc(self.pec_bits, self.pmc_bits)
+            self.ctx.set_dt(float(self.dt))
+            self.set_kernel(kernel)
+            self.E = DeviceField(self.ctx, _lib.FIELD_E, self)
+            self.H = DeviceField(self.ctx, _lib.FIELD_H, self)
+            self.J = DeviceField(self.ctx, _lib.FIELD_J, self)
+        self.solverInitializationTime = time.time() - t0
+
+    @property
+    def dt(self):
+        return self._dt
+
+    @dt.setter
+    def dt(self, value):
+        # the reference reads self.dt at every step (solverFIT3D.py:511, 518),
+        # so a later assignment changes the time step
+        self._dt = value
+        if getattr(self, "ctx", None) is not None:
+            self.ctx.set_dt(float(value))
+
+    def _require_device(self):
+        if self.ctx is None:
+            raise _lib.FitError("solver was built with device=None (host setup only)")
+
+    # -- kernels -------------------------------------------------------------
+    def set_kernel(self, kernel="auto"):
+        k = {"auto": _lib.KERNEL_AUTO, "split": _lib.KERNEL_SPLIT,
+             "fused": _lib.KERNEL_FUSED}[kernel]
+        self.ctx.set_kernel(k)
+        self.kernel = kernel
+
+    # -- materials -----------------------------------------------------------
+    def _apply_stl_materials(self):
+        """wakis/solverFIT3D.py:438-502 on the grid's solid masks."""
+        dtype = self.dtype
+        for name, mask, mat in self.grid.solids:
+            if type(mat) is str:
+                mat = material_lib[mat.lower()]
+            mat = list(mat)
+            eps = mat[0] * eps_0
+            mu = mat[1] * mu_0
+            new_sigma = None  # None: unchanged
+            if len(mat) == 3:
+                sigma = mat[2]
+                if self.use_sibc and sigma > 0.0:
+                    eps = sigma * eps_0
+                new_sigma = _cast(dtype, 0.0 + np.float64(1.0) * sigma)
+                self.use_conductivity = True
+            elif self.sigma_bg > 0.0:
+                new_sigma = 0.0
+            ie = _cast(dtype, np.float64(1.0) / eps)
+            im = _cast(dtype, np.float64(1.0) / mu)
+
+            def fn(row, ie=ie, im=im, new_sigma=new_sigma):
+                s = row[6:9] if new_sigma is None else (new_sigma,) * 3
+                return (ie, ie, ie, im, im, im) + tuple(s)
+
+            self._mt.apply(np.asarray(mask, dtype=bool), fn)
+
+    def _initialize_PML(self):
+        """wakis/boundaries.py:190-305: ieps := 1/eps_0 and a geomspace sigma
+        (1 S/m at the outer cell down to 5e-3) on n_pml planes per PML face,
+        faces in the order xlo, ylo, zlo, xhi, yhi, zhi."""
+        n = self.n_pml
+        lo, hi = 5.0e-3, 1.0
+        ie = _cast(self.dtype, 1 / eps_0)
+        dims = (self.Nx, self.Ny, self.Nz)
+        order = [(0, "lo"), (1, "lo"), (2, "lo"), (0, "hi"), (1, "hi"), (2, "hi")]
+        for ax, side in order:
+            bc = self.bc_low[ax] if side == "lo" else self.bc_high[ax]
+            if bc.lower() != "pml":
+                continue
+            prof = np.geomspace(hi, lo, n) if side == "lo" else np.geomspace(lo, hi, n)
+            for p in range(n):
+                plane = p if side == "lo" else dims[ax] - n + p
+                sig = _cast(self.dtype, 0.0 + 1 * prof[p])
+                region = [slice(None)] * 3
+                region[ax] = plane
+                self._mt.apply(tuple(region),
+                               lambda row, s=sig: (ie, ie, ie) + tuple(row[3:6]) + (s, s, s))
+
+    def _relaxation_clamp(self):
+        """dt <= min(eps/sigma) over conductive, non-PEC entries
+        (wakis/solverFIT3D.py:309-318), evaluated on the material rows in use."""
+        used = np.unique(self._mt.ids)
+        tab = self._mt.table()[used]
+        ieps = tab[:, 0:3].reshape(-1).astype(self.dtype)
+        sig = tab[:, 6:9].reshape(-1).astype(self.dtype)
+        mask = np.logical_and(sig != 0, ieps != 0)
+        tau = (1 / ieps[mask]) / sig[mask]
+        if self.dt > tau.min():
+            self.dt = tau.min()
+
+    def _upload_materials(self):
+        tab = self._mt.table()
+        if len(tab) > 256:
+            raise ValueError(f"{len(tab)} distinct materials; the device table holds 256")
+        ids = self._mt.ids
+        if len(tab) == 1:
+            self.ctx.set_materials(None, tab)
+        else:
+            self.ctx.set_materials(np.reshape(ids.astype(np.uint8), -1, order="F"), tab)
+
+    def _material_field(self, col):
+        if self._mat_fields is None:
+            tab = self._mt.table()
+            ids = np.reshape(self._mt.ids, -1, order="F")
+            out = {}
+            for name, c0 in (("ieps", 0), ("imu", 3), ("sigma", 6)):
+                f = Field(self.Nx, self.Ny, self.Nz, dtype=self.dtype)
+                for d in range(3):
+                    f.array[d * self.N:(d + 1) * self.N] = tab[ids, c0 + d]
+                out[name] = f
+            self._mat_fields = out
+        return self._mat_fields[col]
+
+    def _set_material_field(self, col, value):
+        self._material_field(col)
+        self._mat_fields[col] = value
+
+    ieps = property(lambda s: s._material_field("ieps"),
+                    lambda s, v: s._set_material_field("ieps", v))
+    imu = property(lambda s: s._material_field("imu"),
+                   lambda s, v: s._set_material_field("imu", v))
+    sigma = property(lambda s: s._material_field("sigma"),
+                     lambda s, v: s._set_material_field("sigma", v))
+
+    def update_tensors(self, tensor="all"):
+        """Re-derive the material ids from the (possibly edited) ieps, imu,
+        sigma Fields and upload them (wakis/solverFIT3D.py:378-436); like the
+        reference this also disables the first-step ghost zeroing."""
+        if self._mat_fields is not None:
+            cols = []
+            for name in ("ieps", "imu", "sigma"):
+                a = np.asarray(self._mat_fields[name].toarray()).astype(self.dtype)
+                cols += [a[d * self.N:(d + 1) * self.N].astype(np.float64) for d in range(3)]
+            stacked = np.stack(cols, axis=1)
+            rows, inv = np.unique(stacked, axis=0, return_inverse=True)
+            mt = _MaterialTable((self.Nx, self.Ny, self.Nz), rows[0], self.dtype)
+            mt.rows = [tuple(r) for r in rows]
+            mt.index = {r: i for i, r in enumerate(mt.rows)}
+            mt.ids = np.reshape(inv.reshape(-1).astype(np.uint16), (self.Nx, self.Ny, self.Nz),
+                                order="F")
+            self._mt = mt
+            self._mat_fields = None
+        self._upload_materials()
+        self.ctx.set_dt(float(self.dt))
+        self.step_0 = False
+
+    # -- fields: J bookkeeping -----------------------------------------------
+    def _before_field_read(self, which):
+        if which == _lib.FIELD_J:
+            self.ctx.materialize_j()
+
+    def _before_field_write(self, which, key, value):
+        """Route a write of the whole beam line J[i, j, :, 'z'] to the device
+        beam line (as sources.Beam.update does); any other J write switches the
+        E update to read J from its buffer."""
+        if which != _lib.FIELD_J:
+            return False
+        if key is not None and len(key) == 4 and key[3] in ("z", 2):
+            i, j, kk = key[0], key[1], key[2]
+            if (isinstance(i, (int, np.integer)) and isinstance(j, (int, np.integer))
+                    and isinstance(kk, slice) and kk.indices(self.Nz) == (0, self.Nz, 1)):
+                v = np.broadcast_to(np.asarray(value, dtype=np.float64), (self.Nz,))
+                jm, bm = self.ctx.get_state()
+                if bm == _lib.BEAM_OFF or (self._beam_ij == (int(i) % self.Nx, int(j) % self.Ny)):
+                    self._set_beam_line(int(i) % self.Nx, int(j) % self.Ny, v)
+                    return True
+        # general write: bring J into its buffer first
+        self.ctx.materialize_j()
+        self.ctx.beam_off()
+        self._beam_ij = None
+        self.ctx.set_jmode(_lib.J_BUFFER)
+        return False
+
+    _beam_ij = None
+
+    def _set_beam_line(self, ixs, iys, values):
+        self.ctx.set_beam_values(ixs, iys, values)
+        self._beam_ij = (ixs, iys)
+
+    # -- time stepping (wakis/solverFIT3D.py:504-525) ------------------------
+    def _one_step(self):
+        self._require_device()
+        if self.step_0:
+            self.ctx.zero_ghosts()
+            self.step_0 = False
+        self.ctx.step(1, conductive=self.use_conductivity)
+
+    def _advance(self, nsteps):
+        """nsteps calls of one_step in one asynchronous launch sequence."""
+        self._require_device()
+        if nsteps <= 0:
+            return
+        if self.step_0:
+            self.ctx.zero_ghosts()
+            self.step_0 = False
+        self.ctx.step(nsteps, conductive=self.use_conductivity)
+
+    def synchronize(self):
+        self.ctx.sync()
+
+    # -- ABC (wakis/boundaries.py:307-404, CPU aliasing semantics) -----------
+    def _abc_low_mask(self):
+        mask = 0
+        for d in range(3):
+            lo = self.bc_low[d].lower() == "abc"
+            hi = self.bc_high[d].lower() == "abc"
+            if lo and hi:
+                # get_abc rebuilds the axis dict for the high face, so the
+                # low-face key is missing in update_abc
+                raise KeyError("xyz"[d] + "lo")
+            if lo:
+                mask |= 1 << d
+        return mask
+
+    def get_abc(self):
+        return self._abc_low_mask(), None
+
+    def update_abc(self, E_abc=None, H_abc=None):
+        mask = self._abc_low_mask()
+        if mask:
+            self.ctx.apply_abc(mask)
+
+    def close(self):
+        self.ctx.close()

@@ -1,0 +1,263 @@
+"""``WakeSolver``: longitudinal wake potential and impedance on the device.
+
+Parameters and defaults follow wakis/wakeSolver.py:22-209 (fmax = v/(3
+sigmaz), injection time ti = 8.548921333333334*sigmaz/(sqrt(beta)*v)).
+The E_z history recorded by ``SolverFIT3D.wakesolve`` stays in HBM; the
+integral WP(s) = sum_k Ez(z_k, t(s, z_k)) dz_k of wakis/wakeSolver.py:423-551
+runs in one FP64 kernel (one thread per s sample, the reference's per-sample
+order of additions, the WP carry-over between the 9 windows of
+calc_long_WP_3d kept), and the impedance of wakis/wakeSolver.py:628-720 is a
+direct FP64 DFT of the kept bins on the device.  The host computes only the
+time-bucket table it(n, k) = int(ts/dt) with the reference's numpy
+expressions, so every bucket index is identical.
+"""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+from scipy.constants import c as c_light
+
+
+class WakeSolver:
+    def __init__(self, wakelength=None, q=1e-9, sigmaz=1e-3, beta=1.0, xsource=0.0,
+                 ysource=0.0, xtest=0.0, ytest=0.0, chargedist=None, ti=None, fmax=None,
+                 compute_plane="both", skip_cells=0, add_space=None, Ez_file=None, save=True,
+                 results_folder="results/", verbose=1, counter_moving=False):
+        self.verbose = verbose
+        self.q = q
+        self.sigmaz = sigmaz
+        self.beta = beta
+        self.v = self.beta * c_light
+        self.xsource, self.ysource = xsource, ysource
+        self.xtest, self.ytest = xtest, ytest
+        self.chargedist = chargedist
+        self.fmax = self.v / (3.0 * self.sigmaz) if fmax is None else fmax
+        self.skip_cells = skip_cells if add_space is None else add_space
+        self.compute_plane = compute_plane
+        self.counter_moving = counter_moving
+        if ti is not None:
+            self.ti = ti
+        else:
+            self.ti = 8.548921333333334 * self.sigmaz / (np.sqrt(self.beta) * self.v)
+        self.Ez_file = Ez_file
+        self.wakelength = wakelength
+        self.s = None
+        self.lambdas = None
+        self.WP = None
+        self.WP_3d = None
+        self.n_transverse_cells = 1
+        self.WPx = self.WPy = None
+        self.f = None
+        self.Z = None
+        self.Zx = self.Zy = None
+        self.lambdaf = None
+        self.save = save
+        self.folder = results_folder if results_folder.endswith("/") else results_folder + "/"
+        if self.Ez_file is None:
+            self.Ez_file = self.folder + "Ez.h5"
+        if self.save:
+            os.makedirs(self.folder, exist_ok=True)
+        self._ctx = None
+        self._meta = None
+
+    def attach_device_history(self, ctx, meta):
+        """Use the E_z history of a device context (written by wakesolve)."""
+        self._ctx = ctx
+        self._meta = meta
+        self.xf, self.yf, self.zf = meta["x"], meta["y"], meta["z"]
+        self.dx, self.dy, self.dz = meta["dx"], meta["dy"], meta["dz"]
+        self.t = meta["t"]
+
+    def _need(self):
+        if self._ctx is None:
+            raise RuntimeError(
+                "no E_z history on the device: run SolverFIT3D.wakesolve first "
+                "(the Ez.h5 file path needs h5py, which is not used here)")
+        if self.counter_moving:
+            raise NotImplementedError("counter_moving wakes are not part of this build")
+
+    def solve(self, compute_plane=None, **kwargs):
+        """wakis/wakeSolver.py:211-255."""
+        if compute_plane is None:
+            compute_plane = self.compute_plane
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        if compute_plane.lower() in ["both", "transverse"]:
+            self.calc_long_WP_3d()
+            self.calc_trans_WP()
+            self.calc_long_Z()
+            self.calc_trans_Z()
+        elif compute_plane.lower() == "longitudinal":
+            self.calc_long_WP()
+            self.calc_long_Z()
+
+    # -- s axis and time buckets (wakis/wakeSolver.py:446-488, 517-529) ------
+    def _axes(self):
+        nt = len(self.t)
+        dt = self.t[2] - self.t[1]
+        zmax = np.max(self.zf)
+        zmin = np.min(self.zf)
+        if self.skip_cells != 0:
+            zz = slice(self.skip_cells, -self.skip_cells)
+        else:
+            zz = np.s_[:]
+        z = self.zf[zz]
+        if self.wakelength is not None:
+            wakelength = self.wakelength
+        else:
+            wakelength = nt * dt * self.v - (zmax - zmin) - self.ti * self.v
+            self.wakelength = wakelength
+        s = np.arange(-self.ti * self.v, wakelength, dt * self.v)
+        return nt, dt, zmin, z, s
+
+    def bucket_columns(self, ncols_alloc):
+        """cols[k, n]: the E_z time column the reference reads for (s_n, z_k),
+        or -1.  it = int(ts/dt); it >= nt contributes nothing; a negative it
+        indexes from the end of the reference's Ezt array (ncols_alloc
+        columns, only the first nt of which are filled)."""
+        nt_t, dt, zmin, z, s = self._axes()
+        nt = min(nt_t, int(self._meta["nsteps"]))
+        cols = np.full((len(z), len(s)), -1, dtype=np.int32)
+        for k in range(len(z)):
+            ts = (z[k] + s) / self.v - zmin / self.v - self.t[0] + self.ti
+            it = np.trunc(ts / dt).astype(np.int64)
+            c = np.where(it < 0, it + ncols_alloc, it)
+            ok = (it < nt) & (c >= 0) & (c < nt)
+            cols[k, ok] = c[ok]
+        return cols, s, z
+
+    # -- wake potential --------------------------------------------------------
+    def calc_long_WP_3d(self, **kwargs):
+        """wakis/wakeSolver.py:423-551 (9 windows, WP carried over)."""
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        self._need()
+        if self._meta["half"] != 1:
+            raise ValueError("calc_long_WP_3d needs the 3x3 probe window")
+        cols, s, z = self.bucket_columns(len(self.t))
+        wp = self._ctx.wake_potential(np.arange(9), True, cols, self.skip_cells, self.dz,
+                                      self.q * 1e12)
+        i0 = j0 = self.n_transverse_cells
+        self.WP_3d = wp.reshape(2 * i0 + 1, 2 * j0 + 1, len(s))
+        self.s = s
+        self.WP = self.WP_3d[i0, j0, :]
+        self._save_txt("WP.txt", self.s, self.WP, "s [m]", "WP [V/pC]")
+
+    def calc_long_WP(self, Ezt=None, **kwargs):
+        """wakis/wakeSolver.py:311-421 (one window; for a 3x3 history the
+        reference reads window index (2, 2))."""
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        self._need()
+        nt = min(len(self.t), int(self._meta["nsteps"]))
+        cols, s, z = self.bucket_columns(nt)
+        wid = 8 if self._meta["half"] == 1 else 0
+        wp = self._ctx.wake_potential([wid], False, cols, self.skip_cells, self.dz,
+                                      self.q * 1e12)
+        self.s = s
+        self.WP = wp[0]
+        self._save_txt("WP.txt", self.s, self.WP, "s [m]", "WP [V/pC]")
+
+    def calc_trans_WP(self, **kwargs):
+        """Panofsky-Wenzel transverse wake of wakis/wakeSolver.py:553-626:
+        int_WP[..., n] = sum(WP_3d[..., 0:n]) * ds, then a centred difference
+        over the window.  The prefix sums use a running sum (the reference uses
+        numpy's pairwise sum per n; both agree to rounding)."""
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        ds = self.s[2] - self.s[1]
+        i0 = j0 = self.n_transverse_cells
+        cs = np.zeros_like(self.WP_3d)
+        cs[..., 1:] = np.cumsum(self.WP_3d[..., :-1], axis=-1)
+        int_WP = cs * ds
+        dxf = np.diff(self.xf)
+        dyf = np.diff(self.yf)
+        self.WPx = -(int_WP[i0 + 1, j0] - int_WP[i0 - 1, j0]) / (dxf[i0 - 1] + dxf[i0])
+        self.WPy = -(int_WP[i0, j0 + 1] - int_WP[i0, j0 - 1]) / (dyf[j0 - 1] + dyf[j0])
+        self._save_txt("WPx.txt", self.s, self.WPx, "s [m]", "WP [V/pC]")
+        self._save_txt("WPy.txt", self.s, self.WPy, "s [m]", "WP [V/pC]")
+
+    # -- impedance -------------------------------------------------------------
+    def calc_lambdas_analytic(self, **kwargs):
+        """wakis/wakeSolver.py:842-874."""
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        self.lambdas = (1 / (self.sigmaz * np.sqrt(2 * np.pi))
+                        * np.exp(-(self.s**2) / (2 * self.sigmaz**2)))
+        self._save_txt("lambda.txt", self.s, self.lambdas, "s [Hz]", "Charge distribution [C/m]")
+
+    def calc_lambdas(self, **kwargs):
+        """wakis/wakeSolver.py:787-840 (user charge distribution)."""
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        if isinstance(self.chargedist, dict):
+            keys = list(self.chargedist.keys())
+            z, chargedist = self.chargedist[keys[0]], self.chargedist[keys[1]]
+        elif isinstance(self.chargedist, str):
+            d = np.loadtxt(self.chargedist)
+            z, chargedist = d[:, 0], d[:, 1]
+        else:
+            chargedist = self.chargedist
+            if self.z is not None and len(self.z) == len(chargedist):
+                z = self.z
+            elif len(self.zf) == len(chargedist):
+                z = self.zf
+            else:
+                raise ValueError("Dimension error: check input dimensions")
+        self.lambdas = np.interp(self.s, z, chargedist / self.q)
+
+    z = None
+
+    def _dft_setup(self, samples, fmax):
+        ds = np.mean(self.s[1:] - self.s[:-1])
+        fmax = self.fmax if fmax is None else fmax
+        N = int((self.v / ds) // fmax * samples)
+        ffft = np.fft.fftfreq(N, ds / self.v)
+        mask = np.logical_and(ffft >= 0, ffft < fmax)
+        K = int(mask.sum())
+        if K and not mask[:K].all():
+            raise RuntimeError("kept DFT bins are not the leading bins")
+        return ds, N, K, ffft[mask]
+
+    def _ctx_for_dft(self):
+        if self._ctx is None:
+            raise RuntimeError("no device context: run SolverFIT3D.wakesolve first")
+        return self._ctx
+
+    def calc_long_Z(self, samples=1001, fmax=None, **kwargs):
+        """wakis/wakeSolver.py:628-720: Z = -(DFT_N(WP*1e12)*ds) /
+        (DFT_N(lambda*v)*ds) on the bins 0 <= f < fmax."""
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        if self.lambdas is None and self.chargedist is not None:
+            self.calc_lambdas()
+        elif self.lambdas is None:
+            self.calc_lambdas_analytic()
+        ds, N, K, f = self._dft_setup(samples, fmax)
+        Z, X, L = self._ctx_for_dft().impedance(self.WP * 1e12, self.lambdas * self.v, N, K, ds)
+        self.f = f
+        self.Z = Z
+        self.lambdaf = L
+        self.N_dft = N
+        self._save_txt("Z.txt", self.f, self.Z, "f [Hz]", "Z [Ohm]")
+        self._save_txt("spectrum.txt", self.f, self.lambdaf, "f [Hz]",
+                       "Charge distribution spectrum [C/s]")
+
+    def calc_trans_Z(self, samples=1001, fmax=None):
+        """wakis/wakeSolver.py:722-785: Z_x,y = 1j*DFT(WP_x,y*1e12)*ds/lambdaf."""
+        ds, N, K, f = self._dft_setup(samples, fmax)
+        ctx = self._ctx_for_dft()
+        _, Xx, L = ctx.impedance(self.WPx * 1e12, self.lambdas * self.v, N, K, ds)
+        _, Xy, _ = ctx.impedance(self.WPy * 1e12, self.lambdas * self.v, N, K, ds)
+        self.Zx = 1j * Xx / L
+        self.Zy = 1j * Xy / L
+        self.fx = self.fy = f
+        self._save_txt("Zx.txt", self.fx, self.Zx, "f [Hz]", "Zx [Ohm]")
+        self._save_txt("Zy.txt", self.fy, self.Zy, "f [Hz]", "Zy [Ohm]")
+
+    def _save_txt(self, name, a, b, ha, hb):
+        if self.save:
+            np.savetxt(self.folder + name, np.c_[a, b],
+                       header="   " + ha + " " * 20 + hb + "\n" + "-" * 48)

@@ -1,0 +1,90 @@
+"""Synthetic cavity workloads (geometry + materials + beam) used by the
+benchmark and the tests.
+
+Cells are marked with the reference's STL rule: a cell belongs to a solid if
+at least one of its 8 corner nodes lies inside it (wakis/gridFIT3D.py:
+502-510).  Shapes are analytic, so no STL file is needed.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def cells_from_nodes(inside: np.ndarray) -> np.ndarray:
+    """(Nx+1, Ny+1, Nz+1) node flags -> (Nx, Ny, Nz) cells with >= 1 node in."""
+    nx, ny, nz = (s - 1 for s in inside.shape)
+    c = np.zeros((nx, ny, nz), dtype=bool)
+    for a in (0, 1):
+        for b in (0, 1):
+            for d in (0, 1):
+                c |= inside[a:a + nx, b:b + ny, d:d + nz]
+    return c
+
+
+def rect_cavity_mask(x, y, z, half_box, half_len, half_pipe):
+    """Square box |x|,|y| <= half_box, |z| <= half_len, plus a square pipe
+    |x|,|y| <= half_pipe along the whole z range."""
+    bx = (np.abs(x) <= half_box)[:, None, None]
+    by = (np.abs(y) <= half_box)[None, :, None]
+    bz = (np.abs(z) <= half_len)[None, None, :]
+    px = (np.abs(x) <= half_pipe)[:, None, None]
+    py = (np.abs(y) <= half_pipe)[None, :, None]
+    return cells_from_nodes((bx & by & bz) | (px & py & np.ones_like(bz)))
+
+
+def pillbox_mask(x, y, z, radius, half_len, pipe_radius):
+    """Cylinder r <= radius, |z| <= half_len, plus a round pipe r <= pipe_radius."""
+    r2 = x[:, None] ** 2 + y[None, :] ** 2
+    cav = (r2 <= radius**2)[:, :, None] & (np.abs(z) <= half_len)[None, None, :]
+    pipe = (r2 <= pipe_radius**2)[:, :, None] & np.ones((1, 1, len(z)), dtype=bool)
+    return cells_from_nodes(cav | pipe)
+
+
+def pec_cubic_cavity(nx=64, ny=64, nz=192):
+    """PEC box 100x100x100 mm with a 30x30 mm square pipe to z = +-200 mm,
+    vacuum inside, PEC outside; bunch q = 1 nC, sigma_z = 25 mm, beta = 1;
+    wakelength 3 m; cfl 1.0."""
+    x = np.linspace(-0.05, 0.05, nx + 1)
+    y = np.linspace(-0.05, 0.05, ny + 1)
+    z = np.linspace(-0.2, 0.2, nz + 1)
+    mask = rect_cavity_mask(x, y, z, 0.05, 0.05, 0.015)
+    return dict(
+        name=f"pec_cubic_cavity_{nx}x{ny}x{nz}",
+        grid=dict(x=x, y=y, z=z, solids=[(mask, "vacuum")]),
+        solver=dict(bc_low=["pec"] * 3, bc_high=["pec"] * 3, bg="pec", use_stl=True, cfln=1.0),
+        wake=dict(q=1e-9, sigmaz=25e-3, beta=1.0, wakelength=3.0),
+    )
+
+
+def lossy_pillbox(nx=256, ny=256, nz=768):
+    """Pillbox cavity r = 60 mm, length 80 mm, beam pipe r = 20 mm, staircased
+    in a 120 x 120 mm transverse box with dz = dx (z = +-180 mm at 768 cells).
+    Walls are a conductor sigma = 1e5 S/m given as a solid (so, as in the
+    reference's STL path with use_sibc, eps = sigma*eps_0 there); 10-cell PML
+    (the reference's geomspace grading) on both z ends, PEC on x and y;
+    bunch sigma_z = 10 mm, beta = 1, wakelength 5 m."""
+    half_t = 0.06
+    dx = 2 * half_t / nx
+    half_z = dx * nz / 2
+    x = np.linspace(-half_t, half_t, nx + 1)
+    y = np.linspace(-half_t, half_t, ny + 1)
+    z = np.linspace(-half_z, half_z, nz + 1)
+    vac = pillbox_mask(x, y, z, 0.06, 0.04, 0.02)
+    return dict(
+        name=f"lossy_pillbox_{nx}x{ny}x{nz}",
+        grid=dict(x=x, y=y, z=z, solids=[(~vac, [1.0, 1.0, 1e5])]),
+        solver=dict(bc_low=["pec", "pec", "pml"], bc_high=["pec", "pec", "pml"], bg="vacuum",
+                    use_stl=True, n_pml=10, cfln=1.0),
+        wake=dict(q=1e-9, sigmaz=10e-3, beta=1.0, wakelength=5.0),
+    )
+
+
+def build(workload: dict, dtype=np.float64, device=0, kernel="auto", save=False):
+    """(grid, wake, solver) of the wakis_amd API for a workload dict."""
+    from . import GridFIT3D, SolverFIT3D, WakeSolver
+
+    g = GridFIT3D(**workload["grid"])
+    w = WakeSolver(save=save, **workload["wake"])
+    s = SolverFIT3D(g, w, dtype=dtype, device=device, kernel=kernel, **workload["solver"])
+    return g, w, s
