@@ -159,9 +159,10 @@ int fit_wake_history(fit_ctx *ctx, void *host, int64_t count);
 int fit_step(fit_ctx *ctx, int32_t nsteps, int32_t conductive);
 /* fit_step timed with HIP events on the context stream (synchronises).
  * per_kernel = 0: ms[0] = elapsed time of the whole launch sequence.
- * per_kernel = 1: an event pair around every launch; ms[1] = summed time of
- * the first kernel of a step (H update, or the fused sweep), ms[2] = summed
- * time of the second kernel (E update; 0 for the fused sweep); ms[0] = total.
+ * per_kernel = 1: events around every launch; ms[1] = summed time of the
+ * first kernel of a step (H update, or the fused interior sweep), ms[2] =
+ * summed time of the second (E update, or the fused sweep's boundary shell);
+ * ms[0] = total.
  * No counterpart in the reference (it has no timing facility beyond
  * time.time() deltas, wakis/routines.py:437, 484). */
 int fit_step_timed(fit_ctx *ctx, int32_t nsteps, int32_t conductive, int32_t per_kernel,

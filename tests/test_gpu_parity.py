@@ -33,13 +33,18 @@ def _solids(shape):
     ]
 
 
+KERNELS = ["split", "fused"]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("name,bl,bh", BC_CASES, ids=[c[0] for c in BC_CASES])
 @pytest.mark.parametrize("nonuniform", [False, True], ids=["uniform", "nonuniform"])
-def test_step_fp64(name, bl, bh, nonuniform):
+def test_step_fp64(name, bl, bh, nonuniform, kernel):
     shape = (7, 6, 11)
-    print(f"[case] step_fp64 {name} nonuniform={nonuniform} shape={shape}", flush=True)
+    print(f"[case] step_fp64 {name} nonuniform={nonuniform} kernel={kernel} shape={shape}",
+          flush=True)
     osv, psv = make_pair(*shape, bl, bh, solids=_solids(shape), nonuniform=nonuniform,
-                         seed=3, n_pml=3, kernel="split")
+                         seed=3, n_pml=3, kernel=kernel)
     for _ in range(25):
         osv.one_step()
         psv.one_step()
@@ -48,12 +53,13 @@ def test_step_fp64(name, bl, bh, nonuniform):
     assert max(err.values()) <= TOL64, err
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("name,bl,bh", BC_CASES[:4], ids=[c[0] for c in BC_CASES[:4]])
-def test_step_fp32(name, bl, bh):
+def test_step_fp32(name, bl, bh, kernel):
     shape = (8, 7, 10)
-    print(f"[case] step_fp32 {name} shape={shape}", flush=True)
+    print(f"[case] step_fp32 {name} kernel={kernel} shape={shape}", flush=True)
     osv, psv = make_pair(*shape, bl, bh, dtype=np.float32, solids=_solids(shape), seed=5,
-                         kernel="split")
+                         kernel=kernel)
     for _ in range(25):
         osv.one_step()
         psv.one_step()
@@ -165,3 +171,35 @@ def test_field_indexing_roundtrip():
     psv.H[:, 0, :, "z"] = np.arange(35.0).reshape(5, 7)
     assert np.array_equal(psv.E.toarray(), host.toarray())
     assert np.array_equal(psv.H[:, 0, :, "z"], np.arange(35.0).reshape(5, 7))
+
+
+@pytest.mark.parametrize("name,bl,bh", BC_CASES, ids=[c[0] for c in BC_CASES])
+def test_fused_multi_tile_fp64(name, bl, bh):
+    """A grid spanning several fused tiles in x, y and z (tile 64 x 8, chunk
+    32): tile seams, z-chunk restarts, XCD remap, the shell."""
+    shape = (131, 19, 70)
+    print(f"[case] fused multi-tile {name} shape={shape}", flush=True)
+    Nx, Ny, Nz = shape
+    solids = [(block_mask(shape, (3, 2, 5), (Nx - 4, Ny - 3, Nz - 6)), "vacuum"),
+              (block_mask(shape, (40, 5, 20), (90, 12, 40)), [3.0, 1.2, 20.0]),
+              (block_mask(shape, (60, 8, 30), (70, 15, 50)), "pec")]
+    osv, psv = make_pair(*shape, bl, bh, solids=solids, seed=21, n_pml=4, kernel="fused")
+    for _ in range(6):
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    print(name, err)
+    assert max(err.values()) <= TOL64, err
+
+
+def test_fused_equals_split_bitwise_structure():
+    """Fused and split kernels agree to rounding on a mid-size grid (FP64)."""
+    shape = (70, 20, 40)
+    print("[case] fused vs split", flush=True)
+    _, a = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, seed=8, kernel="split")
+    _, b = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, seed=8, kernel="fused")
+    for _ in range(20):
+        a.one_step()
+        b.one_step()
+    for n in "EHJ":
+        assert rel(getattr(b, n).toarray(), getattr(a, n).toarray()) <= 1e-13

@@ -23,6 +23,9 @@ def _cavity(shape):
 
 
 WAKE_KW = dict(wakelength=0.15, q=1e-9, sigmaz=0.012, beta=1.0, skip_cells=2)
+# the reference's 1-window path reads the whole E_z line into a skip_cells
+# sliced array and fails for skip_cells != 0 (wakeSolver.py:370-389)
+WAKE_KW_LONG = dict(WAKE_KW, skip_cells=0)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["fp64", "fp32"])
@@ -30,8 +33,9 @@ WAKE_KW = dict(wakelength=0.15, q=1e-9, sigmaz=0.012, beta=1.0, skip_cells=2)
 def test_wakesolve(dtype, plane):
     shape = (14, 13, 36)
     print(f"[case] wakesolve {np.dtype(dtype).name} {plane} shape={shape}", flush=True)
+    kw = WAKE_KW if plane == "both" else WAKE_KW_LONG
     osv, psv = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, dtype=dtype, bg="pec",
-                         solids=_cavity(shape), init=False, cfln=1.0, wake_kw=WAKE_KW)
+                         solids=_cavity(shape), init=False, cfln=1.0, wake_kw=kw)
     osv.wakesolve(compute_plane=plane)
     psv.wakesolve(compute_plane=plane)
     ow, pw = osv.wake, psv.wake
@@ -70,13 +74,24 @@ def test_wake_potential_bit_exact():
     assert np.array_equal(ow.WP_3d, pw.WP_3d)
 
 
+def test_longitudinal_skip_cells_fails_like_reference():
+    shape = (10, 10, 24)
+    print("[case] longitudinal skip_cells", flush=True)
+    osv, psv = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, bg="pec", solids=_cavity(shape),
+                         init=False, cfln=1.0, wake_kw=WAKE_KW)
+    with pytest.raises(ValueError):
+        osv.wakesolve(compute_plane="longitudinal")
+    with pytest.raises(ValueError):
+        psv.wakesolve(compute_plane="longitudinal")
+
+
 def test_wakesolve_callback_path_matches_fast_path():
     shape = (10, 10, 24)
     print("[case] callback vs fast path", flush=True)
     _, a = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, bg="pec", solids=_cavity(shape),
-                     init=False, cfln=1.0, wake_kw=WAKE_KW)
+                     init=False, cfln=1.0, wake_kw=WAKE_KW_LONG)
     _, b = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, bg="pec", solids=_cavity(shape),
-                     init=False, cfln=1.0, wake_kw=WAKE_KW)
+                     init=False, cfln=1.0, wake_kw=WAKE_KW_LONG)
     a.wakesolve(compute_plane="longitudinal")
     seen = []
     b.wakesolve(compute_plane="longitudinal", callback=lambda s, t: seen.append(t))
@@ -94,7 +109,7 @@ def test_conductive_wakesolve_pml_fp64():
     wall = ~block_mask(shape, (2, 2, 0), (Nx - 2, Ny - 2, Nz))
     solids = [(wall, [1.0, 1.0, 1e3])]
     osv, psv = make_pair(*shape, ["pec", "pec", "pml"], ["pec", "pec", "pml"], bg=[1.0, 1.0],
-                         solids=solids, init=False, cfln=0.9, n_pml=5, wake_kw=WAKE_KW)
+                         solids=solids, init=False, cfln=0.9, n_pml=5, wake_kw=WAKE_KW_LONG)
     assert float(psv.dt) == float(osv.dt)
     osv.wakesolve(compute_plane="longitudinal")
     psv.wakesolve(compute_plane="longitudinal")

@@ -123,13 +123,15 @@ __global__ void materialize_j(StepArgs<T> a) {
 
 // Longitudinal wake potential (wakis/wakeSolver.py:492-534): thread n owns
 // sample n and sums over k in order, with separate multiply and add roundings
-// (no contraction) exactly as the reference's WP[n] = WP[n] + Ezt*dz.
+// (no contraction into an FMA) exactly as the reference's WP[n] = WP[n] +
+// Ezt*dz, so that the same history gives the same bits.
 template <typename T>
 __global__ void wake_potential(const T *__restrict__ hist, int64_t hist_nt, int nzh,
                                const int32_t *__restrict__ wids, int nwin, int carry,
                                int ns, int nzw, int k0, const int32_t *__restrict__ cols,
                                const double *__restrict__ dz, double qnorm,
                                double *__restrict__ wp) {
+#pragma clang fp contract(off)
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= ns) return;
   double acc = 0.0;
@@ -140,7 +142,8 @@ __global__ void wake_potential(const T *__restrict__ hist, int64_t hist_nt, int 
       const int c = cols[(int64_t)k * ns + n];
       if (c >= 0) {
         const double e = (double)hw[(int64_t)(k0 + k) * hist_nt + c];
-        acc = __dadd_rn(acc, __dmul_rn(e, dz[k]));
+        const double prod = e * dz[k];
+        acc = acc + prod;
       }
     }
     acc = acc / qnorm;

@@ -42,6 +42,7 @@ struct fit_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   void *E = nullptr, *H = nullptr, *J = nullptr;
+  void *E_alt = nullptr, *H_alt = nullptr;  // second buffers of the fused sweep
   uint8_t *mat = nullptr;
   bool mat_uniform = true;
   void *mtab = nullptr;
@@ -168,16 +169,38 @@ fit::StepArgs<T> make_args(fit_ctx *c) {
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// The fused sweep writes new buffers; allocate them on first use.  With
+// FIT_KERNEL_AUTO a failed allocation selects the split kernels instead.
+int want_fused(fit_ctx *c, bool *fused) {
+  *fused = false;
+  if (!fit::fused_supported(c->kernel, c->nx, c->ny, c->nz)) return 0;
+  if (!c->E_alt || !c->H_alt) {
+    const size_t fb = (size_t)3 * c->N * c->esz;
+    if (!c->E_alt && hipMalloc(&c->E_alt, fb) != hipSuccess) c->E_alt = nullptr;
+    if (!c->H_alt && hipMalloc(&c->H_alt, fb) != hipSuccess) c->H_alt = nullptr;
+    if (!c->E_alt || !c->H_alt) {
+      (void)hipGetLastError();
+      if (c->kernel == FIT_KERNEL_FUSED)
+        return fail(FIT_ENOMEM, "fused sweep: second field buffers do not fit");
+      return 0;
+    }
+  }
+  *fused = true;
+  return 0;
+}
+
 // ev (optional): 3 events per step [before first kernel, between, after].
 template <typename T>
 int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) {
-  const bool fused = fit::fused_supported(c->kernel, c->nx, c->ny, c->nz);
+  bool fused = false;
+  if (int rc = want_fused(c, &fused)) return rc;
   for (int s = 0; s < nsteps; ++s) {
     fit::StepArgs<T> a = make_args<T>(c);
     if (ev) HIPCHK(hipEventRecord(ev[3 * s], c->stream));
     if (fused) {
-      fit::launch_fused<T>(a, c->stream);
-      if (ev) HIPCHK(hipEventRecord(ev[3 * s + 1], c->stream));
+      fit::launch_fused<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, ev ? ev[3 * s + 1] : nullptr);
+      std::swap(c->E, c->E_alt);
+      std::swap(c->H, c->H_alt);
     } else {
       hipLaunchKernelGGL(fit::h_update_split<T>, dim3(nblocks(c->N, 256)), dim3(256), 0,
                          c->stream, a);
@@ -231,7 +254,7 @@ int fit_create(const fit_desc *d, fit_ctx **out) {
   if (d->nx < 2 || d->ny < 2 || d->nz < 2) return fail(FIT_EINVAL, "need nx, ny, nz >= 2");
   if (d->dtype != FIT_F64 && d->dtype != FIT_F32) return fail(FIT_EINVAL, "bad dtype");
   const int64_t N = (int64_t)d->nx * d->ny * d->nz;
-  if (N > (int64_t)1 << 31) return fail(FIT_EINVAL, "grid larger than 2^31 cells");
+  if (N >= (int64_t)1 << 31) return fail(FIT_EINVAL, "grid must have fewer than 2^31 cells");
   fit_ctx *c = new fit_ctx();
   c->nx = d->nx;
   c->ny = d->ny;
@@ -273,7 +296,8 @@ int fit_destroy(fit_ctx *c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void *ps[] = {c->E, c->H, c->J, c->mat, c->mtab, c->geo, c->jline, c->zc, c->hist, c->staging};
+  void *ps[] = {c->E,   c->H,     c->J,  c->E_alt, c->H_alt, c->mat,
+                c->mtab, c->geo, c->jline, c->zc, c->hist, c->staging};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -616,9 +640,14 @@ int fit_get_step_index(fit_ctx *c, int64_t *step) {
 int fit_set_kernel(fit_ctx *c, int32_t kernel) {
   if (!c) return fail(FIT_EINVAL, "null context");
   if (kernel < FIT_KERNEL_AUTO || kernel > FIT_KERNEL_FUSED) return fail(FIT_EINVAL, "bad kernel id");
-  if (kernel == FIT_KERNEL_FUSED && !fit::fused_supported(kernel, c->nx, c->ny, c->nz))
-    return fail(FIT_EINVAL, "fused kernel does not support this grid");
   c->kernel = kernel;
+  if (kernel == FIT_KERNEL_SPLIT) {
+    // the second buffers are no longer needed
+    if (int rc = set_device(c)) return rc;
+    if (c->E_alt) HIPCHK(hipFree(c->E_alt));
+    if (c->H_alt) HIPCHK(hipFree(c->H_alt));
+    c->E_alt = c->H_alt = nullptr;
+  }
   return 0;
 }
 

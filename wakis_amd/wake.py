@@ -151,6 +151,12 @@ class WakeSolver:
         for key, val in kwargs.items():
             setattr(self, key, val)
         self._need()
+        if self._meta["half"] == 0 and self.skip_cells != 0:
+            # the reference copies the whole E_z line into an array of the
+            # skip_cells-sliced length and fails (wakis/wakeSolver.py:370-389)
+            raise ValueError(
+                f"could not broadcast input array from shape ({len(self.zf)},) into shape "
+                f"({len(self.zf) - 2 * self.skip_cells},)")
         nt = min(len(self.t), int(self._meta["nsteps"]))
         cols, s, z = self.bucket_columns(nt)
         wid = 8 if self._meta["half"] == 1 else 0
