@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--grid", default="256,256,768")
     ap.add_argument("--kernel", default="auto", choices=["auto", "split", "fused"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
+    ap.add_argument("--tile", type=int, default=None, help="fused tile configuration id")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -90,6 +92,8 @@ def main():
     wl = lossy_pillbox(nx, ny, nz)
     t0 = time.perf_counter()
     grid, wake, solver = build(wl, dtype=dtype, kernel=args.kernel)
+    if args.tile is not None:
+        solver.ctx.set_tile(args.tile)
     setup_s = time.perf_counter() - t0
     ctx = solver.ctx
     N = solver.N
@@ -108,8 +112,12 @@ def main():
     solver.step_0 = False
     cond = solver.use_conductivity
 
-    print(f"[bench] {wl['name']} {args.dtype} N={N} setup {setup_s:.1f}s warmup {args.warmup}",
-          file=sys.stderr, flush=True)
+    tune_ms = None
+    if not args.no_tune and args.tile is None and solver.ctx.kernel_in_use() == 2:
+        tune_ms = solver.tune(reps=3)
+    tile = ctx.get_tile()
+    print(f"[bench] {wl['name']} {args.dtype} N={N} setup {setup_s:.1f}s warmup {args.warmup} "
+          f"tile {tile} tune {tune_ms}", file=sys.stderr, flush=True)
     ctx.step(args.warmup, cond)
     ctx.sync()
     wall0 = time.perf_counter()
@@ -121,18 +129,27 @@ def main():
           file=sys.stderr, flush=True)
 
     # per-kernel durations for the roofline of the dominant kernel
+    from wakis_amd import _lib
+
+    fused = ctx.kernel_in_use() == _lib.KERNEL_FUSED
     nk = min(args.steps, 100)
     _, k1, k2 = ctx.step_timed(nk, cond, per_kernel=True)
-    fused = k2 == 0.0
     if fused:
-        kname, kms, words = "fit_fused_sweep", k1 / nk, 12
+        # one launch per step: E and H read once and written once per cell
+        kname, kms, alg_bytes = "fused_sweep", k1 / nk, 12 * esz * N
+        extra = {}
     elif k2 >= k1:
-        kname, kms, words = "e_update_split", k2 / nk, 9
+        # E update: reads H and E, writes E (9 field words per cell)
+        kname, kms, alg_bytes = "e_update_split", k2 / nk, 9 * esz * N
+        extra = {"h_update_avg_ms": round(k1 / nk, 5)}
     else:
-        kname, kms, words = "h_update_split", k1 / nk, 9
-    alg_bytes = words * esz * N
+        kname, kms, alg_bytes = "h_update_split", k1 / nk, 9 * esz * N
+        extra = {"e_update_avg_ms": round(k2 / nk, 5)}
     achieved = alg_bytes / (kms * 1e-3) / 1e9
     step_alg = 12 * esz * N / (ms_per_step * 1e-3) / 1e9
+    copy_bytes = 2 * 12 * esz * N  # same footprint as the fields (read + write)
+    copy_ms = _lib.copy_bench(copy_bytes // 2, reps=10)
+    copy_gbs = copy_bytes / (copy_ms * 1e-3) / 1e9
 
     out = {
         "metric": "Gcell-steps/s",
@@ -152,7 +169,10 @@ def main():
                         "10-cell z PML, wakesolve inner loop (beam + E_z probe + step)",
             "grid": [nx, ny, nz],
             "kernel": "fused" if fused else "split",
+            "tile": {"cfg": tile[0], "tx": tile[1], "ty": tile[2], "kc": tile[3]} if fused else None,
+            "step_alg_gbs": round(step_alg, 1),
             "step_hbm_frac": round(step_alg / HBM_PEAK_GBS, 4),
+            "copy_kernel_gbs": round(copy_gbs, 1),
         },
         "roofline": {
             "bound": "hbm",
@@ -164,6 +184,7 @@ def main():
             "traffic": None,
             "alg_bytes_per_launch": alg_bytes,
             "avg_launch_ms": round(kms, 5),
+            **extra,
         },
     }
     if not args.no_cpu:

@@ -160,18 +160,35 @@ int fit_step(fit_ctx *ctx, int32_t nsteps, int32_t conductive);
 /* fit_step timed with HIP events on the context stream (synchronises).
  * per_kernel = 0: ms[0] = elapsed time of the whole launch sequence.
  * per_kernel = 1: events around every launch; ms[1] = summed time of the
- * first kernel of a step (H update, or the fused interior sweep), ms[2] =
- * summed time of the second (E update, or the fused sweep's boundary shell);
- * ms[0] = total.
+ * first kernel of a step (H update, or the fused sweep), ms[2] = summed time
+ * of the second (E update; 0 for the fused sweep); ms[0] = total.
  * No counterpart in the reference (it has no timing facility beyond
  * time.time() deltas, wakis/routines.py:437, 484). */
 int fit_step_timed(fit_ctx *ctx, int32_t nsteps, int32_t conductive, int32_t per_kernel,
                    double *ms);
+/* Plain device-to-device streaming copy of nbytes (16-byte vectors), timed
+ * over reps launches: *ms = average milliseconds per launch.  Calibrates the
+ * achievable HBM bandwidth and the rocprof FETCH/WRITE counters.  No
+ * counterpart in the reference. */
+int fit_copy_bench(int32_t device, int64_t nbytes, int32_t reps, double *ms);
 /* Step counter used by the beam time and the probe column. */
 int fit_set_step_index(fit_ctx *ctx, int64_t step);
 int fit_get_step_index(fit_ctx *ctx, int64_t *step);
 /* Select the step kernel (FIT_KERNEL_*). */
 int fit_set_kernel(fit_ctx *ctx, int32_t kernel);
+/* Tile configuration of the fused sweep (0 .. FIT_NTILE-1; -1 = default
+ * for the grid); fit_get_tile reports the one in use and its tile sizes. */
+enum { FIT_NTILE = 6 };
+int fit_set_tile(fit_ctx *ctx, int32_t cfg);
+int fit_get_tile(fit_ctx *ctx, int32_t *cfg, int32_t *tx, int32_t *ty, int32_t *kc);
+/* Autotuner: time every tile configuration of the fused sweep on this
+ * context's grid and data (reps launches each; the state is not changed)
+ * and select the fastest; ms[FIT_NTILE] receives the average times. */
+int fit_tune(fit_ctx *ctx, int32_t reps, double *ms);
+/* The step kernel the next fit_step will use (FIT_KERNEL_SPLIT or
+ * FIT_KERNEL_FUSED; AUTO resolves to FUSED when its second field buffers
+ * fit in device memory). */
+int fit_kernel_in_use(fit_ctx *ctx, int32_t *kernel);
 int fit_sync(fit_ctx *ctx);
 
 /* Longitudinal wake potential on the device (wakis/wakeSolver.py:423-551,

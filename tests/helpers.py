@@ -28,7 +28,7 @@ def block_mask(shape, lo, hi):
 
 def make_pair(Nx, Ny, Nz, bc_low, bc_high, dtype=np.float64, bg=(1.0, 1.0), solids=(),
               n_pml=3, cfln=0.5, nonuniform=False, seed=0, init=True, wake_kw=None,
-              kernel="auto"):
+              kernel="auto", tile=None):
     import wakis_amd as wa
 
     x, y, z = axes(Nx, Ny, Nz, nonuniform, seed)
@@ -39,7 +39,8 @@ def make_pair(Nx, Ny, Nz, bc_low, bc_high, dtype=np.float64, bg=(1.0, 1.0), soli
                   solids=list(solids), bg=bg, dtype=dtype, n_pml=n_pml)
     g = wa.GridFIT3D(x=x, y=y, z=z, solids=list(solids))
     psv = wa.SolverFIT3D(g, pwake, cfln=cfln, bc_low=bc_low, bc_high=bc_high,
-                         use_stl=bool(solids), bg=bg, dtype=dtype, n_pml=n_pml, kernel=kernel)
+                         use_stl=bool(solids), bg=bg, dtype=dtype, n_pml=n_pml, kernel=kernel,
+                         tile=tile)
     if init:
         rng = np.random.default_rng(seed + 100)
         for name in ("E", "H", "J"):

@@ -173,17 +173,21 @@ def test_field_indexing_roundtrip():
     assert np.array_equal(psv.H[:, 0, :, "z"], np.arange(35.0).reshape(5, 7))
 
 
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("name,bl,bh", BC_CASES, ids=[c[0] for c in BC_CASES])
-def test_fused_multi_tile_fp64(name, bl, bh):
-    """A grid spanning several fused tiles in x, y and z (tile 64 x 8, chunk
-    32): tile seams, z-chunk restarts, XCD remap, the shell."""
+def test_fused_multi_tile_fp64(name, bl, bh, tile):
+    """A grid spanning several fused tiles in x, y and z for every tile
+    configuration: tile seams, z-chunk restarts, partial tiles, the XCD
+    remap, and the flat-window wrap at every face."""
     shape = (131, 19, 70)
-    print(f"[case] fused multi-tile {name} shape={shape}", flush=True)
+    print(f"[case] fused multi-tile {name} tile={tile} shape={shape}", flush=True)
     Nx, Ny, Nz = shape
     solids = [(block_mask(shape, (3, 2, 5), (Nx - 4, Ny - 3, Nz - 6)), "vacuum"),
               (block_mask(shape, (40, 5, 20), (90, 12, 40)), [3.0, 1.2, 20.0]),
               (block_mask(shape, (60, 8, 30), (70, 15, 50)), "pec")]
-    osv, psv = make_pair(*shape, bl, bh, solids=solids, seed=21, n_pml=4, kernel="fused")
+    # tile 3 (256 x 4) does not fit the LDS in FP64: the default tile runs
+    osv, psv = make_pair(*shape, bl, bh, solids=solids, seed=21, n_pml=4, kernel="fused",
+                         tile=None if tile == 3 else tile)
     for _ in range(6):
         osv.one_step()
         psv.one_step()
@@ -203,3 +207,47 @@ def test_fused_equals_split_bitwise_structure():
         b.one_step()
     for n in "EHJ":
         assert rel(getattr(b, n).toarray(), getattr(a, n).toarray()) <= 1e-13
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
+def test_fused_tiles_fp32(tile):
+    """Every tile configuration in FP32 (256x4 fits the LDS only in FP32)."""
+    shape = (140, 21, 40)
+    print(f"[case] fused tiles fp32 tile={tile}", flush=True)
+    osv, psv = make_pair(*shape, ["periodic", "pec", "pml"], ["pec", "periodic", "pml"],
+                         dtype=np.float32, solids=_solids(shape), seed=31, kernel="fused",
+                         tile=tile)
+    for _ in range(8):
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    assert max(err.values()) <= TOL32, err
+
+
+def test_tiny_grids_fused():
+    """Grids smaller than one tile in every direction and with 2 cells on an
+    axis (the window wraps several rows)."""
+    for shape in [(2, 3, 4), (5, 2, 3), (3, 4, 2), (9, 7, 5)]:
+        print(f"[case] tiny {shape}", flush=True)
+        osv, psv = make_pair(*shape, ["periodic", "pec", "pec"], ["periodic", "pmc", "pec"],
+                             seed=41, kernel="fused")
+        for _ in range(6):
+            osv.one_step()
+            psv.one_step()
+        err = compare_fields(osv, psv)
+        assert max(err.values()) <= TOL64, (shape, err)
+
+
+def test_tune_keeps_state():
+    shape = (70, 20, 40)
+    print("[case] tune", flush=True)
+    osv, psv = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, seed=8, kernel="fused")
+    before = psv.E.toarray().copy()
+    ms = psv.tune(reps=2)
+    assert len(ms) >= 5 and all(v > 0 for v in ms.values())
+    assert np.array_equal(psv.E.toarray(), before)
+    for _ in range(5):
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    assert max(err.values()) <= TOL64, err

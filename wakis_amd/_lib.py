@@ -19,6 +19,7 @@ FIELD_E, FIELD_H, FIELD_J = 0, 1, 2
 J_ZERO, J_SIGMA_E, J_BUFFER = 0, 1, 2
 BEAM_OFF, BEAM_VALUES, BEAM_GAUSS = 0, 1, 2
 KERNEL_AUTO, KERNEL_SPLIT, KERNEL_FUSED = 0, 1, 2
+NTILE = 6
 
 
 class FitError(RuntimeError):
@@ -67,9 +68,14 @@ SIGNATURES = {
     "fit_wake_history": [_P, _P, _I64],
     "fit_step": [_P, _I32, _I32],
     "fit_step_timed": [_P, _I32, _I32, _I32, _PD],
+    "fit_copy_bench": [_I32, _I64, _I32, _PD],
     "fit_set_step_index": [_P, _I64],
     "fit_get_step_index": [_P, C.POINTER(_I64)],
     "fit_set_kernel": [_P, _I32],
+    "fit_kernel_in_use": [_P, _PI32],
+    "fit_set_tile": [_P, _I32],
+    "fit_get_tile": [_P, _PI32, _PI32, _PI32, _PI32],
+    "fit_tune": [_P, _I32, _PD],
     "fit_sync": [_P],
     "fit_wake_potential": [_P, _I32, _PI32, _I32, _I32, _I32, _I32, _PI32, _PD, _D, _PD],
     "fit_impedance": [_P, _PD, _PD, _I64, _I64, _I32, _D, _PD, _PD, _PD, _PD, _PD, _PD],
@@ -118,6 +124,13 @@ def iptr(a: np.ndarray):
 def vptr(a: np.ndarray):
     assert a.flags.c_contiguous
     return C.c_void_p(a.ctypes.data)
+
+
+def copy_bench(nbytes: int, reps: int = 20, device: int = 0) -> float:
+    """Average ms of a plain nbytes device copy (reads + writes nbytes each)."""
+    out = np.zeros(1, dtype=np.float64)
+    check(load().fit_copy_bench(int(device), int(nbytes), int(reps), dptr(out)), "fit_copy_bench")
+    return float(out[0])
 
 
 class Context:
@@ -262,6 +275,26 @@ class Context:
 
     def set_kernel(self, k: int):
         check(self.lib.fit_set_kernel(self.h, k), "fit_set_kernel")
+
+    def kernel_in_use(self) -> int:
+        v = C.c_int32()
+        check(self.lib.fit_kernel_in_use(self.h, C.byref(v)), "fit_kernel_in_use")
+        return v.value
+
+    def set_tile(self, cfg: int):
+        check(self.lib.fit_set_tile(self.h, int(cfg)), "fit_set_tile")
+
+    def get_tile(self):
+        v = [C.c_int32() for _ in range(4)]
+        check(self.lib.fit_get_tile(self.h, *[C.byref(x) for x in v]), "fit_get_tile")
+        return tuple(x.value for x in v)
+
+    def tune(self, reps: int = 5):
+        """Time every fused tile configuration, select the fastest; returns
+        the per-configuration average ms."""
+        out = np.zeros(NTILE, dtype=np.float64)
+        check(self.lib.fit_tune(self.h, int(reps), dptr(out)), "fit_tune")
+        return out
 
     def sync(self):
         check(self.lib.fit_sync(self.h), "fit_sync")

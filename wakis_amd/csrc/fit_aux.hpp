@@ -97,6 +97,18 @@ __global__ void abc_low(T *E, T *H, int axis, int nx, int ny, int nz, int64_t N)
   }
 }
 
+// Beam line of steps step0 .. step0+nsteps-1 (wakis/sources.py:96-110),
+// evaluated once per fit_step call so the step kernels only read a table.
+template <typename T>
+__global__ void beam_table(StepArgs<T> a, int64_t step0, int nsteps, T *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nsteps * a.nz) return;
+  const int s = (int)(t / a.nz);
+  const int k = (int)(t - (int64_t)s * a.nz);
+  a.step = step0 + s;
+  out[t] = beam_gauss<T>(a, k);
+}
+
 // J := sigma*E (jmode J_SIGMA_E) or 0 (J_ZERO), then the beam line.
 template <typename T>
 __global__ void materialize_j(StepArgs<T> a) {

@@ -65,6 +65,9 @@ struct fit_ctx {
   void *staging = nullptr;
   size_t staging_bytes = 0;
   int kernel = FIT_KERNEL_AUTO;
+  int fused_cfg = -1;  // tile configuration of the fused sweep (-1: default)
+  void *btab = nullptr;  // beam line table of the steps of one fit_step call
+  int64_t btab_steps = 0;
 };
 
 namespace {
@@ -119,6 +122,7 @@ fit::StepArgs<T> make_args(fit_ctx *c) {
   a.J = (const T *)c->J;
   a.mat = c->mat_uniform ? nullptr : c->mat;
   a.mtab = (const T *)c->mtab;
+  a.nmat = c->nmat;
   const T *g = (const T *)c->geo;
   const int nx = c->nx, ny = c->ny, nz = c->nz;
   a.g.Lx = g;
@@ -194,19 +198,42 @@ template <typename T>
 int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) {
   bool fused = false;
   if (int rc = want_fused(c, &fused)) return rc;
+  // the Gaussian bunch of all steps of this call, evaluated once
+  const bool gauss = c->beam_mode == FIT_BEAM_GAUSS;
+  if (gauss && nsteps > 0) {
+    if (c->btab_steps < nsteps) {
+      if (c->btab) HIPCHK(hipFree(c->btab));
+      c->btab = nullptr;
+      c->btab_steps = 0;
+      if (hipMalloc(&c->btab, (size_t)nsteps * c->nz * c->esz) != hipSuccess)
+        return fail(FIT_ENOMEM, "beam table allocation failed");
+      c->btab_steps = nsteps;
+    }
+    fit::StepArgs<T> a = make_args<T>(c);
+    const int64_t cnt = (int64_t)nsteps * c->nz;
+    hipLaunchKernelGGL(fit::beam_table<T>, dim3(nblocks(cnt, 256)), dim3(256), 0, c->stream, a,
+                       c->step, nsteps, (T *)c->btab);
+    HIPCHK(hipGetLastError());
+  }
   for (int s = 0; s < nsteps; ++s) {
     fit::StepArgs<T> a = make_args<T>(c);
+    if (gauss) {
+      a.beam_mode = FIT_BEAM_VALUES;
+      a.jline = (const T *)c->btab + (int64_t)s * c->nz;
+    }
     if (ev) HIPCHK(hipEventRecord(ev[3 * s], c->stream));
     if (fused) {
-      fit::launch_fused<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, ev ? ev[3 * s + 1] : nullptr);
+      const int cfg = c->fused_cfg >= 0 ? c->fused_cfg : fit::fused_default(c->nx, c->dtype == FIT_F64);
+      fit::launch_fused<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg);
       std::swap(c->E, c->E_alt);
       std::swap(c->H, c->H_alt);
-    } else {
-      hipLaunchKernelGGL(fit::h_update_split<T>, dim3(nblocks(c->N, 256)), dim3(256), 0,
-                         c->stream, a);
       if (ev) HIPCHK(hipEventRecord(ev[3 * s + 1], c->stream));
-      hipLaunchKernelGGL(fit::e_update_split<T>, dim3(nblocks(c->N, 256)), dim3(256), 0,
-                         c->stream, a);
+    } else {
+      const dim3 grid(nblocks(c->nx, fit::SPLIT_BX), nblocks(c->ny, fit::SPLIT_BY), c->nz);
+      const dim3 block(fit::SPLIT_BX, fit::SPLIT_BY);
+      hipLaunchKernelGGL(fit::h_update_split<T>, grid, block, 0, c->stream, a);
+      if (ev) HIPCHK(hipEventRecord(ev[3 * s + 1], c->stream));
+      hipLaunchKernelGGL(fit::e_update_split<T>, grid, block, 0, c->stream, a);
     }
     if (ev) HIPCHK(hipEventRecord(ev[3 * s + 2], c->stream));
     HIPCHK(hipGetLastError());
@@ -241,6 +268,46 @@ int check_box(fit_ctx *c, int which, int comp, const int32_t *b) {
   return 0;
 }
 
+template <typename T>
+int tune_impl(fit_ctx *c, int reps, double *ms_out) {
+  // The sweep reads E, H and writes the second buffers, so timing launches
+  // without swapping leaves the solver state unchanged.
+  fit::StepArgs<T> a = make_args<T>(c);
+  a.hist = nullptr;  // no probe writes from tuning launches
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int best = -1;
+  double best_ms = 1e300;
+  int rc = 0;
+  if (a.beam_mode == FIT_BEAM_GAUSS) a.beam_mode = FIT_BEAM_OFF;  // timing only
+  for (int cfg = 0; cfg < fit::FUSED_NCFG && !rc; ++cfg) {
+    if (!fit::fused_cfg_ok<T>(cfg)) {
+      if (ms_out) ms_out[cfg] = -1.0;
+      continue;
+    }
+    fit::launch_fused<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg);
+    if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(FIT_EHIP, "event");
+    for (int r = 0; r < reps && !rc; ++r)
+      fit::launch_fused<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg);
+    float t = 0;
+    if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+      rc = fail(FIT_EHIP, "tuning timing");
+    if (hipGetLastError() != hipSuccess && !rc) rc = fail(FIT_EHIP, "tuning launch");
+    const double ms = (double)t / reps;
+    if (ms_out) ms_out[cfg] = ms;
+    if (!rc && ms < best_ms) {
+      best_ms = ms;
+      best = cfg;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!rc) c->fused_cfg = best;
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -255,6 +322,7 @@ int fit_create(const fit_desc *d, fit_ctx **out) {
   if (d->dtype != FIT_F64 && d->dtype != FIT_F32) return fail(FIT_EINVAL, "bad dtype");
   const int64_t N = (int64_t)d->nx * d->ny * d->nz;
   if (N >= (int64_t)1 << 31) return fail(FIT_EINVAL, "grid must have fewer than 2^31 cells");
+  if (d->nz > 65535) return fail(FIT_EINVAL, "nz must be <= 65535");
   fit_ctx *c = new fit_ctx();
   c->nx = d->nx;
   c->ny = d->ny;
@@ -296,8 +364,8 @@ int fit_destroy(fit_ctx *c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void *ps[] = {c->E,   c->H,     c->J,  c->E_alt, c->H_alt, c->mat,
-                c->mtab, c->geo, c->jline, c->zc, c->hist, c->staging};
+  void *ps[] = {c->E,    c->H,   c->J,     c->E_alt, c->H_alt, c->mat,   c->mtab,
+                c->geo,  c->jline, c->zc, c->hist,  c->staging, c->btab};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -624,6 +692,42 @@ int fit_step_timed(fit_ctx *c, int32_t nsteps, int32_t conductive, int32_t per_k
   return rc;
 }
 
+int fit_copy_bench(int32_t device, int64_t nbytes, int32_t reps, double *ms) {
+  if (!ms || nbytes < 16 || reps < 1) return fail(FIT_EINVAL, "bad copy benchmark arguments");
+  HIPCHK(hipSetDevice(device));
+  const int64_t n4 = nbytes / 16;
+  float4 *src = nullptr, *dst = nullptr;
+  if (hipMalloc(&src, n4 * 16) != hipSuccess || hipMalloc(&dst, n4 * 16) != hipSuccess) {
+    (void)hipFree(src);
+    return fail(FIT_ENOMEM, "copy benchmark buffers");
+  }
+  hipStream_t st;
+  hipEvent_t e0, e1;
+  int rc = 0;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = fail(FIT_EHIP, "stream");
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(FIT_EHIP, "events");
+  if (!rc) {
+    const unsigned blocks = 256 * 8 * 4;
+    (void)hipMemsetAsync(src, 0, n4 * 16, st);
+    hipLaunchKernelGGL(fit::stream_copy, dim3(blocks), dim3(256), 0, st, src, dst, n4);
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(fit::stream_copy, dim3(blocks), dim3(256), 0, st, src, dst, n4);
+    (void)hipEventRecord(e1, st);
+    float t = 0;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+      rc = fail(FIT_EHIP, "copy benchmark timing");
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  return rc;
+}
+
 int fit_set_step_index(fit_ctx *c, int64_t step) {
   if (!c) return fail(FIT_EINVAL, "null context");
   if (step < 0) return fail(FIT_EINVAL, "step < 0");
@@ -648,6 +752,45 @@ int fit_set_kernel(fit_ctx *c, int32_t kernel) {
     if (c->H_alt) HIPCHK(hipFree(c->H_alt));
     c->E_alt = c->H_alt = nullptr;
   }
+  return 0;
+}
+
+int fit_set_tile(fit_ctx *c, int32_t cfg) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (cfg < -1 || cfg >= fit::FUSED_NCFG) return fail(FIT_EINVAL, "bad tile configuration");
+  if (cfg >= 0 && !(c->dtype == FIT_F64 ? fit::fused_cfg_ok<double>(cfg) : fit::fused_cfg_ok<float>(cfg)))
+    return fail(FIT_EINVAL, "tile configuration does not fit the LDS for this dtype");
+  c->fused_cfg = cfg;
+  return 0;
+}
+
+int fit_get_tile(fit_ctx *c, int32_t *cfg, int32_t *tx, int32_t *ty, int32_t *kc) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  const int k = c->fused_cfg >= 0 ? c->fused_cfg : fit::fused_default(c->nx, c->dtype == FIT_F64);
+  int a = 0, b = 0, d = 0;
+  fit::fused_dims(k, a, b, d);
+  if (cfg) *cfg = k;
+  if (tx) *tx = a;
+  if (ty) *ty = b;
+  if (kc) *kc = d;
+  return 0;
+}
+
+int fit_tune(fit_ctx *c, int32_t reps, double *ms) {
+  if (int rc = check_ready(c)) return rc;
+  if (reps < 1) return fail(FIT_EINVAL, "reps < 1");
+  bool fused = false;
+  if (int rc = want_fused(c, &fused)) return rc;
+  if (!fused) return fail(FIT_ESTATE, "the fused sweep is not in use");
+  if (int rc = set_device(c)) return rc;
+  return c->dtype == FIT_F64 ? tune_impl<double>(c, reps, ms) : tune_impl<float>(c, reps, ms);
+}
+
+int fit_kernel_in_use(fit_ctx *c, int32_t *kernel) {
+  if (!c || !kernel) return fail(FIT_EINVAL, "null argument");
+  bool fused = false;
+  if (int rc = want_fused(c, &fused)) return rc;
+  *kernel = fused ? FIT_KERNEL_FUSED : FIT_KERNEL_SPLIT;
   return 0;
 }
 

@@ -25,6 +25,7 @@ enum { BEAM_OFF = 0, BEAM_VALUES = 1, BEAM_GAUSS = 2 };
 
 // Material table row: dt/mu x,y,z | dt/eps x,y,z | sigma x,y,z.
 constexpr int MAT_STRIDE = 9;
+constexpr int MAX_MATERIALS = 256;
 
 template <typename T>
 struct Geo {
@@ -41,6 +42,7 @@ struct StepArgs {
   const T *J;      // 3N (J_BUFFER only)
   const uint8_t *mat;  // N material ids (nullptr: all 0)
   const T *mtab;   // nmat * MAT_STRIDE
+  int nmat;
   Geo<T> g;
   int nx, ny, nz;
   int64_t N;
@@ -147,6 +149,19 @@ template <typename T>
 __device__ __forceinline__ T w_at(const StepArgs<T> &a, int comp, int i, int j, int k) {
   if (masked(a.pmc, comp, i, j, k, a.nx, a.ny, a.nz)) return T(0);
   return tLof(a.g, comp, i, j, k) * a.H[comp * a.N + flat(i, j, k, a.nx, a.ny)];
+}
+
+// Wake probe: E_z of the state before the update of this step
+// (wakis/routines.py:438-448) into history column `step`.
+template <typename T>
+__device__ __forceinline__ void probe(const StepArgs<T> &a, int i, int j, int k, T ez) {
+  if (a.hist && a.step < a.hist_nt) {
+    const int di = i - a.ixt, dj = j - a.iyt;
+    if (di >= -a.hw && di <= a.hw && dj >= -a.hw && dj <= a.hw) {
+      const int w = (di + a.hw) * (2 * a.hw + 1) + (dj + a.hw);
+      a.hist[((int64_t)w * a.nz + k) * a.hist_nt + a.step] = ez;
+    }
+  }
 }
 
 // Beam current J_z on the beam line at plane k (wakis/sources.py:96-110).

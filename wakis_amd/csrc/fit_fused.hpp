@@ -1,293 +1,135 @@
 // Fused single-sweep leapfrog step: one launch reads E, H (old buffers) and
-// writes E, H (new buffers) once per cell.
+// writes E, H (new buffers), every field word once per cell.
 //
-// Interior cells (1 <= i <= nx-2, 1 <= j <= ny-2, 1 <= k <= nz-2) are swept
-// by z-marching workgroups: a workgroup owns a TX x TY column of cells and a
-// chunk of KC planes; per plane it stages E_old of the plane above into LDS
-// (tile plus a one-cell halo), computes H_new for the tile and for its low
-// x/y halo (so no other workgroup's result is needed), keeps H_new of the
-// previous plane in registers, and finishes E_new of the plane.  Neighbours
-// of interior cells never wrap, so the interior path uses plain 3-D indexing;
-// the face masks and the metric tables are still looked up per cell.
+// A workgroup owns a TX x TY tile of (i, j) columns and a chunk of KC planes
+// and marches up in z.  It stages a window of E_old in LDS: window cell
+// (gi, gj) of plane k, gi in [x0-1, x0+TX], gj in [y0-1, y0+TY], holds the
+// field at FLAT index gi + nx*gj + nx*ny*k (0 outside [0, N)).  Because the
+// reference's curl is built from flat-index shift matrices
+// (wakis/solverFIT3D.py:207-227), every stencil neighbour n+-1, n+-nx,
+// n+-nx*ny of a cell is the adjacent window cell: the helical x/y wrap and the
+// dropped neighbours beyond the array ends come out exactly, for every cell
+// of the grid, boundary or not.  Each staged value is stored pre-scaled with
+// the metric and boundary mask of the cell it really is (u = L*pec*E), so the
+// stencil arithmetic is uniform.
 //
-// Shell cells (any index on the first or last plane of an axis) take the
-// generic path: H_new and the E_new of the cell are recomputed from the old
-// buffers with the flat-index neighbour rule (helical wrap, dropped z
-// neighbour), recomputing the H_new of the n-1, n-nx, n-nx*ny neighbours
-// that the E update reads.  Both paths read only old buffers and write only
-// new buffers, so they need no ordering between them.
-//
-// Blocks are remapped so that consecutive tiles land on the same XCD (the
-// hardware dispatches block b to XCD b % 8), keeping halo re-reads in one L2.
-// The interior sweep and the shell are two launches on the same stream (the
-// shell's generic recomputation needs many more registers; a separate kernel
-// keeps the interior sweep's occupancy).
+// Per plane k (one barrier per plane):
+//   1. scale and store the prefetched E_old plane k+2 into the LDS ring
+//      (4 slots; the slot overwritten held plane k-2, last read before the
+//      previous iteration's barrier);
+//   2. issue the global loads of plane k+3 and of H_old / material / E_old
+//      of the next planes (consumed one iteration later);
+//   3. barrier;
+//   4. E_new(k) from E_old(k), w = tL*pmc*H_new of plane k (LDS: tile plus
+//      the low x/y halo, which the workgroup recomputes itself) and of plane
+//      k-1 (registers);
+//   5. H_new(k+1) of the tile and its low halo into the other w buffer.
+// Blocks are remapped so that consecutive tiles share an XCD (the hardware
+// dispatches block b to XCD b % 8), keeping halo re-reads in one L2.
 #pragma once
 
 #include "fit_common.hpp"
 
 namespace fit {
 
-constexpr int FUSED_TX = 64;
-constexpr int FUSED_TY = 8;
-constexpr int FUSED_KC = 32;
-constexpr int FUSED_THREADS = FUSED_TX * FUSED_TY;
-constexpr int SHELL_THREADS = 256;
-
 template <typename T>
 struct FusedArgs {
-  StepArgs<T> a;   // a.E, a.H: old buffers
-  T *En, *Hn;      // new buffers
+  StepArgs<T> a;  // a.E, a.H: old buffers
+  T *En, *Hn;     // new buffers
   int ntx, nty, nkc;
-  int nb_interior;
-  int64_t nshell;
+  int nb;
 };
 
-// ---- generic (flat-index) path -------------------------------------------
-
-template <typename T>
-__device__ __forceinline__ T h_new_generic(const StepArgs<T> &a, int comp, int i, int j, int k) {
-  const int nx = a.nx, ny = a.ny, nz = a.nz;
-  // per component: the two E components and the shift axes of C's row
-  //   x: u_y[n] - u_y[n+sz] - u_z[n] + u_z[n+sy]
-  //   y: u_z[n] - u_z[n+1]  - u_x[n] + u_x[n+sz]
-  //   z: u_x[n] - u_x[n+sy] - u_y[n] + u_y[n+1]
-  const int ca = comp == 0 ? 1 : (comp == 1 ? 2 : 0);
-  const int cb = comp == 0 ? 2 : (comp == 1 ? 0 : 1);
-  const int sa = comp == 0 ? 2 : (comp == 1 ? 0 : 1);  // shift axis of ca
-  const int sb = comp == 0 ? 1 : (comp == 1 ? 2 : 0);  // shift axis of cb
-  const T a0 = u_at(a, ca, i, j, k);
-  const T b0 = u_at(a, cb, i, j, k);
-  T a1 = 0, b1 = 0;
-  {
-    int ii = i, jj = j, kk = k;
-    if (nb_plus(sa, ii, jj, kk, nx, ny, nz)) a1 = u_at(a, ca, ii, jj, kk);
+// Boundary mask of component c at true cell coordinates, split into the part
+// fixed by (i, j) and the z faces (last writer wins: z faces are written
+// last, then y, then x; wakis/boundaries.py:90-100).
+__device__ __forceinline__ bool xy_mask(uint32_t m, int c, int i, int j, int nx, int ny) {
+  if (c != 1) {
+    if (j == ny - 1) return (m & YHI) != 0;
+    if (j == 0) return (m & YLO) != 0;
   }
-  {
-    int ii = i, jj = j, kk = k;
-    if (nb_plus(sb, ii, jj, kk, nx, ny, nz)) b1 = u_at(a, cb, ii, jj, kk);
+  if (c != 0) {
+    if (i == nx - 1) return (m & XHI) != 0;
+    if (i == 0) return (m & XLO) != 0;
   }
-  const int64_t n = flat(i, j, k, nx, ny);
-  const int m = a.mat ? a.mat[n] : 0;
-  T c = T(0);
-  if (!masked(a.pmc, comp, i, j, k, nx, ny, nz)) {
-    const T area = comp == 0 ? a.g.iAx[j + (int64_t)ny * k]
-                             : (comp == 1 ? a.g.iAy[i + (int64_t)nx * k] : a.g.iAz[i + (int64_t)nx * j]);
-    c = a.mtab[m * MAT_STRIDE + comp] * area;
-  }
-  const T curl = ((a0 - a1) - b0) + b1;
-  return a.H[comp * a.N + n] - c * curl;
+  return false;
 }
 
-// w_d of the E update from a freshly computed H_new.
-template <typename T>
-__device__ __forceinline__ T w_new_generic(const StepArgs<T> &a, int comp, int i, int j, int k) {
-  if (masked(a.pmc, comp, i, j, k, a.nx, a.ny, a.nz)) return T(0);
-  return tLof(a.g, comp, i, j, k) * h_new_generic(a, comp, i, j, k);
+__device__ __forceinline__ bool full_mask(uint32_t m, int c, bool xy, int k, int nz) {
+  if (c != 2) {
+    if (k == nz - 1) return (m & ZHI) != 0;
+    if (k == 0) return (m & ZLO) != 0;
+  }
+  return xy;
 }
 
-template <typename T>
-__device__ __forceinline__ T j_of(const StepArgs<T> &a, const T *mt, int comp, int64_t n, T e) {
-  if (a.jmode == J_SIGMA_E) return mt[6 + comp] * e;
-  if (a.jmode == J_BUFFER) return a.J[comp * a.N + n];
-  return T(0);
-}
+template <int TX, int TY>
+struct Tile {
+  static constexpr int NT = TX * TY;
+  static constexpr int RW = TX + 2;
+  static constexpr int RH = TY + 2;
+  static constexpr int RC = RW * RH;
+  static constexpr int RL = (RC + NT - 1) / NT;
+  static constexpr int NRING = 4;
+};
 
-template <typename T>
-__device__ __forceinline__ void probe(const StepArgs<T> &a, int i, int j, int k, T ez) {
-  if (a.hist && a.step < a.hist_nt) {
-    const int di = i - a.ixt, dj = j - a.iyt;
-    if (di >= -a.hw && di <= a.hw && dj >= -a.hw && dj <= a.hw) {
-      const int w = (di + a.hw) * (2 * a.hw + 1) + (dj + a.hw);
-      a.hist[((int64_t)w * a.nz + k) * a.hist_nt + a.step] = ez;
-    }
-  }
-}
-
-template <typename T>
-__device__ void shell_cell(const FusedArgs<T> &f, int i, int j, int k) {
-  const StepArgs<T> &a = f.a;
-  const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int64_t n = flat(i, j, k, nx, ny);
-  const T hx = h_new_generic(a, 0, i, j, k);
-  const T hy = h_new_generic(a, 1, i, j, k);
-  const T hz = h_new_generic(a, 2, i, j, k);
-  f.Hn[n] = hx;
-  f.Hn[a.N + n] = hy;
-  f.Hn[2 * a.N + n] = hz;
-
-  const T wx0 = masked(a.pmc, 0, i, j, k, nx, ny, nz) ? T(0) : a.g.tLx[i] * hx;
-  const T wy0 = masked(a.pmc, 1, i, j, k, nx, ny, nz) ? T(0) : a.g.tLy[j] * hy;
-  const T wz0 = masked(a.pmc, 2, i, j, k, nx, ny, nz) ? T(0) : a.g.tLz[k] * hz;
-  T wy_mx = 0, wz_mx = 0, wx_my = 0, wz_my = 0, wx_mz = 0, wy_mz = 0;
-  {
-    int ii = i, jj = j, kk = k;
-    if (nb_minus(0, ii, jj, kk, nx, ny, nz)) {
-      wy_mx = w_new_generic(a, 1, ii, jj, kk);
-      wz_mx = w_new_generic(a, 2, ii, jj, kk);
-    }
-  }
-  {
-    int ii = i, jj = j, kk = k;
-    if (nb_minus(1, ii, jj, kk, nx, ny, nz)) {
-      wx_my = w_new_generic(a, 0, ii, jj, kk);
-      wz_my = w_new_generic(a, 2, ii, jj, kk);
-    }
-  }
-  {
-    int ii = i, jj = j, kk = k;
-    if (nb_minus(2, ii, jj, kk, nx, ny, nz)) {
-      wx_mz = w_new_generic(a, 0, ii, jj, kk);
-      wy_mz = w_new_generic(a, 1, ii, jj, kk);
-    }
-  }
-  const int m = a.mat ? a.mat[n] : 0;
-  const T *mt = a.mtab + m * MAT_STRIDE;
-  const T ex = a.E[n], ey = a.E[a.N + n], ez = a.E[2 * a.N + n];
-  probe(a, i, j, k, ez);
-  const T jx = j_of(a, mt, 0, n, ex), jy = j_of(a, mt, 1, n, ey);
-  T jz = j_of(a, mt, 2, n, ez);
-  if (a.beam_mode != BEAM_OFF && i == a.ixs && j == a.iys)
-    jz = (a.beam_mode == BEAM_VALUES) ? a.jline[k] : beam_gauss<T>(a, k);
-  const T ax = masked(a.pec, 0, i, j, k, nx, ny, nz) ? T(0) : a.g.itAx[j + (int64_t)ny * k];
-  const T ay = masked(a.pec, 1, i, j, k, nx, ny, nz) ? T(0) : a.g.itAy[i + (int64_t)nx * k];
-  const T az = masked(a.pec, 2, i, j, k, nx, ny, nz) ? T(0) : a.g.itAz[i + (int64_t)nx * j];
-  const T cx = ((wz0 - wz_my) - wy0) + wy_mz;
-  const T cy = ((wx0 - wx_mz) - wz0) + wz_mx;
-  const T cz = ((wy0 - wy_mx) - wx0) + wx_my;
-  f.En[n] = ex + mt[3] * (ax * cx - jx);
-  f.En[a.N + n] = ey + mt[4] * (ay * cy - jy);
-  f.En[2 * a.N + n] = ez + mt[5] * (az * cz - jz);
-}
-
-// shell index -> (i, j, k): the planes k = 0 and k = nz-1, then for every
-// inner plane the ring j = 0, j = ny-1, i = 0, i = nx-1.
-__device__ __forceinline__ bool shell_ijk(int64_t t, int nx, int ny, int nz, int &i, int &j,
-                                          int &k) {
-  const int64_t plane = (int64_t)nx * ny;
-  if (t < 2 * plane) {
-    k = t < plane ? 0 : nz - 1;
-    const int64_t r = t < plane ? t : t - plane;
-    j = (int)(r / nx);
-    i = (int)(r - (int64_t)j * nx);
-    return true;
-  }
-  const int64_t ring = 2 * (int64_t)nx + 2 * (int64_t)(ny - 2);
-  const int64_t r = t - 2 * plane;
-  k = 1 + (int)(r / ring);
-  if (k > nz - 2) return false;
-  const int64_t p = r % ring;
-  if (p < nx) {
-    i = (int)p;
-    j = 0;
-  } else if (p < 2 * nx) {
-    i = (int)(p - nx);
-    j = ny - 1;
-  } else {
-    const int64_t q = p - 2 * nx;
-    if (q < ny - 2) {
-      i = 0;
-      j = 1 + (int)q;
-    } else {
-      i = nx - 1;
-      j = 1 + (int)(q - (ny - 2));
-    }
-  }
-  return true;
-}
-
-// ---- interior (z-marching) path -----------------------------------------
-
-template <typename T>
+template <typename T, int TX, int TY>
 struct FusedSmem {
-  T e[2][3][FUSED_TY + 2][FUSED_TX + 2];  // E_old planes (ring of 2), tile + halo
-  T h[3][FUSED_TY + 1][FUSED_TX + 1];     // H_new of the plane, tile + low halo
+  using G = Tile<TX, TY>;
+  T u[G::NRING][3][G::RH][G::RW];   // u = L*pec*E_old, window planes (ring)
+  T w[2][3][TY + 1][TX + 1];        // w = tL*pmc*H_new, tile + low halo
+  T mt[MAX_MATERIALS * MAT_STRIDE];  // material table (dt/mu, dt/eps, sigma)
 };
 
-template <typename T>
-__device__ __forceinline__ void load_plane(const T *__restrict__ Ex, const T *__restrict__ Ey,
-                                           const T *__restrict__ Ez, int nx, int ny,
-                                           T (*dst)[FUSED_TY + 2][FUSED_TX + 2], int x0, int y0,
-                                           int k) {
-  constexpr int W = FUSED_TX + 2, Hh = FUSED_TY + 2;
-  const int base = nx * ny * k;
-#pragma unroll
-  for (int r0 = 0; r0 < W * Hh; r0 += FUSED_THREADS) {
-    const int r = r0 + threadIdx.x;
-    if (r < W * Hh) {
-      const int rj = r / W, ri = r - rj * W;
-      const int gi = x0 - 1 + ri, gj = y0 - 1 + rj;
-      T vx = 0, vy = 0, vz = 0;
-      if (gi < nx && gj < ny) {
-        const int o = base + gi + nx * gj;
-        vx = Ex[o];
-        vy = Ey[o];
-        vz = Ez[o];
-      }
-      dst[0][rj][ri] = vx;
-      dst[1][rj][ri] = vy;
-      dst[2][rj][ri] = vz;
-    }
-  }
+// A window cell's true coordinates, fixed over the march (dk: plane offset
+// from the y carry).
+struct WinCell {
+  int i, j, dk;
+  bool ok;
+};
+
+__device__ __forceinline__ WinCell win_cell(int gi, int gj, int nx, int ny) {
+  WinCell w;
+  const int cx = gi >= 0 ? gi / nx : -((-gi + nx - 1) / nx);
+  w.i = gi - cx * nx;
+  const int jj = gj + cx;
+  const int cy = jj >= 0 ? jj / ny : -((-jj + ny - 1) / ny);
+  w.j = jj - cy * ny;
+  w.dk = cy;
+  w.ok = true;
+  return w;
 }
 
-// u_d = L_d * pecmask_d * E_d from a staged plane at tile-local (li, lj); the
-// generic form, used for the low-halo cells.
-template <typename T>
-__device__ __forceinline__ T u_s(const StepArgs<T> &a, const T (*pl)[FUSED_TY + 2][FUSED_TX + 2],
-                                 int comp, int i, int j, int k, int li, int lj) {
-  if (masked(a.pec, comp, i, j, k, a.nx, a.ny, a.nz)) return T(0);
-  return Lof(a.g, comp, i, j, k) * pl[comp][lj][li];
-}
-
-// Generic H_new of a low-halo cell (i, j, k) from staged planes k, k+1.
-template <typename T>
-__device__ __forceinline__ T h_new_staged(const StepArgs<T> &a,
-                                          const T (*pk)[FUSED_TY + 2][FUSED_TX + 2],
-                                          const T (*pk1)[FUSED_TY + 2][FUSED_TX + 2], int comp,
-                                          int i, int j, int k, int li, int lj, int m) {
-  T curl;
-  if (comp == 0) {
-    const T uy0 = u_s(a, pk, 1, i, j, k, li, lj), uy1 = u_s(a, pk1, 1, i, j, k + 1, li, lj);
-    const T uz0 = u_s(a, pk, 2, i, j, k, li, lj), uz1 = u_s(a, pk, 2, i, j + 1, k, li, lj + 1);
-    curl = ((uy0 - uy1) - uz0) + uz1;
-  } else if (comp == 1) {
-    const T uz0 = u_s(a, pk, 2, i, j, k, li, lj), uz1 = u_s(a, pk, 2, i + 1, j, k, li + 1, lj);
-    const T ux0 = u_s(a, pk, 0, i, j, k, li, lj), ux1 = u_s(a, pk1, 0, i, j, k + 1, li, lj);
-    curl = ((uz0 - uz1) - ux0) + ux1;
-  } else {
-    const T ux0 = u_s(a, pk, 0, i, j, k, li, lj), ux1 = u_s(a, pk, 0, i, j + 1, k, li, lj + 1);
-    const T uy0 = u_s(a, pk, 1, i, j, k, li, lj), uy1 = u_s(a, pk, 1, i + 1, j, k, li + 1, lj);
-    curl = ((ux0 - ux1) - uy0) + uy1;
-  }
-  T c = T(0);
-  if (!masked(a.pmc, comp, i, j, k, a.nx, a.ny, a.nz)) {
-    const T area = comp == 0 ? a.g.iAx[j + a.ny * k]
-                             : (comp == 1 ? a.g.iAy[i + a.nx * k] : a.g.iAz[i + a.nx * j]);
-    c = a.mtab[m * MAT_STRIDE + comp] * area;
-  }
-  return a.H[(int64_t)comp * a.N + (i + a.nx * (j + a.ny * k))] - c * curl;
-}
-
-// The interior cell (i, j, k) has no face of its own, so its row masks and
-// its own column masks are 1; only neighbours on the last/first plane of an
-// axis can be masked.  With the reference's last-writer rule the relevant
-// flags reduce to the per-thread booleans below.
-template <typename T>
-__device__ void interior_tile(const FusedArgs<T> &f, int tile, FusedSmem<T> &sm) {
+// After the barrier of a plane the kernel reads only LDS and registers: every
+// global load it needs there (own and halo H_old/E_old/material, the 2-D
+// metric entries that vary along x, the staged plane and its L_z) is issued
+// one plane earlier, so the in-order completion of vector loads never makes
+// the compute of a plane wait for the prefetch of the next one.  The material
+// table lives in LDS; metric entries that are uniform over a wave (one row j
+// per wave, one plane k per iteration) are scalar loads.
+template <typename T, int TX, int TY, int KC>
+__global__ void __launch_bounds__(TX *TY) fused_sweep(FusedArgs<T> f) {
+  using G = Tile<TX, TY>;
+  static_assert(TX % 64 == 0, "a wave must cover one tile row");
+  constexpr int NT = G::NT, RW = G::RW, RH = G::RH, RC = G::RC, RL = G::RL, NR = G::NRING;
+  __shared__ FusedSmem<T, TX, TY> sm;
   const StepArgs<T> &a = f.a;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int N = (int)a.N;
+  const int plane = nx * ny;
+
+  // XCD-aware tile order
+  const int b = blockIdx.x;
+  const int q8 = f.nb / 8;
+  const int tile = (b < 8 * q8) ? (b % 8) * q8 + b / 8 : b;
   const int tx = tile % f.ntx;
-  const int r = tile / f.ntx;
-  const int ty = r % f.nty;
-  const int kc = r / f.nty;
-  const int x0 = 1 + tx * FUSED_TX, y0 = 1 + ty * FUSED_TY;
-  const int k0 = 1 + kc * FUSED_KC;
-  const int k1 = min(k0 + FUSED_KC, nz - 1);
-  const int lx = threadIdx.x % FUSED_TX, ly = threadIdx.x / FUSED_TX;
-  const int i = x0 + lx, j = y0 + ly;
-  const bool own = i <= nx - 2 && j <= ny - 2;
+  const int tr = tile / f.ntx;
+  const int ty = tr % f.nty;
+  const int kc = tr / f.nty;
+  const int x0 = tx * TX, y0 = ty * TY;
+  const int k0 = kc * KC;
+  const int k1 = min(k0 + KC, nz);  // planes k0 .. k1-1
+
   const T *__restrict__ Ex = a.E;
   const T *__restrict__ Ey = a.E + N;
   const T *__restrict__ Ez = a.E + 2 * (int64_t)N;
@@ -301,147 +143,366 @@ __device__ void interior_tile(const FusedArgs<T> &f, int tile, FusedSmem<T> &sm)
   T *__restrict__ Hny = f.Hn + N;
   T *__restrict__ Hnz = f.Hn + 2 * (int64_t)N;
   const uint8_t *__restrict__ mat = a.mat;
-  const T *__restrict__ mtab = a.mtab;
+  const Geo<T> &g = a.g;
+  const T *__restrict__ gLz = g.Lz;
+  const T *__restrict__ gtLz = g.tLz;
+  const T *__restrict__ giAx = g.iAx;
+  const T *__restrict__ giAy = g.iAy;
+  const T *__restrict__ gitAx = g.itAx;
+  const T *__restrict__ gitAy = g.itAy;
+  const uint32_t pec = a.pec, pmc = a.pmc;
 
-  // low-halo work: threads 0..TX-1 -> row (x0+t, y0-1); TX..TX+TY-1 ->
-  // column (x0-1, y0+t-TX)
-  int hi = -1, hj = -1;
-  if (threadIdx.x < FUSED_TX) {
-    hi = x0 + threadIdx.x;
-    hj = y0 - 1;
-    if (hi > nx - 2) hi = -1;
-  } else if (threadIdx.x < FUSED_TX + FUSED_TY) {
-    hi = x0 - 1;
-    hj = y0 + (threadIdx.x - FUSED_TX);
-    if (hj > ny - 2) hi = -1;
+  for (int t = threadIdx.x; t < a.nmat * MAT_STRIDE; t += NT) sm.mt[t] = a.mtab[t];
+
+  // ---- staging cells of this thread (window slots q) ----
+  WinCell sc[RL];
+  bool smx[RL][3];  // (i, j) part of the pec mask of each staged cell
+  T sL[RL][2];      // L_x(i'), L_y(j')
+#pragma unroll
+  for (int q = 0; q < RL; ++q) {
+    const int c = threadIdx.x + q * NT;
+    const int rj = c / RW, ri = c - rj * RW;
+    sc[q] = win_cell(x0 - 1 + ri, y0 - 1 + rj, nx, ny);
+    sc[q].ok = c < RC;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) smx[q][d] = xy_mask(pec, d, sc[q].i, sc[q].j, nx, ny);
+    sL[q][0] = g.Lx[sc[q].i];
+    sL[q][1] = g.Ly[sc[q].j];
   }
-  const int hli = hi - x0 + 1, hlj = hj - y0 + 1;
+  // raw E of a window plane plus L_z of each staged cell
+  auto fetch = [&](int k, T(&v)[RL][4]) {
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int kk = k + sc[q].dk;
+      T vx = 0, vy = 0, vz = 0, lz = 0;
+      if (sc[q].ok && kk >= 0 && kk < nz) {
+        const int o = sc[q].i + nx * sc[q].j + plane * kk;
+        vx = Ex[o];
+        vy = Ey[o];
+        vz = Ez[o];
+        lz = gLz[kk];
+      }
+      v[q][0] = vx;
+      v[q][1] = vy;
+      v[q][2] = vz;
+      v[q][3] = lz;
+    }
+  };
+  auto stage = [&](int k, const T(&v)[RL][4]) {
+    T(*dst)[RH][RW] = sm.u[k & (NR - 1)];
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int c = threadIdx.x + q * NT;
+      if (c < RC) {
+        const int rj = c / RW, ri = c - rj * RW;
+        const int kk = k + sc[q].dk;
+        dst[0][rj][ri] = full_mask(pec, 0, smx[q][0], kk, nz) ? T(0) : sL[q][0] * v[q][0];
+        dst[1][rj][ri] = full_mask(pec, 1, smx[q][1], kk, nz) ? T(0) : sL[q][1] * v[q][1];
+        dst[2][rj][ri] = full_mask(pec, 2, smx[q][2], kk, nz) ? T(0) : v[q][3] * v[q][2];
+      }
+    }
+  };
+
+  // ---- own cell ----
+  const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+  const int i = x0 + lx, j = y0 + ly;
+  const bool own = i < nx && j < ny;
+  const int ic = min(i, nx - 1);
+  // one tile row per wave: j is wave-uniform, so row-indexed metrics are
+  // scalar loads
+  const int jc = __builtin_amdgcn_readfirstlane(min(j, ny - 1));
   const int li = lx + 1, lj = ly + 1;
+  const int col = ic + nx * jc;
+  bool opec[3], opmc[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    opec[d] = xy_mask(pec, d, ic, jc, nx, ny);
+    opmc[d] = xy_mask(pmc, d, ic, jc, nx, ny);
+  }
+  const T tLx0 = g.tLx[ic], tLy0 = g.tLy[jc];
+  const T iAz0 = g.iAz[ic + nx * jc];
+  const T itAz0 = g.itAz[ic + nx * jc];
+  const bool beam_col = a.beam_mode != BEAM_OFF && i == a.ixs && j == a.iys;
 
-  // per-thread constants (clamped indices keep non-owned lanes in range)
-  const int ic = min(i, nx - 2), jc = min(j, ny - 2);
-  const T Lx0 = a.g.Lx[ic], Ly0 = a.g.Ly[jc];
-  const bool mxh = (ic + 1 == nx - 1) && (a.pec & XHI);  // E at i+1 on the xhi face
-  const bool myh = (jc + 1 == ny - 1) && (a.pec & YHI);  // E at j+1 on the yhi face
-  const T uLy_x1 = mxh ? T(0) : Ly0;  // u_y(i+1) = L_y(j) * E_y(i+1)
-  const T uLx_y1 = myh ? T(0) : Lx0;  // u_x(j+1) = L_x(i) * E_x(j+1)
-  const bool pxl = (ic - 1 == 0) && (a.pmc & XLO);  // H at i-1 on the xlo face
-  const bool pyl = (jc - 1 == 0) && (a.pmc & YLO);  // H at j-1 on the ylo face
-  const T tLx0 = a.g.tLx[ic], tLy0 = a.g.tLy[jc];
-  const T iAz0 = a.g.iAz[ic + nx * jc];
-  const T itAz0 = a.g.itAz[ic + nx * jc];
+  // ---- low-halo cell of this thread: row (x0+t, y0-1) -> H_x, H_z;
+  //      column (x0-1, y0+t-TX) -> H_y, H_z ----
+  const bool hrow = threadIdx.x < TX;
+  const bool hcolm = !hrow && threadIdx.x < TX + TY;
+  int hli = 0, hlj = 0;
+  WinCell hc{0, 0, 0, false};
+  if (hrow) {
+    hli = threadIdx.x + 1;
+    hlj = 0;
+    hc = win_cell(x0 + threadIdx.x, y0 - 1, nx, ny);
+    hc.ok = x0 + (int)threadIdx.x < nx;  // needed by an own cell only then
+  } else if (hcolm) {
+    hli = 0;
+    hlj = threadIdx.x - TX + 1;
+    hc = win_cell(x0 - 1, y0 + ((int)threadIdx.x - TX), nx, ny);
+    hc.ok = y0 + ((int)threadIdx.x - TX) < ny;
+  }
+  const int hc0 = hrow ? 0 : 1;  // first halo component (second is z)
+  const int hcolo = hc.i + nx * hc.j;
+  bool hpmc[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) hpmc[d] = xy_mask(pmc, d, hc.i, hc.j, nx, ny);
+  const T h_tla = hrow ? g.tLx[hc.i] : g.tLy[hc.j];  // tL of the first halo component
+  const T h_iAz = g.iAz[hc.i + nx * hc.j];
 
-  load_plane(Ex, Ey, Ez, nx, ny, sm.e[0], x0, y0, k0 - 1);
-  load_plane(Ex, Ey, Ez, nx, ny, sm.e[1], x0, y0, k0);
-  __syncthreads();
-  T hx_prev = 0, hy_prev = 0;
+  // Halo prefetch packet of plane kk: H_old (2 comps), material, area of the
+  // first component, tL_z.
+  struct HaloPk {
+    T ha, hb, area_a, tlz;
+    int m;
+    bool in;
+  };
+  auto halo_load = [&](int kk, HaloPk &p) {
+    const int hk = kk + hc.dk;
+    p.ha = p.hb = p.area_a = p.tlz = 0;
+    p.m = 0;
+    p.in = (hrow || hcolm) && hc.ok && hk >= 0 && hk < nz;
+    if (p.in) {
+      const int hn = hcolo + plane * hk;
+      p.ha = hrow ? Hx[hn] : Hy[hn];
+      p.hb = Hz[hn];
+      p.m = mat ? mat[hn] : 0;
+      p.area_a = hrow ? giAx[hc.j + ny * hk] : giAy[hc.i + nx * hk];
+      p.tlz = gtLz[hk];
+    }
+  };
+  // curl of H component d at window (wi, wj) from slots p0 (plane) / p1 (above)
+  auto curl_u = [&](int d, int p0, int p1, int wi, int wj) -> T {
+    const T(*u0)[RH][RW] = sm.u[p0];
+    const T(*u1)[RH][RW] = sm.u[p1];
+    if (d == 0) return ((u0[1][wj][wi] - u1[1][wj][wi]) - u0[2][wj][wi]) + u0[2][wj + 1][wi];
+    if (d == 1) return ((u0[2][wj][wi] - u0[2][wj][wi + 1]) - u0[0][wj][wi]) + u1[0][wj][wi];
+    return ((u0[0][wj][wi] - u0[0][wj + 1][wi]) - u0[1][wj][wi]) + u0[1][wj][wi + 1];
+  };
+  auto halo_w = [&](int kk, int p0, int p1, int s, const HaloPk &p) {
+    T wa = 0, wb = 0;
+    if (p.in) {
+      const int hk = kk + hc.dk;
+      const bool ra = full_mask(pmc, hc0, hpmc[hc0], hk, nz);
+      const bool rb = full_mask(pmc, 2, hpmc[2], hk, nz);
+      const T *mt = sm.mt + p.m * MAT_STRIDE;
+      const T ha = p.ha - (ra ? T(0) : mt[hc0] * p.area_a) * curl_u(hc0, p0, p1, hli, hlj);
+      const T hb = p.hb - (rb ? T(0) : mt[2] * h_iAz) * curl_u(2, p0, p1, hli, hlj);
+      wa = ra ? T(0) : h_tla * ha;
+      wb = rb ? T(0) : p.tlz * hb;
+    }
+    sm.w[s][hc0][hlj][hli] = wa;
+    sm.w[s][2][hlj][hli] = wb;
+  };
+
+  // ---- prologue: planes k0-1, k0, k0+1 staged; w of planes k0-1 and k0 ----
+  T pE[RL][4];
+  {
+    fetch(k0 - 1, pE);
+    stage(k0 - 1, pE);
+    fetch(k0, pE);
+    stage(k0, pE);
+    fetch(k0 + 1, pE);
+    stage(k0 + 1, pE);
+  }
+  if (k0 + 2 <= k1) fetch(k0 + 2, pE);
+  T eox = 0, eoy = 0, eoz = 0, itAy_k = 0;
+  int mcur = 0;
+  T hx0o = 0, hy0o = 0, hz0o = 0, iAy0 = 0;
+  T hxmo = 0, hymo = 0, iAym = 0;
+  int mm = 0;
   if (own) {
-    const int m = mat ? mat[i + nx * (j + ny * (k0 - 1))] : 0;
-    hx_prev = h_new_staged(a, sm.e[0], sm.e[1], 0, i, j, k0 - 1, li, lj, m);
-    hy_prev = h_new_staged(a, sm.e[0], sm.e[1], 1, i, j, k0 - 1, li, lj, m);
+    const int n0 = col + plane * k0;
+    eox = Ex[n0];
+    eoy = Ey[n0];
+    eoz = Ez[n0];
+    itAy_k = gitAy[ic + nx * k0];
+    mcur = mat ? mat[n0] : 0;
+    hx0o = Hx[n0];
+    hy0o = Hy[n0];
+    hz0o = Hz[n0];
+    iAy0 = giAy[ic + nx * k0];
+    if (k0 >= 1) {
+      const int nm = n0 - plane;
+      hxmo = Hx[nm];
+      hymo = Hy[nm];
+      mm = mat ? mat[nm] : 0;
+      iAym = giAy[ic + nx * (k0 - 1)];
+    }
   }
-  int cur = 1;
+  HaloPk hp0;
+  halo_load(k0, hp0);
+  __syncthreads();
+  T wxp = 0, wyp = 0;        // own w of plane k-1 (x, y)
+  T wx = 0, wy = 0, wz = 0;  // own w of plane k
+  {
+    const int s = k0 & 1;
+    const int pm = (k0 - 1) & (NR - 1), p0 = k0 & (NR - 1), p1 = (k0 + 1) & (NR - 1);
+    if (own && k0 >= 1) {
+      const int km = k0 - 1;
+      const bool r0 = full_mask(pmc, 0, opmc[0], km, nz);
+      const bool r1 = full_mask(pmc, 1, opmc[1], km, nz);
+      const T *mt = sm.mt + mm * MAT_STRIDE;
+      const T hxm = hxmo - (r0 ? T(0) : mt[0] * giAx[jc + ny * km]) * curl_u(0, pm, p0, li, lj);
+      const T hym = hymo - (r1 ? T(0) : mt[1] * iAym) * curl_u(1, pm, p0, li, lj);
+      wxp = r0 ? T(0) : tLx0 * hxm;
+      wyp = r1 ? T(0) : tLy0 * hym;
+    }
+    if (own) {
+      const int n0 = col + plane * k0;
+      const bool r0 = full_mask(pmc, 0, opmc[0], k0, nz);
+      const bool r1 = full_mask(pmc, 1, opmc[1], k0, nz);
+      const bool r2 = full_mask(pmc, 2, opmc[2], k0, nz);
+      const T *mt = sm.mt + mcur * MAT_STRIDE;
+      const T hx = hx0o - (r0 ? T(0) : mt[0] * giAx[jc + ny * k0]) * curl_u(0, p0, p1, li, lj);
+      const T hy = hy0o - (r1 ? T(0) : mt[1] * iAy0) * curl_u(1, p0, p1, li, lj);
+      const T hz = hz0o - (r2 ? T(0) : mt[2] * iAz0) * curl_u(2, p0, p1, li, lj);
+      Hnx[n0] = hx;
+      Hny[n0] = hy;
+      Hnz[n0] = hz;
+      wx = r0 ? T(0) : tLx0 * hx;
+      wy = r1 ? T(0) : tLy0 * hy;
+      wz = r2 ? T(0) : gtLz[k0] * hz;
+    }
+    sm.w[s][0][lj][li] = wx;
+    sm.w[s][1][lj][li] = wy;
+    sm.w[s][2][lj][li] = wz;
+    if (hrow || hcolm) halo_w(k0, p0, p1, s, hp0);
+  }
+  // packet of plane k0+1 for the H update at iteration k0 and the E update at
+  // iteration k0+1
+  T pHx = 0, pHy = 0, pHz = 0, pEx = 0, pEy = 0, pEz = 0, piAy = 0, pitAy = 0;
+  int pM = 0;
+  HaloPk php;
+  php.in = false;
+  php.ha = php.hb = php.area_a = php.tlz = 0;
+  php.m = 0;
+  T pJ = 0;
+  if (k0 + 1 < k1) {
+    if (own) {
+      const int n1 = col + plane * (k0 + 1);
+      pHx = Hx[n1];
+      pHy = Hy[n1];
+      pHz = Hz[n1];
+      pEx = Ex[n1];
+      pEy = Ey[n1];
+      pEz = Ez[n1];
+      pM = mat ? mat[n1] : 0;
+      piAy = giAy[ic + nx * (k0 + 1)];
+      pitAy = gitAy[ic + nx * (k0 + 1)];
+    }
+    halo_load(k0 + 1, php);
+  }
+  T jcur = beam_col ? a.jline[k0] : T(0);
+  if (beam_col && k0 + 1 < k1) pJ = a.jline[k0 + 1];
+
   for (int k = k0; k < k1; ++k) {
-    const int nxt = cur ^ 1;
-    __syncthreads();  // plane k-1 and sm.h of plane k-1 no longer read
-    load_plane(Ex, Ey, Ez, nx, ny, sm.e[nxt], x0, y0, k + 1);
-    const int n = ic + nx * (jc + ny * k);
-    const int m = (own && mat) ? mat[n] : 0;
-    const T *mt = mtab + m * MAT_STRIDE;
-    const T hox = own ? Hx[n] : T(0), hoy = own ? Hy[n] : T(0), hoz = own ? Hz[n] : T(0);
-    const T Lz0 = a.g.Lz[k];
-    const bool mzh = (k + 1 == nz - 1) && (a.pec & ZHI);  // E at k+1 on the zhi face
-    const bool pzl = (k - 1 == 0) && (a.pmc & ZLO);       // H at k-1 on the zlo face
-    const T iAx0 = a.g.iAx[jc + ny * k], iAy0 = a.g.iAy[ic + nx * k];
-    const T itAx0 = a.g.itAx[jc + ny * k], itAy0 = a.g.itAy[ic + nx * k];
-    const T tLz0 = a.g.tLz[k];
-    __syncthreads();
-    const T(*pk)[FUSED_TY + 2][FUSED_TX + 2] = sm.e[cur];
-    const T(*pk1)[FUSED_TY + 2][FUSED_TX + 2] = sm.e[nxt];
-    T hx = 0, hy = 0, hz = 0;
-    if (own) {
-      const T ex = pk[0][lj][li], ey = pk[1][lj][li], ez = pk[2][lj][li];
-      const T ux0 = Lx0 * ex, uy0 = Ly0 * ey, uz0 = Lz0 * ez;
-      const T uy_pz = mzh ? T(0) : Ly0 * pk1[1][lj][li];
-      const T ux_pz = mzh ? T(0) : Lx0 * pk1[0][lj][li];
-      const T uz_py = myh ? T(0) : Lz0 * pk[2][lj + 1][li];
-      const T uz_px = mxh ? T(0) : Lz0 * pk[2][lj][li + 1];
-      const T ux_py = uLx_y1 * pk[0][lj + 1][li];
-      const T uy_px = uLy_x1 * pk[1][lj][li + 1];
-      hx = hox - (mt[0] * iAx0) * (((uy0 - uy_pz) - uz0) + uz_py);
-      hy = hoy - (mt[1] * iAy0) * (((uz0 - uz_px) - ux0) + ux_pz);
-      hz = hoz - (mt[2] * iAz0) * (((ux0 - ux_py) - uy0) + uy_px);
-      Hnx[n] = hx;
-      Hny[n] = hy;
-      Hnz[n] = hz;
-    }
-    sm.h[0][lj][li] = hx;
-    sm.h[1][lj][li] = hy;
-    sm.h[2][lj][li] = hz;
-    if (hi >= 0) {
-      const int hn = hi + nx * (hj + ny * k);
-      const int hm = mat ? mat[hn] : 0;
-      if (threadIdx.x < FUSED_TX) {  // row y0-1 feeds w_x and w_z
-        sm.h[0][hlj][hli] = h_new_staged(a, pk, pk1, 0, hi, hj, k, hli, hlj, hm);
-        sm.h[2][hlj][hli] = h_new_staged(a, pk, pk1, 2, hi, hj, k, hli, hlj, hm);
-      } else {  // column x0-1 feeds w_y and w_z
-        sm.h[1][hlj][hli] = h_new_staged(a, pk, pk1, 1, hi, hj, k, hli, hlj, hm);
-        sm.h[2][hlj][hli] = h_new_staged(a, pk, pk1, 2, hi, hj, k, hli, hlj, hm);
+    // 1. stage plane k+2 (its loads were issued one iteration ago)
+    if (k + 2 <= k1) stage(k + 2, pE);
+    // 2. prefetch plane k+3 and the own/halo packet of plane k+2
+    if (k + 3 <= k1) fetch(k + 3, pE);
+    T qHx = 0, qHy = 0, qHz = 0, qEx = 0, qEy = 0, qEz = 0, qiAy = 0, qitAy = 0, qJ = 0;
+    int qM = 0;
+    HaloPk qhp;
+    qhp.in = false;
+    qhp.ha = qhp.hb = qhp.area_a = qhp.tlz = 0;
+    qhp.m = 0;
+    if (k + 2 < k1) {
+      if (own) {
+        const int n2 = col + plane * (k + 2);
+        qHx = Hx[n2];
+        qHy = Hy[n2];
+        qHz = Hz[n2];
+        qEx = Ex[n2];
+        qEy = Ey[n2];
+        qEz = Ez[n2];
+        qM = mat ? mat[n2] : 0;
+        qiAy = giAy[ic + nx * (k + 2)];
+        qitAy = gitAy[ic + nx * (k + 2)];
       }
+      halo_load(k + 2, qhp);
+      if (beam_col) qJ = a.jline[k + 2];
     }
+    const int n = col + plane * k;
+    T jbx = 0, jby = 0, jbz = 0;
+    if (own && a.jmode == J_BUFFER) {
+      jbx = a.J[n];
+      jby = (a.J + N)[n];
+      jbz = (a.J + 2 * (int64_t)N)[n];
+    }
+    const T itAx_k = gitAx[jc + ny * k];                       // scalar
+    const T iAx_k1 = k + 1 < nz ? giAx[jc + ny * (k + 1)] : T(0);  // scalar
+    const T tLz_k1 = k + 1 < nz ? gtLz[k + 1] : T(0);            // scalar
+    // 3.
     __syncthreads();
+    // 4. E_new(k)
     if (own) {
-      const T wx0 = tLx0 * hx, wy0 = tLy0 * hy, wz0 = tLz0 * hz;
-      const T wy_mx = pxl ? T(0) : tLy0 * sm.h[1][lj][li - 1];
-      const T wz_mx = pxl ? T(0) : tLz0 * sm.h[2][lj][li - 1];
-      const T wx_my = pyl ? T(0) : tLx0 * sm.h[0][lj - 1][li];
-      const T wz_my = pyl ? T(0) : tLz0 * sm.h[2][lj - 1][li];
-      const T wx_mz = pzl ? T(0) : tLx0 * hx_prev;
-      const T wy_mz = pzl ? T(0) : tLy0 * hy_prev;
-      const T ex = pk[0][lj][li], ey = pk[1][lj][li], ez = pk[2][lj][li];
-      probe(a, i, j, k, ez);
-      T jx = 0, jy = 0, jz = 0;
+      const T(*ww)[TY + 1][TX + 1] = sm.w[k & 1];
+      const T *mt = sm.mt + mcur * MAT_STRIDE;
+      probe(a, i, j, k, eoz);
+      T jx = jbx, jy = jby, jz = jbz;
       if (a.jmode == J_SIGMA_E) {
-        jx = mt[6] * ex;
-        jy = mt[7] * ey;
-        jz = mt[8] * ez;
-      } else if (a.jmode == J_BUFFER) {
-        jx = a.J[n];
-        jy = (a.J + N)[n];
-        jz = (a.J + 2 * (int64_t)N)[n];
+        jx = mt[6] * eox;
+        jy = mt[7] * eoy;
+        jz = mt[8] * eoz;
       }
-      if (a.beam_mode != BEAM_OFF && i == a.ixs && j == a.iys)
-        jz = (a.beam_mode == BEAM_VALUES) ? a.jline[k] : beam_gauss<T>(a, k);
-      const T cx = ((wz0 - wz_my) - wy0) + wy_mz;
-      const T cy = ((wx0 - wx_mz) - wz0) + wz_mx;
-      const T cz = ((wy0 - wy_mx) - wx0) + wx_my;
-      Enx[n] = ex + mt[3] * (itAx0 * cx - jx);
-      Eny[n] = ey + mt[4] * (itAy0 * cy - jy);
-      Enz[n] = ez + mt[5] * (itAz0 * cz - jz);
+      if (beam_col) jz = jcur;
+      const T ax = full_mask(pec, 0, opec[0], k, nz) ? T(0) : itAx_k;
+      const T ay = full_mask(pec, 1, opec[1], k, nz) ? T(0) : itAy_k;
+      const T az = full_mask(pec, 2, opec[2], k, nz) ? T(0) : itAz0;
+      const T cx = ((wz - ww[2][lj - 1][li]) - wy) + wyp;
+      const T cy = ((wx - wxp) - wz) + ww[2][lj][li - 1];
+      const T cz = ((wy - ww[1][lj][li - 1]) - wx) + ww[0][lj - 1][li];
+      Enx[n] = eox + mt[3] * (ax * cx - jx);
+      Eny[n] = eoy + mt[4] * (ay * cy - jy);
+      Enz[n] = eoz + mt[5] * (az * cz - jz);
     }
-    hx_prev = hx;
-    hy_prev = hy;
-    cur = nxt;
+    wxp = wx;
+    wyp = wy;
+    // 5. H_new(k+1)
+    if (k + 1 < k1) {
+      const int kk = k + 1;
+      const int s = kk & 1;
+      const int p0 = kk & (NR - 1), p1 = (kk + 1) & (NR - 1);
+      if (own) {
+        const int n1 = n + plane;
+        const bool r0 = full_mask(pmc, 0, opmc[0], kk, nz);
+        const bool r1 = full_mask(pmc, 1, opmc[1], kk, nz);
+        const bool r2 = full_mask(pmc, 2, opmc[2], kk, nz);
+        const T *mt = sm.mt + pM * MAT_STRIDE;
+        const T hx = pHx - (r0 ? T(0) : mt[0] * iAx_k1) * curl_u(0, p0, p1, li, lj);
+        const T hy = pHy - (r1 ? T(0) : mt[1] * piAy) * curl_u(1, p0, p1, li, lj);
+        const T hz = pHz - (r2 ? T(0) : mt[2] * iAz0) * curl_u(2, p0, p1, li, lj);
+        Hnx[n1] = hx;
+        Hny[n1] = hy;
+        Hnz[n1] = hz;
+        wx = r0 ? T(0) : tLx0 * hx;
+        wy = r1 ? T(0) : tLy0 * hy;
+        wz = r2 ? T(0) : tLz_k1 * hz;
+      }
+      sm.w[s][0][lj][li] = wx;
+      sm.w[s][1][lj][li] = wy;
+      sm.w[s][2][lj][li] = wz;
+      if (hrow || hcolm) halo_w(kk, p0, p1, s, php);
+    }
+    // rotate the packets
+    mcur = pM;
+    eox = pEx;
+    eoy = pEy;
+    eoz = pEz;
+    itAy_k = pitAy;
+    jcur = pJ;
+    pHx = qHx;
+    pHy = qHy;
+    pHz = qHz;
+    pEx = qEx;
+    pEy = qEy;
+    pEz = qEz;
+    piAy = qiAy;
+    pitAy = qitAy;
+    pM = qM;
+    pJ = qJ;
+    php = qhp;
   }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(FUSED_THREADS) fused_sweep(FusedArgs<T> f) {
-  __shared__ FusedSmem<T> sm;
-  const int b = blockIdx.x;
-  // XCD-aware remap: blocks b, b+8, b+16, ... share an XCD; give them
-  // consecutive tiles
-  const int q = f.nb_interior / 8;
-  const int tile = (b < 8 * q) ? (b % 8) * q + b / 8 : b;
-  interior_tile(f, tile, sm);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(SHELL_THREADS) fused_shell(FusedArgs<T> f) {
-  const int64_t t = (int64_t)blockIdx.x * SHELL_THREADS + threadIdx.x;
-  if (t >= f.nshell) return;
-  int i, j, k;
-  if (shell_ijk(t, f.a.nx, f.a.ny, f.a.nz, i, j, k)) shell_cell(f, i, j, k);
 }
 
 inline bool fused_supported(int kernel, int nx, int ny, int nz) {
@@ -451,34 +512,80 @@ inline bool fused_supported(int kernel, int nx, int ny, int nz) {
   return kernel != 1;  // FIT_KERNEL_SPLIT selects the split kernels
 }
 
-inline int64_t shell_count(int nx, int ny, int nz) {
-  const int64_t plane = (int64_t)nx * ny;
-  const int64_t inner = nz > 2 ? (int64_t)(nz - 2) * (2 * (int64_t)nx + 2 * (int64_t)(ny - 2)) : 0;
-  return (nz >= 2 ? 2 * plane : plane) + inner;
+// Tile configurations (TX, TY, KC) selectable per context.
+enum {
+  FUSED_CFG_256x2 = 0,
+  FUSED_CFG_128x4 = 1,
+  FUSED_CFG_64x8 = 2,
+  FUSED_CFG_256x4 = 3,
+  FUSED_CFG_64x4 = 4,
+  FUSED_CFG_256x2_K64 = 5,
+  FUSED_NCFG = 6
+};
+
+inline void fused_dims(int cfg, int &tx, int &ty, int &kc) {
+  switch (cfg) {
+    case FUSED_CFG_128x4: tx = 128; ty = 4; kc = 32; break;
+    case FUSED_CFG_64x8: tx = 64; ty = 8; kc = 32; break;
+    case FUSED_CFG_256x4: tx = 256; ty = 4; kc = 32; break;
+    case FUSED_CFG_64x4: tx = 64; ty = 4; kc = 32; break;
+    case FUSED_CFG_256x2_K64: tx = 256; ty = 2; kc = 64; break;
+    default: tx = 256; ty = 2; kc = 32; break;
+  }
 }
 
+// Default when no tuning was done: full 256-wide rows when the grid has them.
+inline int fused_default(int nx, bool fp64) {
+  if (nx >= 192 && !fp64) return FUSED_CFG_256x2;
+  if (nx >= 96) return FUSED_CFG_128x4;
+  return FUSED_CFG_64x8;
+}
+
+template <typename T, int TX, int TY>
+constexpr bool cfg_fits() {
+  return sizeof(FusedSmem<T, TX, TY>) <= 160 * 1024;
+}
+
+// Whether a tile configuration fits the LDS of one workgroup for T.
 template <typename T>
-inline void launch_fused(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s,
-                         hipEvent_t shell_ev = nullptr) {
+inline bool fused_cfg_ok(int cfg) {
+  switch (cfg) {
+    case FUSED_CFG_128x4: return cfg_fits<T, 128, 4>();
+    case FUSED_CFG_64x8: return cfg_fits<T, 64, 8>();
+    case FUSED_CFG_256x4: return cfg_fits<T, 256, 4>();
+    case FUSED_CFG_64x4: return cfg_fits<T, 64, 4>();
+    default: return cfg_fits<T, 256, 2>();
+  }
+}
+
+template <typename T, int TX, int TY, int KC>
+inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
+  if constexpr (!cfg_fits<T, TX, TY>()) {
+    (void)a; (void)En; (void)Hn; (void)s;
+    return;
+  } else {
   FusedArgs<T> f;
   f.a = a;
   f.En = En;
   f.Hn = Hn;
-  const int ix = a.nx - 2, iy = a.ny - 2, iz = a.nz - 2;
-  if (ix > 0 && iy > 0 && iz > 0) {
-    f.ntx = (ix + FUSED_TX - 1) / FUSED_TX;
-    f.nty = (iy + FUSED_TY - 1) / FUSED_TY;
-    f.nkc = (iz + FUSED_KC - 1) / FUSED_KC;
-  } else {
-    f.ntx = f.nty = f.nkc = 0;
+  f.ntx = (a.nx + TX - 1) / TX;
+  f.nty = (a.ny + TY - 1) / TY;
+  f.nkc = (a.nz + KC - 1) / KC;
+  f.nb = f.ntx * f.nty * f.nkc;
+  hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC>), dim3(f.nb), dim3(TX * TY), 0, s, f);
   }
-  f.nb_interior = f.ntx * f.nty * f.nkc;
-  f.nshell = shell_count(a.nx, a.ny, a.nz);
-  const int nb_shell = (int)((f.nshell + SHELL_THREADS - 1) / SHELL_THREADS);
-  if (f.nb_interior > 0)
-    hipLaunchKernelGGL(fused_sweep<T>, dim3(f.nb_interior), dim3(FUSED_THREADS), 0, s, f);
-  if (shell_ev) (void)hipEventRecord(shell_ev, s);
-  hipLaunchKernelGGL(fused_shell<T>, dim3(nb_shell), dim3(SHELL_THREADS), 0, s, f);
+}
+
+template <typename T>
+inline void launch_fused(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s, int cfg) {
+  switch (cfg) {
+    case FUSED_CFG_128x4: launch_cfg<T, 128, 4, 32>(a, En, Hn, s); break;
+    case FUSED_CFG_64x8: launch_cfg<T, 64, 8, 32>(a, En, Hn, s); break;
+    case FUSED_CFG_256x4: launch_cfg<T, 256, 4, 32>(a, En, Hn, s); break;
+    case FUSED_CFG_64x4: launch_cfg<T, 64, 4, 32>(a, En, Hn, s); break;
+    case FUSED_CFG_256x2_K64: launch_cfg<T, 256, 2, 64>(a, En, Hn, s); break;
+    default: launch_cfg<T, 256, 2, 32>(a, En, Hn, s); break;
+  }
 }
 
 }  // namespace fit

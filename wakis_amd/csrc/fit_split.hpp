@@ -21,16 +21,18 @@
 
 namespace fit {
 
+// Launch: block (SPLIT_BX, SPLIT_BY), grid (ceil(nx/BX), ceil(ny/BY), nz).
+constexpr int SPLIT_BX = 64;
+constexpr int SPLIT_BY = 4;
+
 template <typename T>
-__global__ void __launch_bounds__(256) h_update_split(StepArgs<T> a) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= a.N) return;
-  const int64_t nxy = (int64_t)a.nx * a.ny;
-  const int k = (int)(n / nxy);
-  const int64_t r = n - (int64_t)k * nxy;
-  const int j = (int)(r / a.nx);
-  const int i = (int)(r - (int64_t)j * a.nx);
+__global__ void __launch_bounds__(SPLIT_BX *SPLIT_BY) h_update_split(StepArgs<T> a) {
+  const int i = blockIdx.x * SPLIT_BX + threadIdx.x;
+  const int j = blockIdx.y * SPLIT_BY + threadIdx.y;
+  const int k = blockIdx.z;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
+  if (i >= nx || j >= ny) return;
+  const int64_t n = flat(i, j, k, nx, ny);
 
   const T ux0 = u_at(a, 0, i, j, k), uy0 = u_at(a, 1, i, j, k), uz0 = u_at(a, 2, i, j, k);
   T uy_px = 0, uz_px = 0, ux_py = 0, uz_py = 0, ux_pz = 0, uy_pz = 0;
@@ -70,15 +72,13 @@ __global__ void __launch_bounds__(256) h_update_split(StepArgs<T> a) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) e_update_split(StepArgs<T> a) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= a.N) return;
-  const int64_t nxy = (int64_t)a.nx * a.ny;
-  const int k = (int)(n / nxy);
-  const int64_t r = n - (int64_t)k * nxy;
-  const int j = (int)(r / a.nx);
-  const int i = (int)(r - (int64_t)j * a.nx);
+__global__ void __launch_bounds__(SPLIT_BX *SPLIT_BY) e_update_split(StepArgs<T> a) {
+  const int i = blockIdx.x * SPLIT_BX + threadIdx.x;
+  const int j = blockIdx.y * SPLIT_BY + threadIdx.y;
+  const int k = blockIdx.z;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
+  if (i >= nx || j >= ny) return;
+  const int64_t n = flat(i, j, k, nx, ny);
 
   const T wx0 = w_at(a, 0, i, j, k), wy0 = w_at(a, 1, i, j, k), wz0 = w_at(a, 2, i, j, k);
   T wy_mx = 0, wz_mx = 0, wx_my = 0, wz_my = 0, wx_mz = 0, wy_mz = 0;
@@ -127,9 +127,7 @@ __global__ void __launch_bounds__(256) e_update_split(StepArgs<T> a) {
     jy = a.J[a.N + n];
     jz = a.J[2 * a.N + n];
   }
-  if (a.beam_mode != BEAM_OFF && i == a.ixs && j == a.iys) {
-    jz = (a.beam_mode == BEAM_VALUES) ? a.jline[k] : beam_gauss<T>(a, k);
-  }
+  if (a.beam_mode != BEAM_OFF && i == a.ixs && j == a.iys) jz = a.jline[k];
 
   const T ax = masked(a.pec, 0, i, j, k, nx, ny, nz) ? T(0) : a.g.itAx[j + (int64_t)ny * k];
   const T ay = masked(a.pec, 1, i, j, k, nx, ny, nz) ? T(0) : a.g.itAy[i + (int64_t)nx * k];
@@ -140,6 +138,19 @@ __global__ void __launch_bounds__(256) e_update_split(StepArgs<T> a) {
   E[n] = ex + mt[3] * (ax * cx - jx);
   E[a.N + n] = ey + mt[4] * (ay * cy - jy);
   E[2 * a.N + n] = ez + mt[5] * (az * cz - jz);
+}
+
+}  // namespace fit
+
+namespace fit {
+
+// Plain streaming copy (calibration of the achievable HBM bandwidth and of
+// the rocprof byte counters): 16-byte vector loads and stores.
+__global__ void __launch_bounds__(256) stream_copy(const float4 *__restrict__ src,
+                                                   float4 *__restrict__ dst, int64_t n4) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += stride)
+    dst[t] = src[t];
 }
 
 }  // namespace fit

@@ -105,7 +105,7 @@ class SolverFIT3D(RoutinesMixin):
                  bc_high=("Periodic", "Periodic", "Periodic"),
                  use_stl=False, use_conductors=False, use_gpu=True, use_mpi=False,
                  use_sibc=True, fmax=1e9, dtype=np.float64, n_pml=10, bg=(1.0, 1.0),
-                 verbose=1, device=0, kernel="auto"):
+                 verbose=1, device=0, kernel="auto", tile=None):
         t0 = time.time()
         if use_mpi:
             raise NotImplementedError(
@@ -195,6 +195,8 @@ class SolverFIT3D(RoutinesMixin):
             self.ctx.set_bc(self.pec_bits, self.pmc_bits)
             self.ctx.set_dt(float(self.dt))
             self.set_kernel(kernel)
+            if tile is not None:
+                self.ctx.set_tile(int(tile))
             self.E = DeviceField(self.ctx, _lib.FIELD_E, self)
             self.H = DeviceField(self.ctx, _lib.FIELD_H, self)
             self.J = DeviceField(self.ctx, _lib.FIELD_J, self)
@@ -217,6 +219,13 @@ class SolverFIT3D(RoutinesMixin):
             raise _lib.FitError("solver was built with device=None (host setup only)")
 
     # -- kernels -------------------------------------------------------------
+    def tune(self, reps=5):
+        """Autotune the fused sweep's tile shape on this grid (the fields are
+        not changed); returns {tile id: ms per step}."""
+        self._require_device()
+        ms = self.ctx.tune(reps)
+        return {i: float(t) for i, t in enumerate(ms) if t >= 0}
+
     def set_kernel(self, kernel="auto"):
         k = {"auto": _lib.KERNEL_AUTO, "split": _lib.KERNEL_SPLIT,
              "fused": _lib.KERNEL_FUSED}[kernel]
