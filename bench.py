@@ -39,7 +39,23 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(nz_sample=32, steps=40):
+def traffic_from_profiles(kernel, dtype, grid, tile):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (each scaled by what the counter reads for a copy kernel of
+    known size), from the committed profile table; None if not profiled."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.tsv")
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None, None
+    for line in lines[1:]:
+        f = line.split("\t")
+        if len(f) >= 11 and f[0] == kernel and f[1] == dtype and f[2] == grid and f[3] == str(tile):
+            return int(float(f[8])), f[10]
+    return None, None
+
+
+def cpu_baseline(nz_sample=32, steps=150):
     """The CPU oracle (scipy.sparse FP64, single-threaded csc/csr mat-vec, the
     reference's arithmetic path) on a z-slab of the same pillbox grid."""
     from oracle.fit_oracle import OGrid, OSolver
@@ -137,7 +153,11 @@ def main():
     if fused:
         # one launch per step: E and H read once and written once per cell
         kname, kms, alg_bytes = "fused_sweep", k1 / nk, 12 * esz * N
-        extra = {}
+        ctype = "float" if args.dtype == "f32" else "double"
+        traffic, tsrc = traffic_from_profiles(
+            f"fused_sweep<{ctype}, {tile[1]}, {tile[2]}, {tile[3]}>", args.dtype,
+            f"{nx}x{ny}x{nz}", tile[0])
+        extra = {"traffic_source": f"profiles/{tsrc}" if tsrc else None}
     elif k2 >= k1:
         # E update: reads H and E, writes E (9 field words per cell)
         kname, kms, alg_bytes = "e_update_split", k2 / nk, 9 * esz * N
@@ -145,6 +165,8 @@ def main():
     else:
         kname, kms, alg_bytes = "h_update_split", k1 / nk, 9 * esz * N
         extra = {"e_update_avg_ms": round(k2 / nk, 5)}
+    if not fused:
+        traffic = None
     achieved = alg_bytes / (kms * 1e-3) / 1e9
     step_alg = 12 * esz * N / (ms_per_step * 1e-3) / 1e9
     copy_bytes = 2 * 12 * esz * N  # same footprint as the fields (read + write)
@@ -181,7 +203,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "alg_bytes_per_launch": alg_bytes,
             "avg_launch_ms": round(kms, 5),
             **extra,

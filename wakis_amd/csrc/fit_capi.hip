@@ -696,6 +696,7 @@ int fit_copy_bench(int32_t device, int64_t nbytes, int32_t reps, double *ms) {
   if (!ms || nbytes < 16 || reps < 1) return fail(FIT_EINVAL, "bad copy benchmark arguments");
   HIPCHK(hipSetDevice(device));
   const int64_t n4 = nbytes / 16;
+  if (n4 > ((int64_t)1 << 31) - 256) return fail(FIT_EINVAL, "copy benchmark size too large");
   float4 *src = nullptr, *dst = nullptr;
   if (hipMalloc(&src, n4 * 16) != hipSuccess || hipMalloc(&dst, n4 * 16) != hipSuccess) {
     (void)hipFree(src);
@@ -708,7 +709,7 @@ int fit_copy_bench(int32_t device, int64_t nbytes, int32_t reps, double *ms) {
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = fail(FIT_EHIP, "events");
   if (!rc) {
-    const unsigned blocks = 256 * 8 * 4;
+    const unsigned blocks = (unsigned)((n4 + 255) / 256);
     (void)hipMemsetAsync(src, 0, n4 * 16, st);
     hipLaunchKernelGGL(fit::stream_copy, dim3(blocks), dim3(256), 0, st, src, dst, n4);
     (void)hipEventRecord(e0, st);
