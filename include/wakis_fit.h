@@ -178,7 +178,7 @@ int fit_get_step_index(fit_ctx *ctx, int64_t *step);
 int fit_set_kernel(fit_ctx *ctx, int32_t kernel);
 /* Tile configuration of the fused sweep (0 .. FIT_NTILE-1; -1 = default
  * for the grid); fit_get_tile reports the one in use and its tile sizes. */
-enum { FIT_NTILE = 6 };
+enum { FIT_NTILE = 12 };
 int fit_set_tile(fit_ctx *ctx, int32_t cfg);
 int fit_get_tile(fit_ctx *ctx, int32_t *cfg, int32_t *tx, int32_t *ty, int32_t *kc);
 /* Autotuner: time every tile configuration of the fused sweep on this

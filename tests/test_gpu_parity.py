@@ -173,7 +173,7 @@ def test_field_indexing_roundtrip():
     assert np.array_equal(psv.H[:, 0, :, "z"], np.arange(35.0).reshape(5, 7))
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", list(range(12)))
 @pytest.mark.parametrize("name,bl,bh", BC_CASES, ids=[c[0] for c in BC_CASES])
 def test_fused_multi_tile_fp64(name, bl, bh, tile):
     """A grid spanning several fused tiles in x, y and z for every tile
@@ -209,7 +209,7 @@ def test_fused_equals_split_bitwise_structure():
         assert rel(getattr(b, n).toarray(), getattr(a, n).toarray()) <= 1e-13
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", list(range(12)))
 def test_fused_tiles_fp32(tile):
     """Every tile configuration in FP32 (256x4 fits the LDS only in FP32)."""
     shape = (140, 21, 40)

@@ -19,7 +19,7 @@ FIELD_E, FIELD_H, FIELD_J = 0, 1, 2
 J_ZERO, J_SIGMA_E, J_BUFFER = 0, 1, 2
 BEAM_OFF, BEAM_VALUES, BEAM_GAUSS = 0, 1, 2
 KERNEL_AUTO, KERNEL_SPLIT, KERNEL_FUSED = 0, 1, 2
-NTILE = 6
+NTILE = 12
 
 
 class FitError(RuntimeError):

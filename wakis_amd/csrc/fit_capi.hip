@@ -14,6 +14,7 @@
 #include "../../include/wakis_fit.h"
 #include "fit_aux.hpp"
 #include "fit_fused.hpp"
+#include "fit_sweep.hpp"
 #include "fit_split.hpp"
 
 namespace {
@@ -223,8 +224,8 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
     }
     if (ev) HIPCHK(hipEventRecord(ev[3 * s], c->stream));
     if (fused) {
-      const int cfg = c->fused_cfg >= 0 ? c->fused_cfg : fit::fused_default(c->nx, c->dtype == FIT_F64);
-      fit::launch_fused<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg);
+      const int cfg = c->fused_cfg >= 0 ? c->fused_cfg : fit::sweep_default(c->nx, c->dtype == FIT_F64);
+      fit::launch_sweep<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg);
       std::swap(c->E, c->E_alt);
       std::swap(c->H, c->H_alt);
       if (ev) HIPCHK(hipEventRecord(ev[3 * s + 1], c->stream));
@@ -281,15 +282,15 @@ int tune_impl(fit_ctx *c, int reps, double *ms_out) {
   double best_ms = 1e300;
   int rc = 0;
   if (a.beam_mode == FIT_BEAM_GAUSS) a.beam_mode = FIT_BEAM_OFF;  // timing only
-  for (int cfg = 0; cfg < fit::FUSED_NCFG && !rc; ++cfg) {
-    if (!fit::fused_cfg_ok<T>(cfg)) {
+  for (int cfg = 0; cfg < fit::SWEEP_NCFG && !rc; ++cfg) {
+    if (!fit::sweep_cfg_ok<T>(cfg) || !fit::sweep_cfg_runs<T>(a, cfg)) {
       if (ms_out) ms_out[cfg] = -1.0;
       continue;
     }
-    fit::launch_fused<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg);
+    fit::launch_sweep<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg);
     if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(FIT_EHIP, "event");
     for (int r = 0; r < reps && !rc; ++r)
-      fit::launch_fused<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg);
+      fit::launch_sweep<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg);
     float t = 0;
     if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                 hipEventElapsedTime(&t, e0, e1) != hipSuccess))
@@ -758,8 +759,8 @@ int fit_set_kernel(fit_ctx *c, int32_t kernel) {
 
 int fit_set_tile(fit_ctx *c, int32_t cfg) {
   if (!c) return fail(FIT_EINVAL, "null context");
-  if (cfg < -1 || cfg >= fit::FUSED_NCFG) return fail(FIT_EINVAL, "bad tile configuration");
-  if (cfg >= 0 && !(c->dtype == FIT_F64 ? fit::fused_cfg_ok<double>(cfg) : fit::fused_cfg_ok<float>(cfg)))
+  if (cfg < -1 || cfg >= fit::SWEEP_NCFG) return fail(FIT_EINVAL, "bad tile configuration");
+  if (cfg >= 0 && !(c->dtype == FIT_F64 ? fit::sweep_cfg_ok<double>(cfg) : fit::sweep_cfg_ok<float>(cfg)))
     return fail(FIT_EINVAL, "tile configuration does not fit the LDS for this dtype");
   c->fused_cfg = cfg;
   return 0;
@@ -767,9 +768,9 @@ int fit_set_tile(fit_ctx *c, int32_t cfg) {
 
 int fit_get_tile(fit_ctx *c, int32_t *cfg, int32_t *tx, int32_t *ty, int32_t *kc) {
   if (!c) return fail(FIT_EINVAL, "null context");
-  const int k = c->fused_cfg >= 0 ? c->fused_cfg : fit::fused_default(c->nx, c->dtype == FIT_F64);
+  const int k = c->fused_cfg >= 0 ? c->fused_cfg : fit::sweep_default(c->nx, c->dtype == FIT_F64);
   int a = 0, b = 0, d = 0;
-  fit::fused_dims(k, a, b, d);
+  fit::sweep_dims(k, a, b, d);
   if (cfg) *cfg = k;
   if (tx) *tx = a;
   if (ty) *ty = b;
