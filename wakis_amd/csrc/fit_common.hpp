@@ -23,6 +23,12 @@ constexpr uint32_t XLO = 1u, XHI = 2u, YLO = 4u, YHI = 8u, ZLO = 16u, ZHI = 32u;
 enum { J_ZERO = 0, J_SIGMA_E = 1, J_BUFFER = 2 };
 enum { BEAM_OFF = 0, BEAM_VALUES = 1, BEAM_GAUSS = 2 };
 
+// T in the constant address space: a load through it is a scalar load when
+// the address is wave-uniform, and does not wait behind the vector loads in
+// flight.  Only for tables that no kernel writes while a step runs.
+template <typename T>
+using CT = __attribute__((address_space(4))) T;
+
 // Material table row: dt/mu x,y,z | dt/eps x,y,z | sigma x,y,z.
 constexpr int MAT_STRIDE = 9;
 constexpr int MAX_MATERIALS = 256;

@@ -144,48 +144,68 @@ __global__ void __launch_bounds__(TX *TY) fused_sweep(FusedArgs<T> f) {
   T *__restrict__ Hnz = f.Hn + 2 * (int64_t)N;
   const uint8_t *__restrict__ mat = a.mat;
   const Geo<T> &g = a.g;
-  const T *__restrict__ gLz = g.Lz;
-  const T *__restrict__ gtLz = g.tLz;
-  const T *__restrict__ giAx = g.iAx;
+  // read-only tables in the constant address space: wave-uniform entries
+  // (one row j per wave, one plane per iteration) are scalar loads
+  const CT<T> *gLz = (const CT<T> *)g.Lz;
+  const CT<T> *gtLz = (const CT<T> *)g.tLz;
+  const CT<T> *giAx = (const CT<T> *)g.iAx;
   const T *__restrict__ giAy = g.iAy;
-  const T *__restrict__ gitAx = g.itAx;
+  const CT<T> *gitAx = (const CT<T> *)g.itAx;
   const T *__restrict__ gitAy = g.itAy;
+  const CT<T> *jline = (const CT<T> *)a.jline;
   const uint32_t pec = a.pec, pmc = a.pmc;
 
   for (int t = threadIdx.x; t < a.nmat * MAT_STRIDE; t += NT) sm.mt[t] = a.mtab[t];
 
   // ---- staging cells of this thread (window slots q) ----
+  // Only window cells that an active lane reads are loaded: columns up to
+  // gi = nx, rows up to gj = ny; their plane carry is -1, 0 or 1.  The others
+  // stage 0 and point at the cell (x0, y0), inside the grid on every plane.
   WinCell sc[RL];
   bool smx[RL][3];  // (i, j) part of the pec mask of each staged cell
   T sL[RL][2];      // L_x(i'), L_y(j')
+  const int ri_max = min(TX + 1, nx - x0 + 1), rj_max = min(TY + 1, ny - y0 + 1);
 #pragma unroll
   for (int q = 0; q < RL; ++q) {
     const int c = threadIdx.x + q * NT;
     const int rj = c / RW, ri = c - rj * RW;
-    sc[q] = win_cell(x0 - 1 + ri, y0 - 1 + rj, nx, ny);
-    sc[q].ok = c < RC;
+    const bool need = c < RC && ri <= ri_max && rj <= rj_max;
+    sc[q] = need ? win_cell(x0 - 1 + ri, y0 - 1 + rj, nx, ny) : WinCell{x0, y0, 0, false};
+    sc[q].ok = need;
 #pragma unroll
     for (int d = 0; d < 3; ++d) smx[q][d] = xy_mask(pec, d, sc[q].i, sc[q].j, nx, ny);
     sL[q][0] = g.Lx[sc[q].i];
     sL[q][1] = g.Ly[sc[q].j];
   }
-  // raw E of a window plane plus L_z of each staged cell
+  // raw E of a window plane plus L_z of each staged cell.  On planes 1 ..
+  // nz-2 every address is inside the grid and the loads are unconditional
+  // (no per-lane branch, so the compiler never waits for the loads in
+  // flight to reuse their registers); near the z ends they are checked.
   auto fetch = [&](int k, T(&v)[RL][4]) {
+    const bool inner = k >= 1 && k <= nz - 2;
 #pragma unroll
     for (int q = 0; q < RL; ++q) {
       const int kk = k + sc[q].dk;
-      T vx = 0, vy = 0, vz = 0, lz = 0;
-      if (sc[q].ok && kk >= 0 && kk < nz) {
+      if (inner) {
         const int o = sc[q].i + nx * sc[q].j + plane * kk;
-        vx = Ex[o];
-        vy = Ey[o];
-        vz = Ez[o];
-        lz = gLz[kk];
+        v[q][0] = Ex[o];
+        v[q][1] = Ey[o];
+        v[q][2] = Ez[o];
+        v[q][3] = gLz[kk];
+      } else {
+        T vx = 0, vy = 0, vz = 0, lz = 0;
+        if (sc[q].ok && kk >= 0 && kk < nz) {
+          const int o = sc[q].i + nx * sc[q].j + plane * kk;
+          vx = Ex[o];
+          vy = Ey[o];
+          vz = Ez[o];
+          lz = gLz[kk];
+        }
+        v[q][0] = vx;
+        v[q][1] = vy;
+        v[q][2] = vz;
+        v[q][3] = lz;
       }
-      v[q][0] = vx;
-      v[q][1] = vy;
-      v[q][2] = vz;
-      v[q][3] = lz;
     }
   };
   auto stage = [&](int k, const T(&v)[RL][4]) {
@@ -196,9 +216,10 @@ __global__ void __launch_bounds__(TX *TY) fused_sweep(FusedArgs<T> f) {
       if (c < RC) {
         const int rj = c / RW, ri = c - rj * RW;
         const int kk = k + sc[q].dk;
-        dst[0][rj][ri] = full_mask(pec, 0, smx[q][0], kk, nz) ? T(0) : sL[q][0] * v[q][0];
-        dst[1][rj][ri] = full_mask(pec, 1, smx[q][1], kk, nz) ? T(0) : sL[q][1] * v[q][1];
-        dst[2][rj][ri] = full_mask(pec, 2, smx[q][2], kk, nz) ? T(0) : v[q][3] * v[q][2];
+        const bool z = !sc[q].ok;
+        dst[0][rj][ri] = (z || full_mask(pec, 0, smx[q][0], kk, nz)) ? T(0) : sL[q][0] * v[q][0];
+        dst[1][rj][ri] = (z || full_mask(pec, 1, smx[q][1], kk, nz)) ? T(0) : sL[q][1] * v[q][1];
+        dst[2][rj][ri] = (z || full_mask(pec, 2, smx[q][2], kk, nz)) ? T(0) : v[q][3] * v[q][2];
       }
     }
   };
@@ -256,18 +277,29 @@ __global__ void __launch_bounds__(TX *TY) fused_sweep(FusedArgs<T> f) {
     int m;
     bool in;
   };
+  // The waves holding halo threads (rows: waves 0 .. TX/64-1, column: wave
+  // TX/64) load for every lane, at a clamped plane; p.in marks the values
+  // that are used.  No per-lane branch around the loads.
+  const bool hwave = (int)(threadIdx.x >> 6) <= TX / 64;  // wave-uniform
   auto halo_load = [&](int kk, HaloPk &p) {
     const int hk = kk + hc.dk;
-    p.ha = p.hb = p.area_a = p.tlz = 0;
-    p.m = 0;
     p.in = (hrow || hcolm) && hc.ok && hk >= 0 && hk < nz;
-    if (p.in) {
-      const int hn = hcolo + plane * hk;
-      p.ha = hrow ? Hx[hn] : Hy[hn];
+    if (hwave) {
+      const int hkc = min(max(hk, 0), nz - 1);
+      const int hn = hcolo + plane * hkc;
       p.hb = Hz[hn];
       p.m = mat ? mat[hn] : 0;
-      p.area_a = hrow ? giAx[hc.j + ny * hk] : giAy[hc.i + nx * hk];
-      p.tlz = gtLz[hk];
+      p.tlz = gtLz[hkc];
+      if (hrow) {  // wave-uniform (TX is a multiple of 64)
+        p.ha = Hx[hn];
+        p.area_a = giAx[hc.j + ny * hkc];
+      } else {
+        p.ha = Hy[hn];
+        p.area_a = giAy[hc.i + nx * hkc];
+      }
+    } else {
+      p.ha = p.hb = p.area_a = p.tlz = 0;
+      p.m = 0;
     }
   };
   // curl of H component d at window (wi, wj) from slots p0 (plane) / p1 (above)
@@ -392,43 +424,31 @@ __global__ void __launch_bounds__(TX *TY) fused_sweep(FusedArgs<T> f) {
     }
     halo_load(k0 + 1, php);
   }
-  T jcur = beam_col ? a.jline[k0] : T(0);
-  if (beam_col && k0 + 1 < k1) pJ = a.jline[k0 + 1];
+  T jcur = beam_col ? jline[k0] : T(0);
+  if (beam_col && k0 + 1 < k1) pJ = jline[k0 + 1];
 
   for (int k = k0; k < k1; ++k) {
     // 1. stage plane k+2 (its loads were issued one iteration ago)
     if (k + 2 <= k1) stage(k + 2, pE);
     // 2. prefetch plane k+3 and the own/halo packet of plane k+2
     if (k + 3 <= k1) fetch(k + 3, pE);
-    T qHx = 0, qHy = 0, qHz = 0, qEx = 0, qEy = 0, qEz = 0, qiAy = 0, qitAy = 0, qJ = 0;
-    int qM = 0;
+    // own packet of plane k+2 for every lane (clamped column and plane: a
+    // value past the chunk or the grid is never used)
+    const int k2 = min(k + 2, nz - 1);
+    const int n2 = col + plane * k2;
+    const T qHx = Hx[n2];
+    const T qHy = Hy[n2];
+    const T qHz = Hz[n2];
+    const T qEx = Ex[n2];
+    const T qEy = Ey[n2];
+    const T qEz = Ez[n2];
+    const int qM = mat ? mat[n2] : 0;
+    const T qiAy = giAy[ic + nx * k2];
+    const T qitAy = gitAy[ic + nx * k2];
     HaloPk qhp;
-    qhp.in = false;
-    qhp.ha = qhp.hb = qhp.area_a = qhp.tlz = 0;
-    qhp.m = 0;
-    if (k + 2 < k1) {
-      if (own) {
-        const int n2 = col + plane * (k + 2);
-        qHx = Hx[n2];
-        qHy = Hy[n2];
-        qHz = Hz[n2];
-        qEx = Ex[n2];
-        qEy = Ey[n2];
-        qEz = Ez[n2];
-        qM = mat ? mat[n2] : 0;
-        qiAy = giAy[ic + nx * (k + 2)];
-        qitAy = gitAy[ic + nx * (k + 2)];
-      }
-      halo_load(k + 2, qhp);
-      if (beam_col) qJ = a.jline[k + 2];
-    }
+    halo_load(k + 2, qhp);
+    const T qJ = beam_col ? jline[k2] : T(0);
     const int n = col + plane * k;
-    T jbx = 0, jby = 0, jbz = 0;
-    if (own && a.jmode == J_BUFFER) {
-      jbx = a.J[n];
-      jby = (a.J + N)[n];
-      jbz = (a.J + 2 * (int64_t)N)[n];
-    }
     const T itAx_k = gitAx[jc + ny * k];                       // scalar
     const T iAx_k1 = k + 1 < nz ? giAx[jc + ny * (k + 1)] : T(0);  // scalar
     const T tLz_k1 = k + 1 < nz ? gtLz[k + 1] : T(0);            // scalar
@@ -439,22 +459,31 @@ __global__ void __launch_bounds__(TX *TY) fused_sweep(FusedArgs<T> f) {
       const T(*ww)[TY + 1][TX + 1] = sm.w[k & 1];
       const T *mt = sm.mt + mcur * MAT_STRIDE;
       probe(a, i, j, k, eoz);
-      T jx = jbx, jy = jby, jz = jbz;
-      if (a.jmode == J_SIGMA_E) {
-        jx = mt[6] * eox;
-        jy = mt[7] * eoy;
-        jz = mt[8] * eoz;
-      }
-      if (beam_col) jz = jcur;
       const T ax = full_mask(pec, 0, opec[0], k, nz) ? T(0) : itAx_k;
       const T ay = full_mask(pec, 1, opec[1], k, nz) ? T(0) : itAy_k;
       const T az = full_mask(pec, 2, opec[2], k, nz) ? T(0) : itAz0;
       const T cx = ((wz - ww[2][lj - 1][li]) - wy) + wyp;
       const T cy = ((wx - wxp) - wz) + ww[2][lj][li - 1];
       const T cz = ((wy - ww[1][lj][li - 1]) - wx) + ww[0][lj - 1][li];
-      Enx[n] = eox + mt[3] * (ax * cx - jx);
-      Eny[n] = eoy + mt[4] * (ay * cy - jy);
-      Enz[n] = eoz + mt[5] * (az * cz - jz);
+      // Each J mode finishes the update in its own branch: the J loads of the
+      // buffer mode never share registers with the other modes, so their
+      // wait does not hold back the loads in flight for the next planes.
+      if (a.jmode == J_BUFFER) {
+        const T jx = a.J[n];
+        const T jy = (a.J + N)[n];
+        const T jz = beam_col ? jcur : (a.J + 2 * (int64_t)N)[n];
+        Enx[n] = eox + mt[3] * (ax * cx - jx);
+        Eny[n] = eoy + mt[4] * (ay * cy - jy);
+        Enz[n] = eoz + mt[5] * (az * cz - jz);
+      } else {
+        const T js = a.jmode == J_SIGMA_E ? T(1) : T(0);
+        const T jx = js * (mt[6] * eox);
+        const T jy = js * (mt[7] * eoy);
+        const T jz = beam_col ? jcur : js * (mt[8] * eoz);
+        Enx[n] = eox + mt[3] * (ax * cx - jx);
+        Eny[n] = eoy + mt[4] * (ay * cy - jy);
+        Enz[n] = eoz + mt[5] * (az * cz - jz);
+      }
     }
     wxp = wx;
     wyp = wy;

@@ -49,11 +49,6 @@ constexpr bool row_fits() {
   return sizeof(RowSmem<T, TX, TY>) <= 160 * 1024 && TX * (TY + 1) <= 1024;
 }
 
-// T in the constant address space (loads through it are scalar when the
-// address is wave-uniform)
-template <typename T>
-using CT = __attribute__((address_space(4))) T;
-
 template <typename T>
 __device__ __forceinline__ T ldg(const T *base, uint32_t byte_off) {
   return *(const T *)((const char *)base + byte_off);
