@@ -213,6 +213,35 @@ int fit_impedance(fit_ctx *ctx, const double *x, const double *lam,
                   double *z_re, double *z_im, double *x_re, double *x_im,
                   double *l_re, double *l_im);
 
+/* ---------------- batches (no counterpart in the reference) ----------------
+ * The reference steps one solver at a time (wakis/routines.py:231-485).  A
+ * batch steps many independent contexts (same device and dtype, fused
+ * sweep) with ONE persistent launch per step; each member's arithmetic is
+ * that of its own fit_step with the same tile configuration, so results are
+ * the same alone and in a batch.  Members keep their own state (fields,
+ * beam, wake probe); after fit_batch_step every member is as if it had run
+ * fit_step(nsteps, conductive[i]).  Members may be replaced between calls
+ * (continuous batching).  Members with host beam values (FIT_BEAM_VALUES)
+ * are rejected. */
+typedef struct fit_batch fit_batch;
+int fit_batch_create(fit_ctx *const *members, int32_t n, const int32_t *conductive,
+                     fit_batch **out);
+int fit_batch_set_members(fit_batch *batch, fit_ctx *const *members, int32_t n,
+                          const int32_t *conductive);
+int fit_batch_destroy(fit_batch *batch);
+/* Step every member nsteps; ms (optional) receives the elapsed time. */
+int fit_batch_step(fit_batch *batch, int64_t nsteps, double *ms);
+/* Tile configuration of the batched launch (0 .. FIT_NTILE-1 of the fused
+ * sweep family; -1: default for the widest member). */
+int fit_batch_set_tile(fit_batch *batch, int32_t cfg);
+int fit_batch_get_tile(fit_batch *batch, int32_t *cfg, int32_t *nblocks, int32_t *ntiles);
+/* Autotuner over the batch (state unchanged); ms[FIT_NTILE], -1 = not run. */
+int fit_batch_tune(fit_batch *batch, int32_t reps, double *ms);
+/* Launch groups of `group` steps (beam table, group sweeps, counter
+ * advance); with on != 0 a group is captured once into a HIP graph and
+ * replayed. */
+int fit_batch_set_graph(fit_batch *batch, int32_t on, int32_t group);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,5 +14,6 @@ from .grid import GridFIT3D  # noqa: F401
 from .solver import SolverFIT3D  # noqa: F401
 from .sources import Beam, PlaneWave  # noqa: F401
 from .wake import WakeSolver  # noqa: F401
+from .batch import BatchStepper, wakesolve_batch  # noqa: F401
 
 __version__ = "0.1.0"

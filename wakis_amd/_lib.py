@@ -79,6 +79,14 @@ SIGNATURES = {
     "fit_sync": [_P],
     "fit_wake_potential": [_P, _I32, _PI32, _I32, _I32, _I32, _I32, _PI32, _PD, _D, _PD],
     "fit_impedance": [_P, _PD, _PD, _I64, _I64, _I32, _D, _PD, _PD, _PD, _PD, _PD, _PD],
+    "fit_batch_create": [C.POINTER(_P), _I32, _PI32, C.POINTER(_P)],
+    "fit_batch_set_members": [_P, C.POINTER(_P), _I32, _PI32],
+    "fit_batch_destroy": [_P],
+    "fit_batch_step": [_P, _I64, _PD],
+    "fit_batch_set_tile": [_P, _I32],
+    "fit_batch_get_tile": [_P, _PI32, _PI32, _PI32],
+    "fit_batch_tune": [_P, _I32, _PD],
+    "fit_batch_set_graph": [_P, _I32, _I32],
 }
 _STR = {"fit_last_error", "fit_version"}
 
@@ -334,3 +342,68 @@ class Context:
         X = bufs[2] + 1j * bufs[3]
         L = bufs[4] + 1j * bufs[5]
         return z, X, L
+
+
+class Batch:
+    """Many contexts stepped by one persistent launch per step (see
+    ``fit_batch_*`` in include/wakis_fit.h)."""
+
+    def __init__(self, contexts, conductive=None):
+        self.lib = load()
+        h = C.c_void_p()
+        arr, cond = self._arrays(contexts, conductive)
+        check(self.lib.fit_batch_create(arr, len(contexts), iptr(cond), C.byref(h)),
+              "fit_batch_create")
+        self.h = h
+        self.contexts = list(contexts)
+
+    @staticmethod
+    def _arrays(contexts, conductive):
+        arr = (_P * max(len(contexts), 1))(*[c.h for c in contexts])
+        if conductive is None:
+            conductive = [0] * len(contexts)
+        cond = np.ascontiguousarray(np.asarray(conductive, dtype=np.int32).reshape(-1))
+        if cond.size < len(contexts):
+            raise ValueError("one conductive flag per member")
+        if cond.size == 0:
+            cond = np.zeros(1, dtype=np.int32)
+        return arr, cond
+
+    def set_members(self, contexts, conductive=None):
+        arr, cond = self._arrays(contexts, conductive)
+        check(self.lib.fit_batch_set_members(self.h, arr, len(contexts), iptr(cond)),
+              "fit_batch_set_members")
+        self.contexts = list(contexts)
+
+    def step(self, nsteps: int) -> float:
+        """Step every member nsteps; returns the elapsed ms (HIP events)."""
+        out = np.zeros(1, dtype=np.float64)
+        check(self.lib.fit_batch_step(self.h, int(nsteps), dptr(out)), "fit_batch_step")
+        return float(out[0])
+
+    def set_tile(self, cfg: int):
+        check(self.lib.fit_batch_set_tile(self.h, int(cfg)), "fit_batch_set_tile")
+
+    def get_tile(self):
+        v = [C.c_int32() for _ in range(3)]
+        check(self.lib.fit_batch_get_tile(self.h, *[C.byref(x) for x in v]), "fit_batch_get_tile")
+        return tuple(x.value for x in v)
+
+    def tune(self, reps: int = 5):
+        out = np.zeros(NTILE, dtype=np.float64)
+        check(self.lib.fit_batch_tune(self.h, int(reps), dptr(out)), "fit_batch_tune")
+        return out
+
+    def set_graph(self, on: bool, group: int = 16):
+        check(self.lib.fit_batch_set_graph(self.h, int(bool(on)), int(group)), "fit_batch_set_graph")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fit_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

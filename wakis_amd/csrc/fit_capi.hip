@@ -899,3 +899,5 @@ int fit_impedance(fit_ctx *c, const double *x, const double *lam, int64_t len, i
 }
 
 }  // extern "C"
+
+#include "fit_batch.inc"

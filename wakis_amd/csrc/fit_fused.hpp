@@ -107,21 +107,20 @@ __device__ __forceinline__ WinCell win_cell(int gi, int gj, int nx, int ny) {
 // the compute of a plane wait for the prefetch of the next one.  The material
 // table lives in LDS; metric entries that are uniform over a wave (one row j
 // per wave, one plane k per iteration) are scalar loads.
+// One tile (TX x TY columns, KC planes) of one step; `tile` is the tile
+// index in this grid's tile order.  Shared by the single-grid launch and the
+// batched launch, so a grid's arithmetic is the same alone and in a batch.
 template <typename T, int TX, int TY, int KC>
-__global__ void __launch_bounds__(TX *TY) fused_sweep(FusedArgs<T> f) {
+__device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile,
+                                           FusedSmem<T, TX, TY> &sm) {
   using G = Tile<TX, TY>;
   static_assert(TX % 64 == 0, "a wave must cover one tile row");
   constexpr int NT = G::NT, RW = G::RW, RH = G::RH, RC = G::RC, RL = G::RL, NR = G::NRING;
-  __shared__ FusedSmem<T, TX, TY> sm;
   const StepArgs<T> &a = f.a;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int N = (int)a.N;
   const int plane = nx * ny;
 
-  // XCD-aware tile order
-  const int b = blockIdx.x;
-  const int q8 = f.nb / 8;
-  const int tile = (b < 8 * q8) ? (b % 8) * q8 + b / 8 : b;
   const int tx = tile % f.ntx;
   const int tr = tile / f.ntx;
   const int ty = tr % f.nty;
@@ -532,6 +531,78 @@ __global__ void __launch_bounds__(TX *TY) fused_sweep(FusedArgs<T> f) {
     pJ = qJ;
     php = qhp;
   }
+}
+
+// Tile order of block b among nb blocks: the hardware dispatches block b to
+// XCD b % 8, so consecutive tiles (which share halo planes in L2) are given
+// to blocks of one XCD.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q8 = nb / 8;
+  return (b < 8 * q8) ? (b % 8) * q8 + b / 8 : b;
+}
+
+template <typename T, int TX, int TY, int KC>
+__global__ void __launch_bounds__(TX *TY) fused_sweep(FusedArgs<T> f) {
+  __shared__ FusedSmem<T, TX, TY> sm;
+  fused_tile<T, TX, TY, KC>(f, xcd_tile(blockIdx.x, f.nb), sm);
+}
+
+// ---------------- batched launch ----------------
+// One member grid of a batch: its step arguments for both buffer parities
+// (parity p: E, H read from f[p].a, written to f[p].En/Hn), its step index
+// when the batch counter is 0, and its beam table (nz values per step of the
+// current launch group, or null).
+template <typename T>
+struct BatchMember {
+  FusedArgs<T> f[2];
+  int64_t step0;
+  const T *btab;
+};
+
+// One step of every member grid in one persistent launch: block b takes the
+// batch's tiles b, b + gridDim.x, ...; tstart[m] is the first tile of member
+// m (ascending, tstart[nmem] = ntiles).  Step = *dstep + g of the batch.
+template <typename T, int TX, int TY, int KC>
+__global__ void __launch_bounds__(TX *TY)
+    fused_batch(const BatchMember<T> *__restrict__ mem, const int *__restrict__ tstart, int nmem,
+                int ntiles, const int64_t *__restrict__ dstep, int g) {
+  __shared__ FusedSmem<T, TX, TY> sm;
+  const int64_t s = *dstep + g;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tstart[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchMember<T> &m = mem[lo];
+    FusedArgs<T> f = m.f[s & 1];
+    f.a.step = m.step0 + s;
+    if (m.btab) f.a.jline = m.btab + (int64_t)g * f.a.nz;
+    fused_tile<T, TX, TY, KC>(f, xcd_tile(t - tstart[lo], f.nb), sm);
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+
+// Beam tables of a launch group: member m, step g (0 .. G-1), plane k.
+template <typename T>
+__global__ void batch_beam_table(const BatchMember<T> *__restrict__ mem, int nmem,
+                                 const int64_t *__restrict__ dstep, int G, T *const *btabs) {
+  const int m = blockIdx.y;
+  if (m >= nmem || !btabs[m]) return;
+  const StepArgs<T> &a0 = mem[m].f[0].a;
+  const int64_t cnt = (int64_t)G * a0.nz;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int g = (int)(t / a0.nz), k = (int)(t % a0.nz);
+    StepArgs<T> a = a0;
+    a.step = mem[m].step0 + *dstep + g;
+    btabs[m][t] = beam_gauss(a, k);
+  }
+}
+
+__global__ void batch_advance(int64_t *dstep, int G) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dstep += G;
 }
 
 inline bool fused_supported(int kernel, int nx, int ny, int nz) {

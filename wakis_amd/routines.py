@@ -65,6 +65,36 @@ class RoutinesMixin:
                   save_J=False, add_space=None, use_field_monitor=False, field_monitor=None,
                   use_edt=None, chunk=4096, **kwargs):
         """wakis/routines.py:231-485 on the device (see module docstring)."""
+        if plot:
+            raise NotImplementedError("plotting is not part of this build")
+        run = self._wake_setup(wakelength, wake, compute_plane, add_space)
+        t0 = time.time()
+        per_step = callback is not None or (use_field_monitor and field_monitor is not None)
+        if per_step:
+            beam = run["beam"]
+            for n in range(run["Nt"]):
+                beam.update(self, n * self.dt)
+                self.one_step()
+                if use_field_monitor and field_monitor is not None:
+                    field_monitor.update(self.E, self.dt)
+                if callback is not None:
+                    callback(self, n * self.dt)
+        else:
+            self._wake_device_beam(run)
+            done = 0
+            while done < run["Nt"]:
+                n = min(chunk, run["Nt"] - done)
+                self._advance(n)
+                done += n
+            self.ctx.sync()
+            self._wake_device_end(run)
+        self.simulationTime = time.time() - t0
+        self._wake_finish(run, compute_plane)
+
+    # -- the parts of wakesolve (shared with the batched solve) ---------------
+    def _wake_setup(self, wakelength, wake, compute_plane, add_space=None):
+        """Test point, Nt, beam and E_z probe of a wake run
+        (wakis/routines.py:366-432)."""
         if wake is not None:
             self.wake = wake
         if self.wake is None:
@@ -73,14 +103,10 @@ class RoutinesMixin:
             wakelength = self.wake.wakelength
         if add_space is not None:
             self.wake.skip_cells = add_space
-        if plot:
-            raise NotImplementedError("plotting is not part of this build")
-
         self.xtest, self.ytest = self.wake.xtest, self.wake.ytest
         self.ixt = int(np.abs(self.x - self.xtest).argmin())
         self.iyt = int(np.abs(self.y - self.ytest).argmin())
         h = 0 if compute_plane.lower() == "longitudinal" else 1
-
         self.wake.wakelength = wakelength
         self.ti = self.wake.ti
         self.v = self.wake.v
@@ -88,53 +114,46 @@ class RoutinesMixin:
         tmax = (wakelength + self.ti * self.v + (z.max() - z.min())) / self.v
         Nt = int(tmax / self.dt)
         self.tmax, self.Nt = tmax, Nt
-
         beam = Beam(q=self.wake.q, sigmaz=self.wake.sigmaz, beta=self.wake.beta,
                     xsource=self.wake.xsource, ysource=self.wake.ysource)
         t_axis = np.arange(0, Nt * self.dt, self.dt)
+        self.ctx.wake_probe(self.ixt, self.iyt, h, max(Nt, 1))
+        self.ctx.set_step_index(0)
+        return {"Nt": Nt, "beam": beam, "h": h, "t": t_axis}
 
-        ctx = self.ctx
-        ctx.wake_probe(self.ixt, self.iyt, h, max(Nt, 1))
-        ctx.set_step_index(0)
-        t0 = time.time()
-        per_step = callback is not None or (use_field_monitor and field_monitor is not None)
-        if per_step:
-            for n in range(Nt):
-                beam.update(self, n * self.dt)
-                self.one_step()
-                if use_field_monitor and field_monitor is not None:
-                    field_monitor.update(self.E, self.dt)
-                if callback is not None:
-                    callback(self, n * self.dt)
+    def _wake_device_beam(self, run):
+        """The Gaussian bunch evaluated on the device at every step."""
+        beam = run["beam"]
+        params = beam.device_params(self)
+        self.ctx.set_beam_gauss(int(beam.ixs), int(beam.iys), params, self.z)
+        self._beam_ij = (int(beam.ixs), int(beam.iys))
+        if self.step_0:
+            self.ctx.zero_ghosts()
+            self.step_0 = False
+
+    def _wake_device_end(self, run):
+        """Leave J as the reference leaves it: the last beam line (held by a
+        non-conductive solver) or sigma*E (a conductive step overwrote it)."""
+        beam, Nt = run["beam"], run["Nt"]
+        if self.use_conductivity or Nt == 0:
+            self.ctx.beam_off()
+            self._beam_ij = None
         else:
-            params = beam.device_params(self)
-            ctx.set_beam_gauss(int(beam.ixs), int(beam.iys), params, self.z)
-            self._beam_ij = (int(beam.ixs), int(beam.iys))
-            done = 0
-            while done < Nt:
-                n = min(chunk, Nt - done)
-                self._advance(n)
-                done += n
-            ctx.sync()
-            # leave J as the reference leaves it: the last beam line (held by a
-            # non-conductive solver) or sigma*E (a conductive step overwrote it)
-            if self.use_conductivity or Nt == 0:
-                ctx.beam_off()
-                self._beam_ij = None
-            else:
-                self._set_beam_line(int(beam.ixs), int(beam.iys),
-                                    beam.line(self, (Nt - 1) * self.dt))
-        self.simulationTime = time.time() - t0
+            self._set_beam_line(int(beam.ixs), int(beam.iys), beam.line(self, (Nt - 1) * self.dt))
 
+    def _wake_finish(self, run, compute_plane):
+        """Hand the device E_z history to the WakeSolver and solve
+        (wakis/routines.py:456-485)."""
+        h = run["h"]
         if h == 1:
             xx = slice(self.ixt - 1, self.ixt + 2)
             yy = slice(self.iyt - 1, self.iyt + 2)
         else:
             xx, yy = self.ixt, self.iyt
         meta = {
-            "x": self.x[xx], "y": self.y[yy], "z": z[slice(0, self.Nz)],
+            "x": self.x[xx], "y": self.y[yy], "z": self.z[slice(0, self.Nz)],
             "dx": self.grid.dx, "dy": self.grid.dy, "dz": self.grid.dz,
-            "t": t_axis, "nsteps": Nt, "half": h,
+            "t": run["t"], "nsteps": run["Nt"], "half": h,
         }
-        self.wake.attach_device_history(ctx, meta)
+        self.wake.attach_device_history(self.ctx, meta)
         self.wake.solve(compute_plane=compute_plane)

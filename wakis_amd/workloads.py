@@ -80,6 +80,48 @@ def lossy_pillbox(nx=256, ny=256, nz=768):
     )
 
 
+def rect_cavity(side, half_pipe, length, dx=1.5e-3, beta=1.0, n_pml=10, sigmaz=10e-3,
+                wakelength=1.0, name=None):
+    """Rectangular PEC cavity: square box of the given side and length, a
+    square beam pipe of half-width half_pipe running one cavity length out
+    on each side (total z = 3*length), vacuum inside, PEC elsewhere, grid
+    spacing dx in every direction, PEC x/y faces and a graded PML on both z
+    ends; bunch q = 1 nC, sigma_z, beta, wakelength."""
+    nxy = max(8, int(round(side / dx)))
+    nz = max(16, int(round(3 * length / dx)))
+    hx = nxy * dx / 2
+    hz = nz * dx / 2
+    x = np.linspace(-hx, hx, nxy + 1)
+    y = np.linspace(-hx, hx, nxy + 1)
+    z = np.linspace(-hz, hz, nz + 1)
+    mask = rect_cavity_mask(x, y, z, side / 2, length / 2, half_pipe)
+    return dict(
+        name=name or f"rect_cavity_{nxy}x{nxy}x{nz}",
+        grid=dict(x=x, y=y, z=z, solids=[(mask, "vacuum")]),
+        solver=dict(bc_low=["pec", "pec", "pml"], bc_high=["pec", "pec", "pml"], bg="pec",
+                    use_stl=True, n_pml=n_pml, cfln=1.0),
+        wake=dict(q=1e-9, sigmaz=sigmaz, beta=float(beta), wakelength=wakelength),
+    )
+
+
+def cavity_batch(n=64, seed=3, dx=1.5e-3, wakelength=1.0, sigmaz=10e-3, n_pml=10):
+    """A batch of n rectangular PEC cavities with box side U(60, 140) mm, pipe
+    half-width U(10, 25) mm, length U(60, 160) mm and beta from
+    {0.8, 0.9, 1.0}, all drawn from one seed; dx = 1.5 mm gives grids from
+    about 40 x 40 x 120 to 93 x 93 x 320 cells."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        side = rng.uniform(0.060, 0.140)
+        pipe = rng.uniform(0.010, 0.025)
+        length = rng.uniform(0.060, 0.160)
+        beta = float(rng.choice([0.8, 0.9, 1.0]))
+        out.append(rect_cavity(side, min(pipe, 0.45 * side), length, dx=dx, beta=beta,
+                               n_pml=n_pml, sigmaz=sigmaz, wakelength=wakelength,
+                               name=f"cavity{i:02d}"))
+    return out
+
+
 def build(workload: dict, dtype=np.float64, device=0, kernel="auto", save=False):
     """(grid, wake, solver) of the wakis_amd API for a workload dict."""
     from . import GridFIT3D, SolverFIT3D, WakeSolver
