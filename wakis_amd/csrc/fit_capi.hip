@@ -224,7 +224,7 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
     }
     if (ev) HIPCHK(hipEventRecord(ev[3 * s], c->stream));
     if (fused) {
-      const int cfg = c->fused_cfg >= 0 ? c->fused_cfg : fit::sweep_default(c->nx, c->dtype == FIT_F64);
+      const int cfg = c->fused_cfg >= 0 ? c->fused_cfg : fit::sweep_default(c->nx, c->ny, c->nz, c->dtype == FIT_F64);
       fit::launch_sweep<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg);
       std::swap(c->E, c->E_alt);
       std::swap(c->H, c->H_alt);
@@ -768,7 +768,7 @@ int fit_set_tile(fit_ctx *c, int32_t cfg) {
 
 int fit_get_tile(fit_ctx *c, int32_t *cfg, int32_t *tx, int32_t *ty, int32_t *kc) {
   if (!c) return fail(FIT_EINVAL, "null context");
-  const int k = c->fused_cfg >= 0 ? c->fused_cfg : fit::sweep_default(c->nx, c->dtype == FIT_F64);
+  const int k = c->fused_cfg >= 0 ? c->fused_cfg : fit::sweep_default(c->nx, c->ny, c->nz, c->dtype == FIT_F64);
   int a = 0, b = 0, d = 0;
   fit::sweep_dims(k, a, b, d);
   if (cfg) *cfg = k;

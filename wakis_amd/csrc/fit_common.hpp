@@ -29,6 +29,11 @@ enum { BEAM_OFF = 0, BEAM_VALUES = 1, BEAM_GAUSS = 2 };
 template <typename T>
 using CT = __attribute__((address_space(4))) T;
 
+// T in the global address space (vector memory instructions, not flat ones,
+// also for a pointer that was read from memory).
+template <typename T>
+using GT = __attribute__((address_space(1))) T;
+
 // Material table row: dt/mu x,y,z | dt/eps x,y,z | sigma x,y,z.
 constexpr int MAT_STRIDE = 9;
 constexpr int MAX_MATERIALS = 256;
@@ -165,7 +170,7 @@ __device__ __forceinline__ void probe(const StepArgs<T> &a, int i, int j, int k,
     const int di = i - a.ixt, dj = j - a.iyt;
     if (di >= -a.hw && di <= a.hw && dj >= -a.hw && dj <= a.hw) {
       const int w = (di + a.hw) * (2 * a.hw + 1) + (dj + a.hw);
-      a.hist[((int64_t)w * a.nz + k) * a.hist_nt + a.step] = ez;
+      ((GT<T> *)a.hist)[((int64_t)w * a.nz + k) * a.hist_nt + a.step] = ez;
     }
   }
 }

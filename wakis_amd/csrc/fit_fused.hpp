@@ -129,32 +129,37 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
   const int k0 = kc * KC;
   const int k1 = min(k0 + KC, nz);  // planes k0 .. k1-1
 
-  const T *__restrict__ Ex = a.E;
-  const T *__restrict__ Ey = a.E + N;
-  const T *__restrict__ Ez = a.E + 2 * (int64_t)N;
-  const T *__restrict__ Hx = a.H;
-  const T *__restrict__ Hy = a.H + N;
-  const T *__restrict__ Hz = a.H + 2 * (int64_t)N;
-  T *__restrict__ Enx = f.En;
-  T *__restrict__ Eny = f.En + N;
-  T *__restrict__ Enz = f.En + 2 * (int64_t)N;
-  T *__restrict__ Hnx = f.Hn;
-  T *__restrict__ Hny = f.Hn + N;
-  T *__restrict__ Hnz = f.Hn + 2 * (int64_t)N;
-  const uint8_t *__restrict__ mat = a.mat;
+  // Field and table pointers typed as global memory (address space 1): in
+  // the batched launch they are read from memory, and a generic pointer
+  // would make every access a flat instruction.
+  const GT<T> *__restrict__ Ex = (const GT<T> *)a.E;
+  const GT<T> *__restrict__ Ey = Ex + N;
+  const GT<T> *__restrict__ Ez = Ex + 2 * (int64_t)N;
+  const GT<T> *__restrict__ Hx = (const GT<T> *)a.H;
+  const GT<T> *__restrict__ Hy = Hx + N;
+  const GT<T> *__restrict__ Hz = Hx + 2 * (int64_t)N;
+  GT<T> *__restrict__ Enx = (GT<T> *)f.En;
+  GT<T> *__restrict__ Eny = Enx + N;
+  GT<T> *__restrict__ Enz = Enx + 2 * (int64_t)N;
+  GT<T> *__restrict__ Hnx = (GT<T> *)f.Hn;
+  GT<T> *__restrict__ Hny = Hnx + N;
+  GT<T> *__restrict__ Hnz = Hnx + 2 * (int64_t)N;
+  const GT<T> *__restrict__ Jx = (const GT<T> *)a.J;
+  const GT<uint8_t> *__restrict__ mat = (const GT<uint8_t> *)a.mat;
   const Geo<T> &g = a.g;
   // read-only tables in the constant address space: wave-uniform entries
   // (one row j per wave, one plane per iteration) are scalar loads
   const CT<T> *gLz = (const CT<T> *)g.Lz;
   const CT<T> *gtLz = (const CT<T> *)g.tLz;
   const CT<T> *giAx = (const CT<T> *)g.iAx;
-  const T *__restrict__ giAy = g.iAy;
+  const GT<T> *__restrict__ giAy = (const GT<T> *)g.iAy;
   const CT<T> *gitAx = (const CT<T> *)g.itAx;
-  const T *__restrict__ gitAy = g.itAy;
+  const GT<T> *__restrict__ gitAy = (const GT<T> *)g.itAy;
   const CT<T> *jline = (const CT<T> *)a.jline;
+  const CT<T> *mtab = (const CT<T> *)a.mtab;
   const uint32_t pec = a.pec, pmc = a.pmc;
 
-  for (int t = threadIdx.x; t < a.nmat * MAT_STRIDE; t += NT) sm.mt[t] = a.mtab[t];
+  for (int t = threadIdx.x; t < a.nmat * MAT_STRIDE; t += NT) sm.mt[t] = mtab[t];
 
   // ---- staging cells of this thread (window slots q) ----
   // Only window cells that an active lane reads are loaded: columns up to
@@ -468,9 +473,9 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
       // buffer mode never share registers with the other modes, so their
       // wait does not hold back the loads in flight for the next planes.
       if (a.jmode == J_BUFFER) {
-        const T jx = a.J[n];
-        const T jy = (a.J + N)[n];
-        const T jz = beam_col ? jcur : (a.J + 2 * (int64_t)N)[n];
+        const T jx = Jx[n];
+        const T jy = (Jx + N)[n];
+        const T jz = beam_col ? jcur : (Jx + 2 * (int64_t)N)[n];
         Enx[n] = eox + mt[3] * (ax * cx - jx);
         Eny[n] = eoy + mt[4] * (ay * cy - jy);
         Enz[n] = eoz + mt[5] * (az * cz - jz);
@@ -548,6 +553,7 @@ __global__ void __launch_bounds__(TX *TY) fused_sweep(FusedArgs<T> f) {
 }
 
 // ---------------- batched launch ----------------
+
 // One member grid of a batch: its step arguments for both buffer parities
 // (parity p: E, H read from f[p].a, written to f[p].En/Hn), its step index
 // when the batch counter is 0, and its beam table (nz values per step of the
@@ -634,9 +640,25 @@ inline void fused_dims(int cfg, int &tx, int &ty, int &kc) {
   }
 }
 
-// Default when no tuning was done: full 256-wide rows when the grid has them.
+// Cost model of a tile configuration for a grid: cells the tiles cover
+// (partial tiles padded) times the configuration's time per covered cell,
+// measured on the 256 x 256 x 768 FP32 pillbox (ms per step per 5.03e7
+// cells: 256x4 0.645, 64x8 0.657, 128x4 0.687, 64x4 0.689, 256x2 K64 0.78,
+// 256x2 1.30).  Used when no tuning was done.
+inline double fused_cost(int cfg, int nx, int ny, int nz) {
+  int tx, ty, kc;
+  fused_dims(cfg, tx, ty, kc);
+  static const double w[FUSED_NCFG] = {1.30, 0.687, 0.657, 0.645, 0.689, 0.78};
+  const double cx = (double)((nx + tx - 1) / tx) * tx;
+  const double cy = (double)((ny + ty - 1) / ty) * ty;
+  const double cz = (double)((nz + kc - 1) / kc) * kc;
+  return cx * cy * cz * w[cfg];
+}
+
+// Default when no tuning was done: full 256-wide rows when the grid has them
+// (256 x 4 measured fastest in FP32 on the 256 x 256 x 768 pillbox).
 inline int fused_default(int nx, bool fp64) {
-  if (nx >= 192 && !fp64) return FUSED_CFG_256x2;
+  if (nx >= 192 && !fp64) return FUSED_CFG_256x4;
   if (nx >= 96) return FUSED_CFG_128x4;
   return FUSED_CFG_64x8;
 }
@@ -656,6 +678,24 @@ inline bool fused_cfg_ok(int cfg) {
     case FUSED_CFG_64x4: return cfg_fits<T, 64, 4>();
     default: return cfg_fits<T, 256, 2>();
   }
+}
+
+// The configuration of least modelled cost over n grids (one grid, or the
+// members of a batch) among those that fit the LDS for T.
+template <typename T>
+inline int fused_pick(const int *nx, const int *ny, const int *nz, int n) {
+  int best = FUSED_CFG_128x4;
+  double bc = 1e300;
+  for (int cfg = 0; cfg < FUSED_NCFG; ++cfg) {
+    if (!fused_cfg_ok<T>(cfg)) continue;
+    double c = 0;
+    for (int i = 0; i < n; ++i) c += fused_cost(cfg, nx[i], ny[i], nz[i]);
+    if (c < bc) {
+      bc = c;
+      best = cfg;
+    }
+  }
+  return best;
 }
 
 template <typename T, int TX, int TY, int KC>

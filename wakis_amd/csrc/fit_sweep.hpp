@@ -699,10 +699,12 @@ inline bool sweep_cfg_ok(int cfg) {
   return cfg < FUSED_NCFG ? fused_cfg_ok<T>(cfg) : row_cfg_ok<T>(cfg - FUSED_NCFG);
 }
 
-// Default when no tuning was done: the fused_sweep tiles (measured faster
-// than the row-wave tiles on the 256 x 256 x 768 pillbox: 0.695 vs 0.726 ms
-// per FP32 step); the autotuner considers both families.
-inline int sweep_default(int nx, bool fp64) { return fused_default(nx, fp64); }
+// Default when no tuning was done: the fused_sweep tile of least modelled
+// cost (fused_cost; the fused_sweep family measured faster than the row-wave
+// tiles on the 256 x 256 x 768 pillbox); the autotuner considers both.
+inline int sweep_default(int nx, int ny, int nz, bool fp64) {
+  return fp64 ? fused_pick<double>(&nx, &ny, &nz, 1) : fused_pick<float>(&nx, &ny, &nz, 1);
+}
 
 // Whether configuration cfg runs as itself for these args (a row-sweep
 // configuration that cannot run falls back to the fused_sweep default).

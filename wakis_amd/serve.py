@@ -1,0 +1,317 @@
+"""Serving loop: impedance requests arriving over time, solved with
+continuous batching on one device.
+
+Requests (cavity wake + impedance solves of different sizes) arrive as a
+Poisson process.  A request is admitted when the device memory it needs fits
+the HBM budget next to the solves already resident; admitted solves keep
+their fields on the device and are stepped together by the batched launch
+(``wakis_amd.batch``), in chunks, so requests that arrive while others run
+join the batch at the next chunk boundary and their work overlaps on the
+device.  A solve leaves the batch after its own Nt steps; its wake
+potential and impedance are then computed on the device and its memory is
+released.  Each member's arithmetic is the same as a solve alone, so the
+results do not depend on the mix.
+
+The host side of a new request (geometry, materials, metric tables, upload)
+is prepared by a worker thread while the device steps the current batch
+(ctypes releases the GIL during library calls).
+
+There is no counterpart in the reference (it runs one solve per process,
+wakis/routines.py:231-485).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import queue
+import threading
+import time
+from typing import Callable
+
+import numpy as np
+
+from .batch import BatchStepper
+from .workloads import build, cells_from_nodes
+
+EPS0 = 8.8541878128e-12
+
+
+@dataclasses.dataclass
+class Request:
+    """One impedance solve: a cavity workload and its arrival time (s)."""
+    rid: int
+    arrival: float
+    workload: dict
+    cells: int
+    # filled by the server
+    admitted: float = float("nan")
+    finished: float = float("nan")
+    steps: int = 0
+    result: dict | None = None
+    prep_s: float = float("nan")     # host preparation (worker thread)
+    finish_s: float = float("nan")   # wake potential + impedance after the last step
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def serve_cavity(shape, radius, length, pipe, cells, slab=None, n_pml=10, sigmaz=10e-3,
+                 beta=1.0, wakelength=1.0, name="req"):
+    """A rectangular (side 2*radius) or staircased cylindrical cavity of the
+    given length with a beam pipe of half-width/radius `pipe` running one
+    cavity length out on each side, meshed with cubic cells so that the grid
+    has about `cells` cells; PEC walls, graded PML on both z ends.  `slab`
+    (eps_r, sigma) adds a dielectric insert filling y < -radius/2 over the
+    middle half of the cavity length."""
+    wt = 2.0 * radius
+    lz = 3.0 * length
+    dx = (wt * wt * lz / float(cells)) ** (1.0 / 3.0)
+    nxy = max(8, int(round(wt / dx)))
+    nz = max(16, int(round(lz / dx)))
+    x = np.linspace(-radius, radius, nxy + 1)
+    y = np.linspace(-radius, radius, nxy + 1)
+    z = np.linspace(-lz / 2, lz / 2, nz + 1)
+    if shape == "rect":
+        bx = (np.abs(x) <= radius)[:, None, None]
+        by = (np.abs(y) <= radius)[None, :, None]
+        px = (np.abs(x) <= pipe)[:, None, None]
+        py = (np.abs(y) <= pipe)[None, :, None]
+        bz = (np.abs(z) <= length / 2)[None, None, :]
+        vac = cells_from_nodes((bx & by & bz) | (px & py & np.ones_like(bz)))
+    else:
+        r2 = x[:, None] ** 2 + y[None, :] ** 2
+        cav = (r2 <= radius**2)[:, :, None] & (np.abs(z) <= length / 2)[None, None, :]
+        pp = (r2 <= pipe**2)[:, :, None] & np.ones((1, 1, len(z)), dtype=bool)
+        vac = cells_from_nodes(cav | pp)
+    solids = [(vac, "vacuum")]
+    if slab is not None:
+        eps_r, sigma = slab
+        sy = (y[:-1] + y[1:]) / 2 < -radius / 2
+        sz = np.abs((z[:-1] + z[1:]) / 2) <= length / 4
+        m = vac & sy[None, :, None] & sz[None, None, :]
+        solids.append((m, [float(eps_r), 1.0, float(sigma)]))
+    return dict(
+        name=name,
+        grid=dict(x=x, y=y, z=z, solids=solids),
+        # use_sibc=False: the slab is a lossy dielectric (eps_r kept), not a
+        # conductor whose eps the reference's surface-impedance rule replaces
+        # by sigma*eps_0 (wakis/solverFIT3D.py:471-472)
+        solver=dict(bc_low=["pec", "pec", "pml"], bc_high=["pec", "pec", "pml"], bg="pec",
+                    use_stl=True, use_sibc=False, n_pml=n_pml, cfln=0.5),
+        wake=dict(q=1e-9, sigmaz=sigmaz, beta=float(beta), wakelength=wakelength),
+    )
+
+
+def make_requests(n, seed=4, rate=2.0, median_cells=8e6, sigma_ln=1.0,
+                  clip=(1e6, 1.2e8), slab_prob=0.5, f_ref=1e9, wakelength=1.0):
+    """n requests with Poisson arrivals of the given rate (1/s): rectangular
+    or cylindrical cavities with radius U(45, 75) mm, length U(60, 100) mm,
+    pipe U(15, 25) mm, a lognormal cell count (median, sigma_ln) clipped to
+    `clip`, and with probability slab_prob a dielectric insert with
+    eps_r U(2, 10) and loss tangent U(1e-4, 1e-2), i.e. a conductivity
+    sigma = 2*pi*f_ref*eps_0*eps_r*tan(delta)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    out = []
+    for i in range(n):
+        shape = "rect" if rng.random() < 0.5 else "cyl"
+        radius = rng.uniform(0.045, 0.075)
+        length = rng.uniform(0.060, 0.100)
+        pipe = rng.uniform(0.015, 0.025)
+        cells = float(np.clip(median_cells * np.exp(sigma_ln * rng.standard_normal()), *clip))
+        slab = None
+        if rng.random() < slab_prob:
+            eps_r = rng.uniform(2.0, 10.0)
+            tand = rng.uniform(1e-4, 1e-2)
+            slab = (eps_r, 2 * np.pi * f_ref * EPS0 * eps_r * tand)
+        wl = serve_cavity(shape, radius, length, pipe, cells, slab=slab, wakelength=wakelength,
+                          name=f"req{i:03d}")
+        n_cells = (len(wl["grid"]["x"]) - 1) * (len(wl["grid"]["y"]) - 1) * (len(wl["grid"]["z"]) - 1)
+        out.append(Request(rid=i, arrival=float(t[i]), workload=wl, cells=n_cells))
+    return out
+
+
+def device_bytes(n_cells: int, nz: int, nt: int, esz: int) -> int:
+    """Device memory of one resident solve: E, H, their second buffers and J
+    (5 x 3N), material ids, metric tables, the E_z probe history (one
+    window) and slack for the beam table."""
+    return int(15 * n_cells * esz + n_cells + 64 * (n_cells ** (2 / 3)) * esz + nz * nt * esz + (1 << 20))
+
+
+class Server:
+    """Continuous-batching server on one device.
+
+    budget: device bytes the resident solves may use (default 288 GB).
+    chunk_steps: steps per batch call (a request admitted while a chunk runs
+    joins at the next call).  clock: seconds since start (wall time by
+    default; tests pass a virtual clock)."""
+
+    def __init__(self, dtype=np.float32, budget=288e9, chunk_steps=256, graph=True, group=16,
+                 tile=None, compute_plane="longitudinal", clock: Callable[[], float] | None = None,
+                 prep_workers=2, precision="mixed"):
+        self.dtype = np.dtype(dtype)
+        self.precision = precision
+        self.budget = float(budget)
+        self.chunk_steps = int(chunk_steps)
+        self.graph, self.group, self.tile = graph, group, tile
+        self.compute_plane = compute_plane
+        self._t0 = None
+        self._clock = clock
+        self.prep_workers = prep_workers
+        self.log = []
+
+    def now(self) -> float:
+        if self._clock is not None:
+            return self._clock()
+        return time.monotonic() - self._t0
+
+    # -- host preparation of a request (worker thread) ------------------------
+    def _prepare(self, req: Request):
+        t0 = time.monotonic()
+        grid, wake, solver = build(req.workload, dtype=self.dtype, precision=self.precision)
+        if self.tile is not None:
+            solver.ctx.set_tile(int(self.tile))
+        run = solver._wake_setup(None, None, self.compute_plane)
+        solver._wake_device_beam(run)
+        solver.ctx.sync()
+        esz = self.dtype.itemsize
+        mem = device_bytes(solver.N, solver.Nz, run["Nt"], esz)
+        req.prep_s = time.monotonic() - t0
+        return {"req": req, "solver": solver, "wake": wake, "run": run, "done": 0, "mem": mem}
+
+    def run(self, requests):
+        """Serve the requests (sorted by arrival); returns them with latency,
+        steps and results filled in."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        self._t0 = time.monotonic()
+        ready: queue.Queue = queue.Queue()
+        todo: queue.Queue = queue.Queue()
+        stop = threading.Event()
+
+        def worker():
+            while not stop.is_set():
+                try:
+                    r = todo.get(timeout=0.05)
+                except queue.Empty:
+                    continue
+                try:
+                    ready.put(self._prepare(r))
+                except BaseException as e:  # surfaced by the main loop
+                    ready.put({"req": r, "error": e})
+
+        threads = [threading.Thread(target=worker, daemon=True) for _ in range(self.prep_workers)]
+        for th in threads:
+            th.start()
+        nxt = 0          # next request not yet handed to the workers
+        waiting = []     # prepared, not admitted (memory)
+        active = []      # resident and stepping
+        used = 0.0
+        stepper = None
+        finished = 0
+        try:
+            while finished < len(reqs):
+                now = self.now()
+                while nxt < len(reqs) and reqs[nxt].arrival <= now:
+                    todo.put(reqs[nxt])
+                    nxt += 1
+                while True:
+                    try:
+                        item = ready.get_nowait()
+                    except queue.Empty:
+                        break
+                    if "error" in item:
+                        raise item["error"]
+                    waiting.append(item)
+                # admission against the budget, in arrival order
+                waiting.sort(key=lambda it: it["req"].arrival)
+                keep = []
+                for it in waiting:
+                    if used + it["mem"] <= self.budget or not active:
+                        it["req"].admitted = self.now()
+                        used += it["mem"]
+                        active.append(it)
+                    else:
+                        keep.append(it)
+                waiting = keep
+                if not active:
+                    # idle: wait for the next arrival or preparation
+                    if nxt < len(reqs) and self._clock is None:
+                        time.sleep(max(0.0005, min(0.01, reqs[nxt].arrival - self.now())))
+                    else:
+                        time.sleep(0.002)
+                    continue
+                n = min(self.chunk_steps, min(it["run"]["Nt"] - it["done"] for it in active))
+                members = [it["solver"] for it in active]
+                if stepper is None:
+                    stepper = BatchStepper(members, graph=self.graph, group=self.group,
+                                           tile=self.tile)
+                else:
+                    stepper.set_members(members)
+                ms = stepper.step(n) if n > 0 else 0.0
+                for it in active:
+                    it["done"] += n
+                    it["req"].steps = it["done"]
+                self.log.append((self.now(), len(active), n, used, ms,
+                                 sum(it["solver"].N for it in active)))
+                still = []
+                for it in active:
+                    if it["done"] >= it["run"]["Nt"]:
+                        s = it["solver"]
+                        tf = time.monotonic()
+                        s._wake_device_end(it["run"])
+                        s._wake_finish(it["run"], self.compute_plane)
+                        w = it["wake"]
+                        it["req"].result = {"WP": np.asarray(w.WP).copy(), "Z": np.asarray(w.Z).copy(),
+                                            "f": np.asarray(w.f).copy(), "s": np.asarray(w.s).copy(),
+                                            "Nt": it["run"]["Nt"], "N": s.N}
+                        it["req"].finished = self.now()
+                        it["req"].finish_s = time.monotonic() - tf
+                        s.ctx.close()
+                        used -= it["mem"]
+                        finished += 1
+                    else:
+                        still.append(it)
+                active = still
+        finally:
+            stop.set()
+            for th in threads:
+                th.join(timeout=5)
+            if stepper is not None:
+                stepper.close()
+        return reqs
+
+
+def solve_alone(req: Request, dtype=np.float64, tile=None, compute_plane="longitudinal",
+                precision="mixed"):
+    """One request by itself with the plain per-solver wakesolve (the
+    one-at-a-time path); returns (seconds, result dict)."""
+    grid, wake, solver = build(req.workload, dtype=dtype, precision=precision)
+    if tile is not None:
+        solver.ctx.set_tile(int(tile))
+    t0 = time.monotonic()
+    solver.wakesolve(compute_plane=compute_plane)
+    solver.ctx.sync()
+    dt = time.monotonic() - t0
+    res = {"WP": np.asarray(wake.WP).copy(), "Z": np.asarray(wake.Z).copy(),
+           "f": np.asarray(wake.f).copy(), "Nt": solver.Nt, "N": solver.N}
+    solver.ctx.close()
+    return dt, res
+
+
+def fifo_latencies(arrivals, durations):
+    """Latencies of a one-at-a-time FIFO server with the given service times."""
+    t = 0.0
+    out = []
+    for a, d in zip(arrivals, durations):
+        start = max(t, a)
+        t = start + d
+        out.append(t - a)
+    return np.asarray(out)
+
+
+def z_rel_error(z, z_ref, f, fcut):
+    """max |Z - Z_ref| / max |Z_ref| over f < fcut."""
+    m = np.asarray(f) < fcut
+    zr = np.asarray(z_ref)[m]
+    return float(np.max(np.abs(np.asarray(z)[m] - zr)) / max(np.max(np.abs(zr)), 1e-300))

@@ -105,7 +105,14 @@ class SolverFIT3D(RoutinesMixin):
                  bc_high=("Periodic", "Periodic", "Periodic"),
                  use_stl=False, use_conductors=False, use_gpu=True, use_mpi=False,
                  use_sibc=True, fmax=1e9, dtype=np.float64, n_pml=10, bg=(1.0, 1.0),
-                 verbose=1, device=0, kernel="auto", tile=None):
+                 verbose=1, device=0, kernel="auto", tile=None, precision="reference"):
+        """wakis/solverFIT3D.py:40-376.  precision (this build's addition):
+        "reference" rounds dt and the material values to `dtype` as the
+        reference does; "mixed" keeps dt, the material values and the beam
+        time in FP64 (the device coefficient table dt/mu, dt/eps, sigma is
+        computed in FP64 and stored in `dtype`), so an FP32 run shares the
+        FP64 run's time axis and differs from it only by the field
+        arithmetic and storage."""
         t0 = time.time()
         if use_mpi:
             raise NotImplementedError(
@@ -138,6 +145,12 @@ class SolverFIT3D(RoutinesMixin):
         if wake is not None and fmax == 1e9:
             self.fmax = wake.fmax
         self.dtype = dtype
+        if precision not in ("reference", "mixed"):
+            raise ValueError("precision: 'reference' or 'mixed'")
+        self.precision = precision
+        # dtype of the coefficient setup (dt, 1/eps, 1/mu, sigma)
+        self.cdtype = np.float64 if precision == "mixed" else dtype
+        cdtype = self.cdtype
         self.bc_low, self.bc_high = list(bc_low), list(bc_high)
 
         # boundaries (wakis/boundaries.py:14-188)
@@ -159,12 +172,12 @@ class SolverFIT3D(RoutinesMixin):
             self.use_conductivity = True
         else:
             self.eps_bg, self.mu_bg, self.sigma_bg = bg[0] * eps_0, bg[1] * mu_0, 0.0
-        dt_np = np.dtype(dtype).type
-        ie = _cast(dtype, np.ones(1, dtype=dtype) * (1.0 / self.eps_bg))
-        im = _cast(dtype, np.ones(1, dtype=dtype) * (1.0 / self.mu_bg))
-        sg = _cast(dtype, np.ones(1, dtype=dtype) * self.sigma_bg)
+        dt_np = np.dtype(cdtype).type
+        ie = _cast(cdtype, np.ones(1, dtype=cdtype) * (1.0 / self.eps_bg))
+        im = _cast(cdtype, np.ones(1, dtype=cdtype) * (1.0 / self.mu_bg))
+        sg = _cast(cdtype, np.ones(1, dtype=cdtype) * self.sigma_bg)
         self._mt = _MaterialTable((self.Nx, self.Ny, self.Nz), [ie] * 3 + [im] * 3 + [sg] * 3,
-                                  dtype)
+                                  cdtype)
         if use_stl:
             self._apply_stl_materials()
         if self.activate_pml:
@@ -235,7 +248,7 @@ class SolverFIT3D(RoutinesMixin):
     # -- materials -----------------------------------------------------------
     def _apply_stl_materials(self):
         """wakis/solverFIT3D.py:438-502 on the grid's solid masks."""
-        dtype = self.dtype
+        dtype = self.cdtype
         for name, mask, mat in self.grid.solids:
             if type(mat) is str:
                 mat = material_lib[mat.lower()]
@@ -266,7 +279,7 @@ class SolverFIT3D(RoutinesMixin):
         faces in the order xlo, ylo, zlo, xhi, yhi, zhi."""
         n = self.n_pml
         lo, hi = 5.0e-3, 1.0
-        ie = _cast(self.dtype, 1 / eps_0)
+        ie = _cast(self.cdtype, 1 / eps_0)
         dims = (self.Nx, self.Ny, self.Nz)
         order = [(0, "lo"), (1, "lo"), (2, "lo"), (0, "hi"), (1, "hi"), (2, "hi")]
         for ax, side in order:
@@ -276,7 +289,7 @@ class SolverFIT3D(RoutinesMixin):
             prof = np.geomspace(hi, lo, n) if side == "lo" else np.geomspace(lo, hi, n)
             for p in range(n):
                 plane = p if side == "lo" else dims[ax] - n + p
-                sig = _cast(self.dtype, 0.0 + 1 * prof[p])
+                sig = _cast(self.cdtype, 0.0 + 1 * prof[p])
                 region = [slice(None)] * 3
                 region[ax] = plane
                 self._mt.apply(tuple(region),
@@ -287,8 +300,8 @@ class SolverFIT3D(RoutinesMixin):
         (wakis/solverFIT3D.py:309-318), evaluated on the material rows in use."""
         used = np.unique(self._mt.ids)
         tab = self._mt.table()[used]
-        ieps = tab[:, 0:3].reshape(-1).astype(self.dtype)
-        sig = tab[:, 6:9].reshape(-1).astype(self.dtype)
+        ieps = tab[:, 0:3].reshape(-1).astype(self.cdtype)
+        sig = tab[:, 6:9].reshape(-1).astype(self.cdtype)
         mask = np.logical_and(sig != 0, ieps != 0)
         tau = (1 / ieps[mask]) / sig[mask]
         if self.dt > tau.min():

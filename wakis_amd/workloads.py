@@ -63,7 +63,11 @@ def lossy_pillbox(nx=256, ny=256, nz=768):
     Walls are a conductor sigma = 1e5 S/m given as a solid (so, as in the
     reference's STL path with use_sibc, eps = sigma*eps_0 there); 10-cell PML
     (the reference's geomspace grading) on both z ends, PEC on x and y;
-    bunch sigma_z = 10 mm, beta = 1, wakelength 5 m."""
+    bunch sigma_z = 10 mm, beta = 1, wakelength 5 m.  CFL factor 0.5 (the
+    reference's default): the conductivity term J = sigma*E is explicit
+    (wakis/solverFIT3D.py:514-525), and at the CFL limit the leapfrog mode of
+    highest frequency then grows (its amplification solves
+    l^2 + (2 + a) l + 1 - a = 0 with a = dt*sigma/eps > 0, so |l| > 1)."""
     half_t = 0.06
     dx = 2 * half_t / nx
     half_z = dx * nz / 2
@@ -75,7 +79,7 @@ def lossy_pillbox(nx=256, ny=256, nz=768):
         name=f"lossy_pillbox_{nx}x{ny}x{nz}",
         grid=dict(x=x, y=y, z=z, solids=[(~vac, [1.0, 1.0, 1e5])]),
         solver=dict(bc_low=["pec", "pec", "pml"], bc_high=["pec", "pec", "pml"], bg="vacuum",
-                    use_stl=True, n_pml=10, cfln=1.0),
+                    use_stl=True, n_pml=10, cfln=0.5),
         wake=dict(q=1e-9, sigmaz=10e-3, beta=1.0, wakelength=5.0),
     )
 
@@ -86,7 +90,8 @@ def rect_cavity(side, half_pipe, length, dx=1.5e-3, beta=1.0, n_pml=10, sigmaz=1
     square beam pipe of half-width half_pipe running one cavity length out
     on each side (total z = 3*length), vacuum inside, PEC elsewhere, grid
     spacing dx in every direction, PEC x/y faces and a graded PML on both z
-    ends; bunch q = 1 nC, sigma_z, beta, wakelength."""
+    ends (CFL factor 0.5: the PML conductivity is explicit, see
+    lossy_pillbox); bunch q = 1 nC, sigma_z, beta, wakelength."""
     nxy = max(8, int(round(side / dx)))
     nz = max(16, int(round(3 * length / dx)))
     hx = nxy * dx / 2
@@ -99,7 +104,7 @@ def rect_cavity(side, half_pipe, length, dx=1.5e-3, beta=1.0, n_pml=10, sigmaz=1
         name=name or f"rect_cavity_{nxy}x{nxy}x{nz}",
         grid=dict(x=x, y=y, z=z, solids=[(mask, "vacuum")]),
         solver=dict(bc_low=["pec", "pec", "pml"], bc_high=["pec", "pec", "pml"], bg="pec",
-                    use_stl=True, n_pml=n_pml, cfln=1.0),
+                    use_stl=True, n_pml=n_pml, cfln=0.5),
         wake=dict(q=1e-9, sigmaz=sigmaz, beta=float(beta), wakelength=wakelength),
     )
 
@@ -122,11 +127,13 @@ def cavity_batch(n=64, seed=3, dx=1.5e-3, wakelength=1.0, sigmaz=10e-3, n_pml=10
     return out
 
 
-def build(workload: dict, dtype=np.float64, device=0, kernel="auto", save=False):
+def build(workload: dict, dtype=np.float64, device=0, kernel="auto", save=False,
+          precision="reference"):
     """(grid, wake, solver) of the wakis_amd API for a workload dict."""
     from . import GridFIT3D, SolverFIT3D, WakeSolver
 
     g = GridFIT3D(**workload["grid"])
     w = WakeSolver(save=save, **workload["wake"])
-    s = SolverFIT3D(g, w, dtype=dtype, device=device, kernel=kernel, **workload["solver"])
+    s = SolverFIT3D(g, w, dtype=dtype, device=device, kernel=kernel, precision=precision,
+                    **workload["solver"])
     return g, w, s
