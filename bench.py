@@ -124,6 +124,12 @@ def bench_batch(args):
     seq_value = cells * seq_steps / seq_s / 1e9
     ms_step = ms / args.steps
     alg = 12 * 4 * cells
+    dims = {0: (256, 2, 32), 1: (128, 4, 32), 2: (64, 8, 32), 3: (256, 4, 32), 4: (64, 4, 32),
+            5: (256, 2, 64)}.get(cfg)
+    traffic, tsrc = (None, None)
+    if dims and len(solvers) == 64:
+        traffic, tsrc = traffic_from_profiles(f"fused_batch<float, {dims[0]}, {dims[1]}, {dims[2]}>",
+                                              "f32", "batch64", cfg)
     achieved = alg / (ms_step * 1e-3) / 1e9
     sizes = sorted(s.N for s in solvers)
     return {
@@ -149,8 +155,8 @@ def bench_batch(args):
         },
         "roofline": {"bound": "hbm", "kernel": "fused_batch", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "alg_bytes_per_launch": alg,
-                     "avg_launch_ms": round(ms_step, 5)},
+                     "traffic": traffic, "traffic_source": f"profiles/{tsrc}" if tsrc else None,
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": round(ms_step, 5)},
     }
 
 
@@ -223,6 +229,70 @@ def bench_serve(args):
     }
 
 
+def bench_precision(args):
+    """FP32 against FP64 on the lossy pillbox (the full wakesolve: wakelength
+    5 m): relative L2 error of E and H after the run and of the wake
+    potential, and max |Z - Z_64| / max |Z_64| over the kept bins (all below
+    the bunch cutoff fmax = v/(3 sigma_z)), for FP32 with the reference's
+    dtype rounding of dt and material values ("reference") and for FP32
+    fields with an FP64 dt, time axis and coefficient setup ("mixed")."""
+    from wakis_amd.workloads import build, lossy_pillbox
+
+    nx, ny, nz = (int(v) for v in args.grid.split(","))
+    wl = lossy_pillbox(nx, ny, nz)
+    if args.wakelength:
+        wl["wake"]["wakelength"] = args.wakelength
+    out = {}
+    res = {}
+    for name, dtype, prec in (("fp64", np.float64, "reference"), ("fp32_mixed", np.float32, "mixed"),
+                              ("fp32_reference", np.float32, "reference")):
+        _, w, s = build(wl, dtype=dtype, precision=prec)
+        t0 = time.perf_counter()
+        s.wakesolve(compute_plane="longitudinal")
+        s.ctx.sync()
+        el = time.perf_counter() - t0
+        res[name] = {"E": s.E.toarray().astype(np.float64), "H": s.H.toarray().astype(np.float64),
+                     "WP": np.asarray(w.WP, dtype=np.float64), "Z": np.asarray(w.Z),
+                     "f": np.asarray(w.f), "Nt": s.Nt, "s": np.asarray(w.s)}
+        out[name] = {"seconds": round(el, 2), "Nt": int(s.Nt),
+                     "gcell_steps_per_s": round(s.N * s.Nt / el / 1e9, 2)}
+        print(f"[bench] precision {name}: Nt {s.Nt} {el:.1f}s", file=sys.stderr, flush=True)
+        s.ctx.close()
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    ref = res["fp64"]
+    for name in ("fp32_mixed", "fp32_reference"):
+        r = res[name]
+        same = r["Nt"] == ref["Nt"] and r["WP"].shape == ref["WP"].shape
+        out[name].update({
+            "same_time_axis": bool(same),
+            "E_rel_l2": rel(r["E"], ref["E"]) if same else None,
+            "H_rel_l2": rel(r["H"], ref["H"]) if same else None,
+            "WP_rel_l2": rel(r["WP"], ref["WP"]) if same else None,
+            "Z_max_rel": (float(np.max(np.abs(r["Z"] - ref["Z"])) / np.max(np.abs(ref["Z"])))
+                          if r["Z"].shape == ref["Z"].shape else None),
+        })
+    return {
+        "metric": "FP32 vs FP64 relative error",
+        "value": out["fp32_mixed"]["Z_max_rel"],
+        "unit": "max |dZ| / max |Z_fp64|",
+        "n_gpus": 1,
+        "steps": int(ref["Nt"]),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"lossy pillbox {nx}x{ny}x{nz}, wakelength "
+                               f"{wl['wake']['wakelength']} m, full wakesolve (longitudinal)",
+                   "runs": out, "target": "Z within 1e-3 relative below the bunch cutoff"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,7 +304,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--tile", type=int, default=None, help="fused tile configuration id")
-    ap.add_argument("--workload", default="pillbox", choices=["pillbox", "batch", "serve"],
+    ap.add_argument("--wakelength", type=float, default=None, help="precision: wakelength (m)")
+    ap.add_argument("--workload", default="pillbox", choices=["pillbox", "batch", "serve", "precision"],
                     help="pillbox: one 256x256x768 solve (the headline); batch: 64 cavities in "
                          "one launch per step; serve: Poisson impedance requests")
     ap.add_argument("--ncav", type=int, default=64, help="batch: number of cavities")
@@ -250,6 +321,9 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(bench_serve(args)), flush=True)
+        return
+    if args.workload == "precision":
+        print(json.dumps(bench_precision(args)), flush=True)
         return
 
     from wakis_amd.workloads import build, lossy_pillbox
