@@ -293,10 +293,131 @@ def bench_precision(args):
     }
 
 
+def bench_large(args):
+    """One large solve, FP64 against FP32: the stratified box (n^3 cells,
+    n = 1024 by default: 1.07e9 cells; 16 dielectric layers eps_r U(1, 12),
+    sigma U(0, 10) S/m, a diagonal-tensor central block, 12-cell PML on all
+    six faces), the Gaussian bunch (sigma_z = 5 mm, beta = 1) evaluated on
+    the device and the E_z probe, `--steps` steps (2000 by default) from
+    zero fields.  Both solves are resident at once (FP64 E, H, their second
+    buffers and J: 15 x 8 B per cell; FP32: 15 x 4 B), stepped one after the
+    other; then the relative L2 error of FP32 E, H and the E_z probe history
+    against FP64, read back slab by slab."""
+    from wakis_amd import _lib
+    from wakis_amd.sources import Beam
+    from wakis_amd.workloads import build, stratified_box
+
+    n = args.n_large
+    wl = stratified_box(n=n)
+    runs, out = {}, {}
+    for name, dtype, prec in (("fp64", np.float64, "reference"), ("fp32_mixed", np.float32, "mixed")):
+        t0 = time.perf_counter()
+        _, wake, s = build(wl, dtype=dtype, precision=prec)
+        setup_s = time.perf_counter() - t0
+        ctx = s.ctx
+        cond = s.use_conductivity
+        # untimed warmup and tile search on the zero fields (they stay zero:
+        # no source, J = sigma*E = 0), then the bunch and the probe from step 0
+        tune_ms = None
+        if not args.no_tune and ctx.kernel_in_use() == _lib.KERNEL_FUSED:
+            tune_ms = s.tune(reps=2)
+        ctx.step(4, cond)
+        ctx.sync()
+        beam = Beam(q=wake.q, sigmaz=wake.sigmaz, beta=wake.beta)
+        ixs, iys = int(np.abs(s.x).argmin()), int(np.abs(s.y).argmin())
+        ctx.wake_probe(ixs, iys, 0, args.steps)
+        ctx.set_step_index(0)
+        ctx.set_beam_gauss(ixs, iys, beam.device_params(s), s.z)
+        ctx.zero_ghosts()
+        s.step_0 = False
+        tile = ctx.get_tile()
+        print(f"[bench] large {name}: {s.N} cells, setup {setup_s:.1f}s, dt {float(s.dt):.6e}, "
+              f"tile {tile}, tune {tune_ms}", file=sys.stderr, flush=True)
+        total_ms, done = 0.0, 0
+        chunk = 250
+        while done < args.steps:
+            k = min(chunk, args.steps - done)
+            ms, _, _ = ctx.step_timed(k, cond, per_kernel=False)
+            total_ms += ms
+            done += k
+            print(f"[bench] large {name}: step {done}/{args.steps}, {ms / k:.3f} ms/step",
+                  file=sys.stderr, flush=True)
+        runs[name] = s
+        out[name] = {"setup_s": round(setup_s, 1), "ms_per_step": round(total_ms / args.steps, 4),
+                     "gcell_steps_per_s": round(s.N * args.steps / (total_ms * 1e-3) / 1e9, 3),
+                     "tile": {"cfg": tile[0], "tx": tile[1], "ty": tile[2], "kc": tile[3]},
+                     "esz": np.dtype(dtype).itemsize}
+
+    # FP32 against FP64, z-slab by z-slab (full x-y planes are contiguous)
+    r64, r32 = runs["fp64"], runs["fp32_mixed"]
+    slab = max(1, (1 << 26) // (n * n))
+    errs = {}
+    for fname, which in (("E", _lib.FIELD_E), ("H", _lib.FIELD_H)):
+        num = den = 0.0
+        for c in range(3):
+            for k0 in range(0, n, slab):
+                box = [0, n, 0, n, k0, min(n, k0 + slab)]
+                a = r64.ctx.read_box(which, c, box)
+                b = r32.ctx.read_box(which, c, box).astype(np.float64)
+                num += float(np.sum((b - a) ** 2))
+                den += float(np.sum(a * a))
+        errs[f"{fname}_rel_l2"] = float(np.sqrt(num / max(den, 1e-300)))
+        print(f"[bench] large {fname} rel L2 {errs[fname + '_rel_l2']:.3e}", file=sys.stderr,
+              flush=True)
+    h64 = r64.ctx.wake_history().astype(np.float64)
+    h32 = r32.ctx.wake_history().astype(np.float64)
+    errs["Ez_probe_rel_l2"] = float(np.linalg.norm(h32 - h64) / max(np.linalg.norm(h64), 1e-300))
+    finite = bool(np.isfinite(h64).all() and np.isfinite(h32).all())
+
+    # average fused-kernel launch over a few more steps (after the comparison)
+    for name, s in runs.items():
+        nk = 20
+        _, k1, _ = s.ctx.step_timed(nk, s.use_conductivity, per_kernel=True)
+        esz = out[name]["esz"]
+        alg = 12 * esz * s.N
+        kms = k1 / nk
+        out[name]["roofline"] = {"bound": "hbm", "kernel": "fused_sweep",
+                                 "achieved": round(alg / (kms * 1e-3) / 1e9, 1),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                 "alg_bytes_per_launch": alg, "avg_launch_ms": round(kms, 4)}
+    N = r32.N
+    f32 = out["fp32_mixed"]
+    tile = f32["tile"]
+    traffic, tsrc = traffic_from_profiles(
+        f"fused_sweep<float, {tile['tx']}, {tile['ty']}, {tile['kc']}>", "f32", f"{n}x{n}x{n}",
+        tile["cfg"])
+    roof = dict(f32["roofline"], traffic=traffic,
+                traffic_source=f"profiles/{tsrc}" if tsrc else None)
+    for s in runs.values():
+        s.ctx.close()
+    return {
+        "metric": "Gcell-steps/s",
+        "value": f32["gcell_steps_per_s"],
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 4,
+        "ms_per_step": f32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"stratified box {n}x{n}x{n} ({N} cells): 16 z-layers eps_r "
+                               "U(1,12) sigma U(0,10) S/m, diagonal-tensor central block, "
+                               "12-cell PML on six faces, bunch sigma_z 5 mm beta 1, "
+                               f"{args.steps} steps, FP32 (mixed) vs FP64",
+                   "runs": out, "fp32_vs_fp64": errs, "finite": finite},
+        "roofline": roof,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 1000; large: 2000 steps of the solve)")
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--grid", default="256,256,768")
@@ -305,13 +426,18 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--tile", type=int, default=None, help="fused tile configuration id")
     ap.add_argument("--wakelength", type=float, default=None, help="precision: wakelength (m)")
-    ap.add_argument("--workload", default="pillbox", choices=["pillbox", "batch", "serve", "precision"],
+    ap.add_argument("--workload", default="pillbox",
+                    choices=["pillbox", "batch", "serve", "precision", "large"],
                     help="pillbox: one 256x256x768 solve (the headline); batch: 64 cavities in "
-                         "one launch per step; serve: Poisson impedance requests")
+                         "one launch per step; serve: Poisson impedance requests; large: "
+                         "the 1024^3 stratified box, FP32 vs FP64")
     ap.add_argument("--ncav", type=int, default=64, help="batch: number of cavities")
     ap.add_argument("--nreq", type=int, default=24, help="serve: number of requests")
     ap.add_argument("--chunk", type=int, default=256, help="serve: steps per batch call")
+    ap.add_argument("--n-large", type=int, default=1024, help="large: cells per side")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 2000 if args.workload == "large" else 1000
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
               file=sys.stderr)
@@ -324,6 +450,9 @@ def main():
         return
     if args.workload == "precision":
         print(json.dumps(bench_precision(args)), flush=True)
+        return
+    if args.workload == "large":
+        print(json.dumps(bench_large(args)), flush=True)
         return
 
     from wakis_amd.workloads import build, lossy_pillbox

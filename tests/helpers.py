@@ -61,3 +61,41 @@ def rel(a, b):
 
 def compare_fields(osv, psv):
     return {n: rel(getattr(psv, n).toarray(), getattr(osv, n).toarray()) for n in "EHJ"}
+
+
+def region_mask(shape, region):
+    m = np.zeros(shape, bool)
+    m[region] = True
+    return m
+
+
+def stratified_pair(n=12, nlayers=4, n_pml=2, dtype=np.float64, device=0, seed=5, kernel="auto"):
+    """The stratified-box workload (index-slice solids, a diagonal-tensor
+    block) as a wakis_amd solver and as an oracle solver; the oracle gets the
+    slice boxes as masks and the tensor block as per-component edits of its
+    ieps Field followed by update_tensors (the reference's route to an
+    anisotropic material)."""
+    import wakis_amd as wa
+    from wakis_amd.workloads import stratified_box
+
+    from scipy.constants import epsilon_0 as eps_0
+    wl = stratified_box(n=n, seed=seed, half=0.256 * n / 1024, n_pml=n_pml, nlayers=nlayers)
+    g = wl["grid"]
+    sv = wl["solver"]
+    shape = (n, n, n)
+    iso, tensors = [], []
+    for region, mat in g["solids"]:
+        (tensors if isinstance(mat, dict) else iso).append((region, mat))
+    og = OGrid(x=g["x"], y=g["y"], z=g["z"])
+    osv = OSolver(og, cfln=sv["cfln"], bc_low=sv["bc_low"], bc_high=sv["bc_high"],
+                  solids=[(region_mask(shape, r), m) for r, m in iso], bg=sv["bg"], dtype=dtype,
+                  n_pml=sv["n_pml"], use_sibc=sv["use_sibc"])
+    for region, mat in tensors:
+        for d, e in zip("xyz", mat["eps"]):
+            osv.ieps[region + (d,)] = np.float64(1.0) / (e * eps_0)
+    if tensors:
+        osv.update_tensors("ieps")
+        osv.step_0 = True
+    psv = wa.SolverFIT3D(wa.GridFIT3D(x=g["x"], y=g["y"], z=g["z"], solids=g["solids"]),
+                         dtype=dtype, device=device, kernel=kernel, **sv)
+    return osv, psv, wl

@@ -251,3 +251,33 @@ def test_tune_keeps_state():
         psv.one_step()
     err = compare_fields(osv, psv)
     assert max(err.values()) <= TOL64, err
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["fp64", "fp32"])
+def test_stratified_box_tensor_block_beam(dtype):
+    """The stratified-box workload at 16^3: index-slice dielectric layers
+    with conductivity, a diagonal-tensor block, graded PML on all six faces
+    and the bunch on the axis (entering within the run), 40 steps from
+    random fields."""
+    import wakis_amd as wa
+    from helpers import stratified_pair
+
+    print(f"[case] stratified box 16^3 {np.dtype(dtype).name}", flush=True)
+    osv, psv, wl = stratified_pair(n=16, nlayers=4, n_pml=3, dtype=dtype)
+    rng = np.random.default_rng(21)
+    for name in ("E", "H"):
+        a = rng.standard_normal(3 * osv.N).astype(dtype)
+        getattr(osv, name).fromarray(a)
+        getattr(psv, name).fromarray(a)
+    w = wl["wake"]
+    ti = 2 * w["sigmaz"] / 299792458.0
+    ob = OBeam(q=w["q"], sigmaz=w["sigmaz"], beta=1.0, ti=ti)
+    pb = wa.Beam(q=w["q"], sigmaz=w["sigmaz"], beta=1.0, ti=ti)
+    for n in range(40):
+        ob.update(osv, n * osv.dt)
+        pb.update(psv, n * psv.dt)
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    print(err)
+    assert max(err.values()) <= (TOL64 if dtype == np.float64 else TOL32), err

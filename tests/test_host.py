@@ -188,3 +188,31 @@ def test_bucket_columns_match_reference_expression():
             ts = (zs[k] + s[n]) / w.v - zc.min() / w.v - w.t[0] + w.ti
             it = int(ts / (w.t[2] - w.t[1]))
             assert cols[k, n] == (it if it < nt else -1)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_slice_solids_and_tensor_material_equal_oracle(dtype):
+    """Index-slice solids and a diagonal-tensor material {"eps": (ex, ey, ez)}
+    give the material Fields the oracle holds after per-component ieps edits
+    (the stratified-box workload at 12^3, 2-cell PML on all faces)."""
+    from helpers import stratified_pair
+
+    osv, psv, wl = stratified_pair(n=12, nlayers=4, n_pml=2, dtype=dtype, device=None)
+    for name in ("ieps", "imu", "sigma"):
+        assert np.array_equal(getattr(psv, name).toarray(),
+                              getattr(osv, name).toarray().astype(np.float64)), name
+    assert psv.dt == osv.dt and psv.use_conductivity == osv.use_conductivity
+    # the tensor block really is anisotropic
+    b = psv.ieps.toarray().reshape(3, -1)[:, np.ravel_multi_index((6, 6, 6), (12, 12, 12), order="F")]
+    assert len(set(b.tolist())) == 3
+
+
+def test_stratified_box_host_setup_is_lean():
+    """The 1024^3 box's setup holds no (n, n, n) solid masks: its solids are
+    slice boxes and its material ids fit the 256-row device table."""
+    from wakis_amd.workloads import stratified_box
+
+    wl = stratified_box(n=1024)
+    assert all(isinstance(r, tuple) for r, _ in wl["grid"]["solids"])
+    assert len(wl["grid"]["solids"]) == 17
+    assert wl["grid"]["x"].size == 1025

@@ -68,6 +68,11 @@ class GridFIT3D:
                 self.solids.append((key, self._mark_cells_in_stl(key), self.stl_materials[key]))
         if solids is not None:
             for n, (mask, mat) in enumerate(solids):
+                if isinstance(mask, tuple) and len(mask) == 3 and all(isinstance(m, slice) for m in mask):
+                    # a box of cells given as index slices (no (Nx, Ny, Nz) mask
+                    # array: grids of 1e9 cells)
+                    self.solids.append((f"solid{n}", mask, mat))
+                    continue
                 mask = np.asarray(mask, dtype=bool)
                 if mask.shape != (self.Nx, self.Ny, self.Nz):
                     raise ValueError("solid mask must have shape (Nx, Ny, Nz)")

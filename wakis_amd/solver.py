@@ -67,7 +67,8 @@ class _MaterialTable:
         self.dtype = dtype
         self.rows = [tuple(row0)]
         self.index = {tuple(row0): 0}
-        self.ids = np.zeros(shape, dtype=np.uint16)
+        # Fortran order: the device's flat index i + Nx*(j + Ny*k)
+        self.ids = np.zeros(shape, dtype=np.uint16, order="F")
 
     def intern(self, row):
         row = tuple(float(v) for v in row)
@@ -79,7 +80,11 @@ class _MaterialTable:
     def apply(self, region, fn):
         """ids[region] := id(fn(rows[old id])) for every old id in region."""
         sub = self.ids[region]
-        old = np.unique(sub)
+        if sub.size == 0:
+            return
+        lo, hi = int(sub.min()), int(sub.max())
+        old = (np.array([lo]) if lo == hi else
+               lo + np.flatnonzero(np.bincount(sub.ravel(order="K") - np.uint16(lo))))
         lut = np.arange(len(self.rows), dtype=np.uint16)
         for o in old:
             nid = self.intern(fn(self.rows[int(o)]))
@@ -250,6 +255,31 @@ class SolverFIT3D(RoutinesMixin):
         """wakis/solverFIT3D.py:438-502 on the grid's solid masks."""
         dtype = self.cdtype
         for name, mask, mat in self.grid.solids:
+            region = mask if isinstance(mask, tuple) else np.asarray(mask, dtype=bool)
+            if isinstance(mat, dict):
+                # per-component material (this build's extension for diagonal
+                # tensors, which the reference sets through the ieps/imu/sigma
+                # Fields and update_tensors): {"eps": r or (rx, ry, rz),
+                # "mu": ..., "sigma": ...}; no surface-impedance rule
+                er = np.broadcast_to(np.asarray(mat.get("eps", 1.0), dtype=np.float64), (3,))
+                mr = np.broadcast_to(np.asarray(mat.get("mu", 1.0), dtype=np.float64), (3,))
+                ie3 = tuple(_cast(dtype, np.float64(1.0) / (e * eps_0)) for e in er)
+                im3 = tuple(_cast(dtype, np.float64(1.0) / (m * mu_0)) for m in mr)
+                sg = mat.get("sigma", None)
+                sg3 = None
+                if sg is not None:
+                    sg3 = tuple(_cast(dtype, v) for v in
+                                np.broadcast_to(np.asarray(sg, dtype=np.float64), (3,)))
+                    if any(v > 0 for v in sg3):
+                        self.use_conductivity = True
+                elif self.sigma_bg > 0.0:
+                    sg3 = (0.0, 0.0, 0.0)
+
+                def fnd(row, ie3=ie3, im3=im3, sg3=sg3):
+                    return ie3 + im3 + (tuple(row[6:9]) if sg3 is None else sg3)
+
+                self._mt.apply(region, fnd)
+                continue
             if type(mat) is str:
                 mat = material_lib[mat.lower()]
             mat = list(mat)
@@ -271,7 +301,7 @@ class SolverFIT3D(RoutinesMixin):
                 s = row[6:9] if new_sigma is None else (new_sigma,) * 3
                 return (ie, ie, ie, im, im, im) + tuple(s)
 
-            self._mt.apply(np.asarray(mask, dtype=bool), fn)
+            self._mt.apply(region, fn)
 
     def _initialize_PML(self):
         """wakis/boundaries.py:190-305: ieps := 1/eps_0 and a geomspace sigma
@@ -298,7 +328,7 @@ class SolverFIT3D(RoutinesMixin):
     def _relaxation_clamp(self):
         """dt <= min(eps/sigma) over conductive, non-PEC entries
         (wakis/solverFIT3D.py:309-318), evaluated on the material rows in use."""
-        used = np.unique(self._mt.ids)
+        used = np.flatnonzero(np.bincount(self._mt.ids.ravel(order="K")))
         tab = self._mt.table()[used]
         ieps = tab[:, 0:3].reshape(-1).astype(self.cdtype)
         sig = tab[:, 6:9].reshape(-1).astype(self.cdtype)

@@ -127,6 +127,36 @@ def cavity_batch(n=64, seed=3, dx=1.5e-3, wakelength=1.0, sigmaz=10e-3, n_pml=10
     return out
 
 
+def stratified_box(n=1024, seed=5, half=0.256, n_pml=12, sigmaz=5e-3, nlayers=16):
+    """Large single solve: an n^3 box of side 2*half (n = 1024: dx = 0.5 mm,
+    1.07e9 cells) filled with `nlayers` z-stratified dielectric layers
+    (eps_r U(1, 12), sigma U(0, 10) S/m, drawn from `seed`) and a central
+    cube of side n/4 whose permittivity is a diagonal tensor (eps_x, eps_y,
+    eps_z each U(1, 12), the layer conductivity kept), graded PML of n_pml
+    cells on all six faces, Gaussian bunch q = 1 nC, sigma_z, beta = 1 on
+    the axis.  Solids are index-slice boxes, so no (n, n, n) mask is built.
+    CFL factor 0.5: with a = dt*sigma/eps <= 0.6 here the explicit
+    conductivity term stays stable (see lossy_pillbox)."""
+    rng = np.random.default_rng(seed)
+    x = np.linspace(-half, half, n + 1)
+    solids = []
+    bounds = np.linspace(0, n, nlayers + 1).round().astype(int)
+    for z0, z1 in zip(bounds[:-1], bounds[1:]):
+        er = float(rng.uniform(1.0, 12.0))
+        sg = float(rng.uniform(0.0, 10.0))
+        solids.append(((slice(None), slice(None), slice(int(z0), int(z1))), [er, 1.0, sg]))
+    b0, b1 = n // 2 - n // 8, n // 2 + n // 8
+    eps3 = tuple(float(v) for v in rng.uniform(1.0, 12.0, 3))
+    solids.append(((slice(b0, b1),) * 3, {"eps": eps3, "mu": 1.0}))
+    return dict(
+        name=f"stratified_box_{n}x{n}x{n}",
+        grid=dict(x=x, y=x.copy(), z=x.copy(), solids=solids),
+        solver=dict(bc_low=["pml"] * 3, bc_high=["pml"] * 3, bg="vacuum", use_stl=True,
+                    use_sibc=False, n_pml=n_pml, cfln=0.5),
+        wake=dict(q=1e-9, sigmaz=sigmaz, beta=1.0, wakelength=1.0),
+    )
+
+
 def build(workload: dict, dtype=np.float64, device=0, kernel="auto", save=False,
           precision="reference"):
     """(grid, wake, solver) of the wakis_amd API for a workload dict."""
