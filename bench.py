@@ -125,7 +125,7 @@ def bench_batch(args):
     ms_step = ms / args.steps
     alg = 12 * 4 * cells
     dims = {0: (256, 2, 32), 1: (128, 4, 32), 2: (64, 8, 32), 3: (256, 4, 32), 4: (64, 4, 32),
-            5: (256, 2, 64)}.get(cfg)
+            5: (256, 4, 64)}.get(cfg)
     traffic, tsrc = (None, None)
     if dims and len(solvers) == 64:
         traffic, tsrc = traffic_from_profiles(f"fused_batch<float, {dims[0]}, {dims[1]}, {dims[2]}>",

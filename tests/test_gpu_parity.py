@@ -185,9 +185,9 @@ def test_fused_multi_tile_fp64(name, bl, bh, tile):
     solids = [(block_mask(shape, (3, 2, 5), (Nx - 4, Ny - 3, Nz - 6)), "vacuum"),
               (block_mask(shape, (40, 5, 20), (90, 12, 40)), [3.0, 1.2, 20.0]),
               (block_mask(shape, (60, 8, 30), (70, 15, 50)), "pec")]
-    # tile 3 (256 x 4) does not fit the LDS in FP64: the default tile runs
+    # tiles 3 and 5 (256 x 4) do not fit the LDS in FP64: the default tile runs
     osv, psv = make_pair(*shape, bl, bh, solids=solids, seed=21, n_pml=4, kernel="fused",
-                         tile=None if tile == 3 else tile)
+                         tile=None if tile in (3, 5) else tile)
     for _ in range(6):
         osv.one_step()
         psv.one_step()
@@ -222,6 +222,51 @@ def test_fused_tiles_fp32(tile):
         psv.one_step()
     err = compare_fields(osv, psv)
     assert max(err.values()) <= TOL32, err
+
+
+@pytest.mark.parametrize("tile", list(range(12)))
+def test_fused_inner_chunks_fp32(tile):
+    """Chunks away from the z ends (the lean buffer-addressed path) in FP32
+    for every tile configuration: 80 planes give chunks starting at plane 32
+    (KC = 32) that touch neither z face; periodic x/y wrap, PMC/PEC faces,
+    conductive solids."""
+    shape = (137, 13, 80)
+    print(f"[case] fused inner chunks fp32 tile={tile} shape={shape}", flush=True)
+    Nx, Ny, Nz = shape
+    solids = [(block_mask(shape, (2, 1, 4), (Nx - 3, Ny - 2, Nz - 5)), "vacuum"),
+              (block_mask(shape, (30, 3, 35), (80, 9, 60)), [3.0, 1.5, 40.0]),
+              (block_mask(shape, (90, 2, 33), (100, 11, 70)), "pec")]
+    osv, psv = make_pair(*shape, ["periodic", "pmc", "pml"], ["pec", "periodic", "pml"],
+                         dtype=np.float32, solids=solids, seed=41, kernel="fused", tile=tile)
+    for _ in range(8):
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    print(err)
+    assert max(err.values()) <= TOL32, err
+
+
+@pytest.mark.parametrize("tile", [1, 2, 4])
+def test_fused_inner_chunks_jbuffer_beam(tile):
+    """The lean path in the J buffer mode (a non-conductive solver whose J is
+    written from the host: Beam.update in a user loop), FP64, 80 planes."""
+    import wakis_amd as wa
+
+    shape = (70, 9, 80)
+    print(f"[case] fused inner chunks J buffer + beam tile={tile}", flush=True)
+    osv, psv = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, seed=43, kernel="fused", tile=tile,
+                         solids=[(block_mask(shape, (1, 1, 0), (69, 8, 80)), "vacuum")],
+                         bg="pec")
+    ob = OBeam(q=1e-9, sigmaz=0.004, beta=1.0, ti=0.0)
+    pb = wa.Beam(q=1e-9, sigmaz=0.004, beta=1.0, ti=0.0)
+    for n in range(12):
+        ob.update(osv, n * osv.dt)
+        pb.update(psv, n * psv.dt)
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    print(err)
+    assert max(err.values()) <= TOL64, err
 
 
 def test_tiny_grids_fused():

@@ -109,7 +109,8 @@ __global__ void beam_table(StepArgs<T> a, int64_t step0, int nsteps, T *__restri
   out[t] = beam_gauss<T>(a, k);
 }
 
-// J := sigma*E (jmode J_SIGMA_E) or 0 (J_ZERO), then the beam line.
+// J := sigma*E (jmode J_SIGMA_E) or 0 (J_ZERO), or kept as it is (jmode < 0:
+// J lives in its buffer), then the beam line.
 template <typename T>
 __global__ void materialize_j(StepArgs<T> a) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -117,10 +118,13 @@ __global__ void materialize_j(StepArgs<T> a) {
   const int m = a.mat ? a.mat[n] : 0;
   const T *mt = a.mtab + m * MAT_STRIDE;
   T *J = const_cast<T *>(a.J);
-  for (int c = 0; c < 3; ++c) {
-    T v = T(0);
-    if (a.jmode == J_SIGMA_E) v = mt[6 + c] * a.E[c * a.N + n];
-    J[c * a.N + n] = v;
+  // jmode < 0: J is in its buffer already (only the beam line is applied)
+  if (a.jmode >= 0) {
+    for (int c = 0; c < 3; ++c) {
+      T v = T(0);
+      if (a.jmode == J_SIGMA_E) v = mt[6 + c] * a.E[c * a.N + n];
+      J[c * a.N + n] = v;
+    }
   }
   if (a.beam_mode != BEAM_OFF) {
     const int64_t nxy = (int64_t)a.nx * a.ny;

@@ -124,6 +124,13 @@ fit::StepArgs<T> make_args(fit_ctx *c) {
   a.mat = c->mat_uniform ? nullptr : c->mat;
   a.mtab = (const T *)c->mtab;
   a.nmat = c->nmat;
+  {
+    const int64_t sz = (int64_t)sizeof(T);
+    const int64_t geo = 2 * ((int64_t)c->nx + c->ny + c->nz) +
+                        2 * ((int64_t)c->ny * c->nz + (int64_t)c->nx * c->nz + (int64_t)c->nx * c->ny);
+    a.geo_bytes = geo * sz;
+    a.buf32 = 3 * c->N * sz < ((int64_t)1 << 32) - 1 && a.geo_bytes < ((int64_t)1 << 32) - 1;
+  }
   const T *g = (const T *)c->geo;
   const int nx = c->nx, ny = c->ny, nz = c->nz;
   a.g.Lx = g;

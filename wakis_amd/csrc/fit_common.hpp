@@ -57,6 +57,8 @@ struct StepArgs {
   Geo<T> g;
   int nx, ny, nz;
   int64_t N;
+  bool buf32;      // a field (3N values) and the metric tables span < 2^32 bytes
+  int64_t geo_bytes;  // bytes of the packed metric tables (g.Lx first)
   uint32_t pec;    // tangential-E column mask faces
   uint32_t pmc;    // tangential-H row mask faces
   int jmode;
@@ -160,6 +162,38 @@ template <typename T>
 __device__ __forceinline__ T w_at(const StepArgs<T> &a, int comp, int i, int j, int k) {
   if (masked(a.pmc, comp, i, j, k, a.nx, a.ny, a.nz)) return T(0);
   return tLof(a.g, comp, i, j, k) * a.H[comp * a.N + flat(i, j, k, a.nx, a.ny)];
+}
+
+// Raw buffer access (CDNA buffer instructions): a 4-SGPR resource per array,
+// a per-lane 32-bit byte offset and a wave-uniform 32-bit byte offset, so an
+// address costs no vector arithmetic.  Used only where every offset is in
+// bounds and the array spans less than 2^32 bytes.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ Rsrc make_rsrc(const void *p, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)(uint32_t)bytes,
+                                           0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bload(Rsrc r, uint32_t v, uint32_t s);
+template <>
+__device__ __forceinline__ float bload<float>(Rsrc r, uint32_t v, uint32_t s) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)v, (int)s, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(Rsrc r, uint32_t v, uint32_t s) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)v, (int)s, 0));
+}
+// Stores are non-temporal (cache policy bit 1, nt): a step's output is read
+// only by the next step, after far more data than the caches hold.
+__device__ __forceinline__ void bstore(Rsrc r, uint32_t v, uint32_t s, float x) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)v, (int)s, 2);
+}
+__device__ __forceinline__ void bstore(Rsrc r, uint32_t v, uint32_t s, double x) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, (int)v, (int)s, 2);
+}
+__device__ __forceinline__ int bload_u8(Rsrc r, uint32_t v, uint32_t s) {
+  return (int)__builtin_amdgcn_raw_buffer_load_b8(r, (int)v, (int)s, 0);
 }
 
 // Wake probe: E_z of the state before the update of this step

@@ -70,16 +70,23 @@ struct Tile {
   static constexpr int RH = TY + 2;
   static constexpr int RC = RW * RH;
   static constexpr int RL = (RC + NT - 1) / NT;
-  static constexpr int NRING = 4;
+  static constexpr int NRING = 3;
 };
 
+// Static LDS of a workgroup; the material table (dt/mu, dt/eps, sigma rows)
+// follows in dynamic LDS, sized to the grid's table.
 template <typename T, int TX, int TY>
 struct FusedSmem {
   using G = Tile<TX, TY>;
   T u[G::NRING][3][G::RH][G::RW];   // u = L*pec*E_old, window planes (ring)
   T w[2][3][TY + 1][TX + 1];        // w = tL*pmc*H_new, tile + low halo
-  T mt[MAX_MATERIALS * MAT_STRIDE];  // material table (dt/mu, dt/eps, sigma)
 };
+
+// Dynamic LDS bytes of the material table of `nmat` rows.
+template <typename T>
+inline size_t fused_mt_bytes(int nmat) {
+  return (size_t)(nmat > 1 ? nmat : 1) * MAT_STRIDE * sizeof(T);
+}
 
 // A window cell's true coordinates, fixed over the march (dk: plane offset
 // from the y carry).
@@ -110,9 +117,12 @@ __device__ __forceinline__ WinCell win_cell(int gi, int gj, int nx, int ny) {
 // One tile (TX x TY columns, KC planes) of one step; `tile` is the tile
 // index in this grid's tile order.  Shared by the single-grid launch and the
 // batched launch, so a grid's arithmetic is the same alone and in a batch.
-template <typename T, int TX, int TY, int KC>
+// JB: the J buffer mode (J read from memory) or not (J = 0 or sigma*E); the
+// two are separate code, so the J loads never meet the other mode's values
+// in a register (a join there would make every wave wait for all its loads).
+template <typename T, int TX, int TY, int KC, bool JB>
 __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile,
-                                           FusedSmem<T, TX, TY> &sm) {
+                                           FusedSmem<T, TX, TY> &sm, T *__restrict__ smt) {
   using G = Tile<TX, TY>;
   static_assert(TX % 64 == 0, "a wave must cover one tile row");
   constexpr int NT = G::NT, RW = G::RW, RH = G::RH, RC = G::RC, RL = G::RL, NR = G::NRING;
@@ -159,7 +169,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
   const CT<T> *mtab = (const CT<T> *)a.mtab;
   const uint32_t pec = a.pec, pmc = a.pmc;
 
-  for (int t = threadIdx.x; t < a.nmat * MAT_STRIDE; t += NT) sm.mt[t] = mtab[t];
+  for (int t = threadIdx.x; t < a.nmat * MAT_STRIDE; t += NT) smt[t] = mtab[t];
 
   // ---- staging cells of this thread (window slots q) ----
   // Only window cells that an active lane reads are loaded: columns up to
@@ -185,6 +195,8 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
   // nz-2 every address is inside the grid and the loads are unconditional
   // (no per-lane branch, so the compiler never waits for the loads in
   // flight to reuse their registers); near the z ends they are checked.
+  // ring slot of window plane k (k >= -1; k is wave-uniform)
+  auto slot = [](int k) { return (k + 3) % NR; };
   auto fetch = [&](int k, T(&v)[RL][4]) {
     const bool inner = k >= 1 && k <= nz - 2;
 #pragma unroll
@@ -213,7 +225,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
     }
   };
   auto stage = [&](int k, const T(&v)[RL][4]) {
-    T(*dst)[RH][RW] = sm.u[k & (NR - 1)];
+    T(*dst)[RH][RW] = sm.u[slot(k)];
 #pragma unroll
     for (int q = 0; q < RL; ++q) {
       const int c = threadIdx.x + q * NT;
@@ -247,7 +259,8 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
   const T tLx0 = g.tLx[ic], tLy0 = g.tLy[jc];
   const T iAz0 = g.iAz[ic + nx * jc];
   const T itAz0 = g.itAz[ic + nx * jc];
-  const bool beam_col = a.beam_mode != BEAM_OFF && i == a.ixs && j == a.iys;
+  const bool beam_on = a.beam_mode != BEAM_OFF;  // wave-uniform
+  const bool beam_col = beam_on && i == a.ixs && j == a.iys;
 
   // ---- low-halo cell of this thread: row (x0+t, y0-1) -> H_x, H_z;
   //      column (x0-1, y0+t-TX) -> H_y, H_z ----
@@ -320,7 +333,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
       const int hk = kk + hc.dk;
       const bool ra = full_mask(pmc, hc0, hpmc[hc0], hk, nz);
       const bool rb = full_mask(pmc, 2, hpmc[2], hk, nz);
-      const T *mt = sm.mt + p.m * MAT_STRIDE;
+      const T *mt = smt + p.m * MAT_STRIDE;
       const T ha = p.ha - (ra ? T(0) : mt[hc0] * p.area_a) * curl_u(hc0, p0, p1, hli, hlj);
       const T hb = p.hb - (rb ? T(0) : mt[2] * h_iAz) * curl_u(2, p0, p1, hli, hlj);
       wa = ra ? T(0) : h_tla * ha;
@@ -328,6 +341,37 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
     }
     sm.w[s][hc0][hlj][hli] = wa;
     sm.w[s][2][hlj][hli] = wb;
+  };
+
+  // ---- prologue: planes k0-1, k0, k0+1 staged; w of planes k0-1 and k0 ----
+  // Packet consumed at iteration k, loaded one iteration earlier: the E
+  // update data of plane k and the H update data (own and halo) of plane
+  // k+1.  Two packets alternate (the loop body is written out twice), so no
+  // register of a load still in flight is ever copied (a copy would make the
+  // wave wait for the load there).
+  struct Pk {
+    T ex, ey, ez, itay;     // plane k: E_old, itA_y
+    T hx, hy, hz, iay;      // plane k+1: H_old, iA_y
+    int m;                  // material of plane k+1
+    HaloPk h;               // halo packet of plane k+1
+  };
+  // every lane loads (clamped column and plane: a value past the chunk or the
+  // grid is never used)
+  auto load_pk = [&](int k, Pk &p) {
+    const int ke = min(k, nz - 1);
+    const int ne = col + plane * ke;
+    p.ex = Ex[ne];
+    p.ey = Ey[ne];
+    p.ez = Ez[ne];
+    p.itay = gitAy[ic + nx * ke];
+    const int kh = min(k + 1, nz - 1);
+    const int nh = col + plane * kh;
+    p.hx = Hx[nh];
+    p.hy = Hy[nh];
+    p.hz = Hz[nh];
+    p.iay = giAy[ic + nx * kh];
+    p.m = mat ? mat[nh] : 0;
+    halo_load(k + 1, p.h);
   };
 
   // ---- prologue: planes k0-1, k0, k0+1 staged; w of planes k0-1 and k0 ----
@@ -341,17 +385,12 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
     stage(k0 + 1, pE);
   }
   if (k0 + 2 <= k1) fetch(k0 + 2, pE);
-  T eox = 0, eoy = 0, eoz = 0, itAy_k = 0;
-  int mcur = 0;
+  int mcur = 0;  // material of the plane of the E update
   T hx0o = 0, hy0o = 0, hz0o = 0, iAy0 = 0;
   T hxmo = 0, hymo = 0, iAym = 0;
   int mm = 0;
   if (own) {
     const int n0 = col + plane * k0;
-    eox = Ex[n0];
-    eoy = Ey[n0];
-    eoz = Ez[n0];
-    itAy_k = gitAy[ic + nx * k0];
     mcur = mat ? mat[n0] : 0;
     hx0o = Hx[n0];
     hy0o = Hy[n0];
@@ -367,17 +406,19 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
   }
   HaloPk hp0;
   halo_load(k0, hp0);
+  Pk A, B;
+  load_pk(k0, A);
   __syncthreads();
   T wxp = 0, wyp = 0;        // own w of plane k-1 (x, y)
   T wx = 0, wy = 0, wz = 0;  // own w of plane k
   {
     const int s = k0 & 1;
-    const int pm = (k0 - 1) & (NR - 1), p0 = k0 & (NR - 1), p1 = (k0 + 1) & (NR - 1);
+    const int pm = slot(k0 - 1), p0 = slot(k0), p1 = slot(k0 + 1);
     if (own && k0 >= 1) {
       const int km = k0 - 1;
       const bool r0 = full_mask(pmc, 0, opmc[0], km, nz);
       const bool r1 = full_mask(pmc, 1, opmc[1], km, nz);
-      const T *mt = sm.mt + mm * MAT_STRIDE;
+      const T *mt = smt + mm * MAT_STRIDE;
       const T hxm = hxmo - (r0 ? T(0) : mt[0] * giAx[jc + ny * km]) * curl_u(0, pm, p0, li, lj);
       const T hym = hymo - (r1 ? T(0) : mt[1] * iAym) * curl_u(1, pm, p0, li, lj);
       wxp = r0 ? T(0) : tLx0 * hxm;
@@ -388,13 +429,13 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
       const bool r0 = full_mask(pmc, 0, opmc[0], k0, nz);
       const bool r1 = full_mask(pmc, 1, opmc[1], k0, nz);
       const bool r2 = full_mask(pmc, 2, opmc[2], k0, nz);
-      const T *mt = sm.mt + mcur * MAT_STRIDE;
+      const T *mt = smt + mcur * MAT_STRIDE;
       const T hx = hx0o - (r0 ? T(0) : mt[0] * giAx[jc + ny * k0]) * curl_u(0, p0, p1, li, lj);
       const T hy = hy0o - (r1 ? T(0) : mt[1] * iAy0) * curl_u(1, p0, p1, li, lj);
       const T hz = hz0o - (r2 ? T(0) : mt[2] * iAz0) * curl_u(2, p0, p1, li, lj);
-      Hnx[n0] = hx;
-      Hny[n0] = hy;
-      Hnz[n0] = hz;
+      __builtin_nontemporal_store(hx, &Hnx[n0]);
+      __builtin_nontemporal_store(hy, &Hny[n0]);
+      __builtin_nontemporal_store(hz, &Hnz[n0]);
       wx = r0 ? T(0) : tLx0 * hx;
       wy = r1 ? T(0) : tLy0 * hy;
       wz = r2 ? T(0) : gtLz[k0] * hz;
@@ -404,89 +445,50 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
     sm.w[s][2][lj][li] = wz;
     if (hrow || hcolm) halo_w(k0, p0, p1, s, hp0);
   }
-  // packet of plane k0+1 for the H update at iteration k0 and the E update at
-  // iteration k0+1
-  T pHx = 0, pHy = 0, pHz = 0, pEx = 0, pEy = 0, pEz = 0, piAy = 0, pitAy = 0;
-  int pM = 0;
-  HaloPk php;
-  php.in = false;
-  php.ha = php.hb = php.area_a = php.tlz = 0;
-  php.m = 0;
-  T pJ = 0;
-  if (k0 + 1 < k1) {
-    if (own) {
-      const int n1 = col + plane * (k0 + 1);
-      pHx = Hx[n1];
-      pHy = Hy[n1];
-      pHz = Hz[n1];
-      pEx = Ex[n1];
-      pEy = Ey[n1];
-      pEz = Ez[n1];
-      pM = mat ? mat[n1] : 0;
-      piAy = giAy[ic + nx * (k0 + 1)];
-      pitAy = gitAy[ic + nx * (k0 + 1)];
-    }
-    halo_load(k0 + 1, php);
-  }
-  T jcur = beam_col ? jline[k0] : T(0);
-  if (beam_col && k0 + 1 < k1) pJ = jline[k0 + 1];
+  // the first iteration stages plane k0+2 into the slot of plane k0-1
+  __syncthreads();
 
-  for (int k = k0; k < k1; ++k) {
+  // One plane: E_new(k) and H_new(k+1) from `cur`; loads `nxt` for plane k+1.
+  auto iter = [&](const int k, const Pk &cur, Pk &nxt) {
     // 1. stage plane k+2 (its loads were issued one iteration ago)
     if (k + 2 <= k1) stage(k + 2, pE);
-    // 2. prefetch plane k+3 and the own/halo packet of plane k+2
+    // 2. prefetch window plane k+3 and the packet of the next iteration
     if (k + 3 <= k1) fetch(k + 3, pE);
-    // own packet of plane k+2 for every lane (clamped column and plane: a
-    // value past the chunk or the grid is never used)
-    const int k2 = min(k + 2, nz - 1);
-    const int n2 = col + plane * k2;
-    const T qHx = Hx[n2];
-    const T qHy = Hy[n2];
-    const T qHz = Hz[n2];
-    const T qEx = Ex[n2];
-    const T qEy = Ey[n2];
-    const T qEz = Ez[n2];
-    const int qM = mat ? mat[n2] : 0;
-    const T qiAy = giAy[ic + nx * k2];
-    const T qitAy = gitAy[ic + nx * k2];
-    HaloPk qhp;
-    halo_load(k + 2, qhp);
-    const T qJ = beam_col ? jline[k2] : T(0);
+    load_pk(k + 1, nxt);
     const int n = col + plane * k;
     const T itAx_k = gitAx[jc + ny * k];                       // scalar
     const T iAx_k1 = k + 1 < nz ? giAx[jc + ny * (k + 1)] : T(0);  // scalar
     const T tLz_k1 = k + 1 < nz ? gtLz[k + 1] : T(0);            // scalar
+    const T jk = beam_on ? jline[k] : T(0);                     // scalar
     // 3.
     __syncthreads();
     // 4. E_new(k)
     if (own) {
       const T(*ww)[TY + 1][TX + 1] = sm.w[k & 1];
-      const T *mt = sm.mt + mcur * MAT_STRIDE;
+      const T *mt = smt + mcur * MAT_STRIDE;
+      const T eox = cur.ex, eoy = cur.ey, eoz = cur.ez;
       probe(a, i, j, k, eoz);
       const T ax = full_mask(pec, 0, opec[0], k, nz) ? T(0) : itAx_k;
-      const T ay = full_mask(pec, 1, opec[1], k, nz) ? T(0) : itAy_k;
+      const T ay = full_mask(pec, 1, opec[1], k, nz) ? T(0) : cur.itay;
       const T az = full_mask(pec, 2, opec[2], k, nz) ? T(0) : itAz0;
       const T cx = ((wz - ww[2][lj - 1][li]) - wy) + wyp;
       const T cy = ((wx - wxp) - wz) + ww[2][lj][li - 1];
       const T cz = ((wy - ww[1][lj][li - 1]) - wx) + ww[0][lj - 1][li];
-      // Each J mode finishes the update in its own branch: the J loads of the
-      // buffer mode never share registers with the other modes, so their
-      // wait does not hold back the loads in flight for the next planes.
-      if (a.jmode == J_BUFFER) {
+      if (JB) {
         const T jx = Jx[n];
         const T jy = (Jx + N)[n];
-        const T jz = beam_col ? jcur : (Jx + 2 * (int64_t)N)[n];
-        Enx[n] = eox + mt[3] * (ax * cx - jx);
-        Eny[n] = eoy + mt[4] * (ay * cy - jy);
-        Enz[n] = eoz + mt[5] * (az * cz - jz);
+        const T jz = beam_col ? jk : (Jx + 2 * (int64_t)N)[n];
+        __builtin_nontemporal_store(eox + mt[3] * (ax * cx - jx), &Enx[n]);
+        __builtin_nontemporal_store(eoy + mt[4] * (ay * cy - jy), &Eny[n]);
+        __builtin_nontemporal_store(eoz + mt[5] * (az * cz - jz), &Enz[n]);
       } else {
         const T js = a.jmode == J_SIGMA_E ? T(1) : T(0);
         const T jx = js * (mt[6] * eox);
         const T jy = js * (mt[7] * eoy);
-        const T jz = beam_col ? jcur : js * (mt[8] * eoz);
-        Enx[n] = eox + mt[3] * (ax * cx - jx);
-        Eny[n] = eoy + mt[4] * (ay * cy - jy);
-        Enz[n] = eoz + mt[5] * (az * cz - jz);
+        const T jz = beam_col ? jk : js * (mt[8] * eoz);
+        __builtin_nontemporal_store(eox + mt[3] * (ax * cx - jx), &Enx[n]);
+        __builtin_nontemporal_store(eoy + mt[4] * (ay * cy - jy), &Eny[n]);
+        __builtin_nontemporal_store(eoz + mt[5] * (az * cz - jz), &Enz[n]);
       }
     }
     wxp = wx;
@@ -495,19 +497,19 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
     if (k + 1 < k1) {
       const int kk = k + 1;
       const int s = kk & 1;
-      const int p0 = kk & (NR - 1), p1 = (kk + 1) & (NR - 1);
+      const int p0 = slot(kk), p1 = slot(kk + 1);
       if (own) {
         const int n1 = n + plane;
         const bool r0 = full_mask(pmc, 0, opmc[0], kk, nz);
         const bool r1 = full_mask(pmc, 1, opmc[1], kk, nz);
         const bool r2 = full_mask(pmc, 2, opmc[2], kk, nz);
-        const T *mt = sm.mt + pM * MAT_STRIDE;
-        const T hx = pHx - (r0 ? T(0) : mt[0] * iAx_k1) * curl_u(0, p0, p1, li, lj);
-        const T hy = pHy - (r1 ? T(0) : mt[1] * piAy) * curl_u(1, p0, p1, li, lj);
-        const T hz = pHz - (r2 ? T(0) : mt[2] * iAz0) * curl_u(2, p0, p1, li, lj);
-        Hnx[n1] = hx;
-        Hny[n1] = hy;
-        Hnz[n1] = hz;
+        const T *mt = smt + cur.m * MAT_STRIDE;
+        const T hx = cur.hx - (r0 ? T(0) : mt[0] * iAx_k1) * curl_u(0, p0, p1, li, lj);
+        const T hy = cur.hy - (r1 ? T(0) : mt[1] * cur.iay) * curl_u(1, p0, p1, li, lj);
+        const T hz = cur.hz - (r2 ? T(0) : mt[2] * iAz0) * curl_u(2, p0, p1, li, lj);
+        __builtin_nontemporal_store(hx, &Hnx[n1]);
+        __builtin_nontemporal_store(hy, &Hny[n1]);
+        __builtin_nontemporal_store(hz, &Hnz[n1]);
         wx = r0 ? T(0) : tLx0 * hx;
         wy = r1 ? T(0) : tLy0 * hy;
         wz = r2 ? T(0) : tLz_k1 * hz;
@@ -515,27 +517,406 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
       sm.w[s][0][lj][li] = wx;
       sm.w[s][1][lj][li] = wy;
       sm.w[s][2][lj][li] = wz;
-      if (hrow || hcolm) halo_w(kk, p0, p1, s, php);
+      if (hrow || hcolm) halo_w(kk, p0, p1, s, cur.h);
     }
-    // rotate the packets
-    mcur = pM;
-    eox = pEx;
-    eoy = pEy;
-    eoz = pEz;
-    itAy_k = pitAy;
-    jcur = pJ;
-    pHx = qHx;
-    pHy = qHy;
-    pHz = qHz;
-    pEx = qEx;
-    pEy = qEy;
-    pEz = qEz;
-    piAy = qiAy;
-    pitAy = qitAy;
-    pM = qM;
-    pJ = qJ;
-    php = qhp;
+    mcur = cur.m;
+  };
+  for (int k = k0; k < k1; k += 2) {
+    iter(k, A, B);
+    if (k + 1 < k1) iter(k + 1, B, A);
   }
+}
+
+// ---------------- lean path for chunks away from the z ends ----------------
+//
+// The same tile step as fused_tile for a chunk whose planes (window and halo
+// included) all lie in 1 .. nz-2: no z-face masks and no clamps apply, the
+// x/y masks are folded into per-lane factors fixed over the march (a masked
+// term is a product with 0 instead of a selected 0: the same value for
+// finite fields), and every global access is a buffer instruction: a
+// per-lane byte offset fixed over the march plus a wave-uniform plane
+// offset, so no address costs vector arithmetic.  Needs the fields and the
+// metric tables each to span less than 2^32 bytes.
+template <typename T, int TX, int TY, int KC, bool JB>
+__device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const int tile,
+                                                 FusedSmem<T, TX, TY> &sm, T *__restrict__ smt) {
+  using G = Tile<TX, TY>;
+  static_assert(TX % 64 == 0, "a wave must cover one tile row");
+  constexpr int NT = G::NT, RW = G::RW, RH = G::RH, RC = G::RC, RL = G::RL, NR = G::NRING;
+  constexpr uint32_t SZ = sizeof(T);
+  const StepArgs<T> &a = f.a;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int plane = nx * ny;
+  const int tx = tile % f.ntx;
+  const int tr = tile / f.ntx;
+  const int ty = tr % f.nty;
+  const int x0 = tx * TX, y0 = ty * TY;
+  const int k0 = (tr / f.nty) * KC;
+  const int k1 = min(k0 + KC, nz);
+  const uint32_t PB = (uint32_t)plane * SZ;    // bytes per plane
+  const uint32_t NB = (uint32_t)a.N * SZ;      // bytes per component
+  const int64_t FB = (int64_t)3 * a.N * SZ;    // bytes per field
+  const Geo<T> &g = a.g;
+  const Rsrc rE = make_rsrc(a.E, FB), rH = make_rsrc(a.H, FB);
+  const Rsrc rEn = make_rsrc(f.En, FB), rHn = make_rsrc(f.Hn, FB);
+  const Rsrc rG = make_rsrc(g.Lx, a.geo_bytes);
+  const bool hasmat = a.mat != nullptr;
+  const Rsrc rM = make_rsrc(hasmat ? (const void *)a.mat : (const void *)g.Lx, hasmat ? a.N : 1);
+  auto gofs = [&](const T *t) __attribute__((always_inline)) -> uint32_t {
+    return (uint32_t)((const char *)t - (const char *)g.Lx);
+  };
+  const uint32_t oLz = gofs(g.Lz), otLz = gofs(g.tLz), oiAy = gofs(g.iAy), oitAy = gofs(g.itAy);
+  const uint32_t rowb = (uint32_t)nx * SZ;  // row stride of iA_y, itA_y
+  const CT<T> *gtLz = (const CT<T> *)g.tLz;
+  const CT<T> *giAx = (const CT<T> *)g.iAx;
+  const CT<T> *gitAx = (const CT<T> *)g.itAx;
+  const CT<T> *jline = (const CT<T> *)a.jline;
+  const CT<T> *mtab = (const CT<T> *)a.mtab;
+  const uint32_t pec = a.pec, pmc = a.pmc;
+
+  for (int t = threadIdx.x; t < a.nmat * MAT_STRIDE; t += NT) smt[t] = mtab[t];
+
+  // ---- window slots: LDS index, byte offsets (per component) from plane
+  //      kk-1, mask factors ----
+  uint32_t wld[RL], wob[RL][3], wlz[RL];
+  T fx[RL], fy[RL], fz[RL];
+  const int ri_max = min(TX + 1, nx - x0 + 1), rj_max = min(TY + 1, ny - y0 + 1);
+#pragma unroll
+  for (int q = 0; q < RL; ++q) {
+    const int c = threadIdx.x + q * NT;
+    const int rj = c / RW, ri = c - rj * RW;
+    const bool need = c < RC && ri <= ri_max && rj <= rj_max;
+    const WinCell w = need ? win_cell(x0 - 1 + ri, y0 - 1 + rj, nx, ny) : WinCell{x0, y0, 0, false};
+    wld[q] = c < RC ? (uint32_t)c : 0u;
+    wob[q][0] = (uint32_t)(w.i + nx * w.j + plane * (w.dk + 1)) * SZ;
+    wob[q][1] = wob[q][0] + NB;
+    wob[q][2] = wob[q][0] + 2 * NB;
+    wlz[q] = oLz + (uint32_t)(w.dk + 1) * SZ;
+    fx[q] = (need && !xy_mask(pec, 0, w.i, w.j, nx, ny)) ? g.Lx[w.i] : T(0);
+    fy[q] = (need && !xy_mask(pec, 1, w.i, w.j, nx, ny)) ? g.Ly[w.j] : T(0);
+    fz[q] = (need && !xy_mask(pec, 2, w.i, w.j, nx, ny)) ? T(1) : T(0);
+  }
+
+  // ---- own cell ----
+  const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+  const int i = x0 + lx, j = y0 + ly;
+  const bool own = i < nx && j < ny;
+  const int ic = min(i, nx - 1);
+  const int jc = __builtin_amdgcn_readfirstlane(min(j, ny - 1));  // one row per wave
+  const int li = lx + 1, lj = ly + 1;
+  const int col = ic + nx * jc;
+  // own cell byte offsets: field components; itA_y of row k (from row k);
+  // iA_y of row k+1 (from row k)
+  const uint32_t cb0 = (uint32_t)col * SZ, cb1 = cb0 + NB, cb2 = cb0 + 2 * NB;
+  const uint32_t itb = oitAy + (uint32_t)ic * SZ, iab = oiAy + (uint32_t)ic * SZ + rowb;
+  const uint32_t colm = (uint32_t)(col + plane);  // material of plane k+1 from plane k
+  const T fe0 = xy_mask(pec, 0, ic, jc, nx, ny) ? T(0) : T(1);
+  const T fe1 = xy_mask(pec, 1, ic, jc, nx, ny) ? T(0) : T(1);
+  const T az0 = xy_mask(pec, 2, ic, jc, nx, ny) ? T(0) : g.itAz[col];
+  const bool m0 = xy_mask(pmc, 0, ic, jc, nx, ny), m1 = xy_mask(pmc, 1, ic, jc, nx, ny);
+  const bool m2 = xy_mask(pmc, 2, ic, jc, nx, ny);
+  const T fm0 = m0 ? T(0) : T(1), fm1 = m1 ? T(0) : T(1), fm2 = m2 ? T(0) : T(1);
+  const T tLx0m = m0 ? T(0) : g.tLx[ic], tLy0m = m1 ? T(0) : g.tLy[jc];
+  const T iAz0m = m2 ? T(0) : g.iAz[col];
+  const bool beam_on = a.beam_mode != BEAM_OFF;  // wave-uniform
+  const bool beam_col = beam_on && i == a.ixs && j == a.iys;
+  // wake probe: E_z history column `step` of this lane's window, or none
+  int pw = -1;
+  GT<T> *phist = nullptr;
+  if (a.hist && a.step < a.hist_nt) {
+    const int di = i - a.ixt, dj = j - a.iyt;
+    if (di >= -a.hw && di <= a.hw && dj >= -a.hw && dj <= a.hw) {
+      pw = (di + a.hw) * (2 * a.hw + 1) + (dj + a.hw);
+      phist = (GT<T> *)a.hist + (int64_t)pw * nz * a.hist_nt + a.step;
+    }
+  }
+
+  // ---- low-halo cell: row (x0+t, y0-1) -> H_x, H_z; column (x0-1, y0+t-TX)
+  //      -> H_y, H_z (rows fill waves 0 .. TX/64-1, the column wave TX/64) ----
+  const bool hrow = threadIdx.x < TX;  // wave-uniform
+  const bool hcolm = !hrow && threadIdx.x < TX + TY;
+  const bool hwave = (int)(threadIdx.x >> 6) <= TX / 64;  // wave-uniform
+  int hli = 0, hlj = 0;
+  WinCell hc{0, 0, 0, false};
+  if (hrow) {
+    hli = threadIdx.x + 1;
+    hc = win_cell(x0 + threadIdx.x, y0 - 1, nx, ny);
+    hc.ok = x0 + (int)threadIdx.x < nx;
+  } else if (hcolm) {
+    hlj = threadIdx.x - TX + 1;
+    hc = win_cell(x0 - 1, y0 + ((int)threadIdx.x - TX), nx, ny);
+    hc.ok = y0 + ((int)threadIdx.x - TX) < ny;
+  }
+  const int hc0 = hrow ? 0 : 1;  // first halo component (second is z)
+  const int hci = hc.i, hcj = hc.j, hdk = hc.dk;
+  const bool hin = (hrow || hcolm) && hc.ok;
+  const int hcolo = hci + nx * hcj;
+  const bool hma = xy_mask(pmc, hc0, hci, hcj, nx, ny), hmb = xy_mask(pmc, 2, hci, hcj, nx, ny);
+  // halo offsets from plane kk-1 (kk: the halo's plane)
+  const uint32_t hom = (uint32_t)(hcolo + plane * (hdk + 1));
+  const uint32_t hoa = hom * SZ + (hrow ? 0u : NB);  // H_x (rows) or H_y (column)
+  const uint32_t hob = hom * SZ + 2 * NB;            // H_z
+  const uint32_t has = (uint32_t)__builtin_amdgcn_readfirstlane(hrow ? ny : nx) * SZ;
+  const uint32_t oiAx = gofs(g.iAx);
+  const uint32_t hab = (hrow ? oiAx : oiAy) +
+                       (uint32_t)(hrow ? hcj + ny * (hdk + 1) : hci + nx * (hdk + 1)) * SZ;
+  const uint32_t hlz = otLz + (uint32_t)(hdk + 2) * SZ;  // tL_z of plane kk+dk from row kk-2
+  const T fha = (hin && !hma) ? T(1) : T(0);
+  const T fhb = (hin && !hmb) ? T(1) : T(0);
+  const T tlx_h = g.tLx[hci], tly_h = g.tLy[hcj];
+  const T h_tlam = (hin && !hma) ? (hrow ? tlx_h : tly_h) : T(0);
+  const T h_iAzm = (hin && !hmb) ? g.iAz[hcolo] : T(0);
+
+  auto slot = [](int k) __attribute__((always_inline)) { return (k + 3) % NR; };
+  auto fetch = [&](int kk, T(&v)[RL][4]) __attribute__((always_inline)) {
+    const uint32_t s0 = (uint32_t)(kk - 1) * PB, sz0 = (uint32_t)(kk - 1) * SZ;
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      v[q][0] = bload<T>(rE, wob[q][0], s0);
+      v[q][1] = bload<T>(rE, wob[q][1], s0);
+      v[q][2] = bload<T>(rE, wob[q][2], s0);
+      v[q][3] = bload<T>(rG, wlz[q], sz0);
+    }
+  };
+  auto stage = [&](int kk, const T(&v)[RL][4]) __attribute__((always_inline)) {
+    T *dst = &sm.u[slot(kk)][0][0][0];
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      if (q < RL - 1 || threadIdx.x + q * NT < RC) {
+        dst[wld[q]] = fx[q] * v[q][0];
+        dst[RC + wld[q]] = fy[q] * v[q][1];
+        dst[2 * RC + wld[q]] = fz[q] * (v[q][3] * v[q][2]);
+      }
+    }
+  };
+  // curl of H component d at window (wi, wj) from slots p0 (plane) / p1 (above)
+  auto curl_u = [&](int d, int p0, int p1, int wi, int wj) __attribute__((always_inline)) -> T {
+    const T(*u0)[RH][RW] = sm.u[p0];
+    const T(*u1)[RH][RW] = sm.u[p1];
+    if (d == 0) return ((u0[1][wj][wi] - u1[1][wj][wi]) - u0[2][wj][wi]) + u0[2][wj + 1][wi];
+    if (d == 1) return ((u0[2][wj][wi] - u0[2][wj][wi + 1]) - u0[0][wj][wi]) + u1[0][wj][wi];
+    return ((u0[0][wj][wi] - u0[0][wj + 1][wi]) - u0[1][wj][wi]) + u0[1][wj][wi + 1];
+  };
+  struct HaloPk {
+    T ha, hb, area_a, tlz;
+    int m;
+  };
+  // halo cell of plane kk (plane kk+dk: base kk-1, offset dk+1); the halo
+  // waves load for every lane (lanes without a halo cell read a valid cell)
+  auto halo_load = [&](int kk, HaloPk &p) __attribute__((always_inline)) {
+    // (fields assigned once, after the branch: stores of different fields
+    // in the two arms would be merged through a pointer select, which keeps
+    // the packet in scratch memory)
+    T ha = 0, hb = 0, tlz = 0, area = 0;
+    int m = 0;
+    if (hwave) {
+      const uint32_t s0 = (uint32_t)(kk - 1) * PB;
+      ha = bload<T>(rH, hoa, s0);
+      hb = bload<T>(rH, hob, s0);
+      m = hasmat ? bload_u8(rM, hom, (uint32_t)(kk - 1) * (uint32_t)plane) : 0;
+      tlz = bload<T>(rG, hlz, (uint32_t)(kk - 2) * SZ);
+      area = bload<T>(rG, hab, (uint32_t)(kk - 1) * has);
+    }
+    p.ha = ha;
+    p.hb = hb;
+    p.area_a = area;
+    p.tlz = tlz;
+    p.m = m;
+  };
+  auto halo_w = [&](int s, int p0, int p1, const HaloPk &p) __attribute__((always_inline)) {
+    const T *mt = smt + p.m * MAT_STRIDE;
+    const T ha = p.ha - (mt[hc0] * (fha * p.area_a)) * curl_u(hc0, p0, p1, hli, hlj);
+    const T hb = p.hb - (mt[2] * h_iAzm) * curl_u(2, p0, p1, hli, hlj);
+    sm.w[s][hc0][hlj][hli] = h_tlam * ha;
+    sm.w[s][2][hlj][hli] = (fhb * p.tlz) * hb;
+  };
+  // Packet consumed at iteration k: E update data of plane k, H update data
+  // (own and halo) of plane k+1.  It is loaded at the end of iteration k-2,
+  // after that iteration's use of the same registers, so two packets (and
+  // two window buffers) alternate with the loop body written out twice: the
+  // loads have about two iterations to arrive, and no register of a load in
+  // flight is ever copied.
+  struct Pk {
+    T ex, ey, ez, itay;  // plane k
+    T hx, hy, hz, iay;   // plane k+1
+    int m;               // material of plane k+1
+    HaloPk h;            // halo of plane k+1
+  };
+  auto load_pk = [&](int k, Pk &p) __attribute__((always_inline)) {
+    const uint32_t se = (uint32_t)k * PB, sh = se + PB, sr = (uint32_t)k * rowb;
+    p.ex = bload<T>(rE, cb0, se);
+    p.ey = bload<T>(rE, cb1, se);
+    p.ez = bload<T>(rE, cb2, se);
+    p.itay = bload<T>(rG, itb, sr);
+    p.hx = bload<T>(rH, cb0, sh);
+    p.hy = bload<T>(rH, cb1, sh);
+    p.hz = bload<T>(rH, cb2, sh);
+    p.iay = bload<T>(rG, iab, sr);
+    p.m = hasmat ? bload_u8(rM, colm, (uint32_t)k * (uint32_t)plane) : 0;
+    halo_load(k + 1, p.h);
+  };
+
+  // ---- prologue: planes k0-1, k0, k0+1 staged; w of planes k0-1 and k0 ----
+  T pEa[RL][4], pEb[RL][4];
+  fetch(k0 - 1, pEa);
+  stage(k0 - 1, pEa);
+  fetch(k0, pEa);
+  stage(k0, pEa);
+  fetch(k0 + 1, pEa);
+  stage(k0 + 1, pEa);
+  if (k0 + 2 <= k1) fetch(k0 + 2, pEa);
+  if (k0 + 3 <= k1) fetch(k0 + 3, pEb);
+  T hxm, hym, iAym, hx0, hy0, hz0, iAy0;
+  int mm, mcur;
+  {
+    const uint32_t sm1 = (uint32_t)(k0 - 1) * PB, s0 = sm1 + PB;
+    hxm = bload<T>(rH, cb0, sm1);
+    hym = bload<T>(rH, cb1, sm1);
+    iAym = bload<T>(rG, iab, (uint32_t)(k0 - 2) * rowb);
+    mm = hasmat ? bload_u8(rM, colm, (uint32_t)(k0 - 2) * (uint32_t)plane) : 0;
+    hx0 = bload<T>(rH, cb0, s0);
+    hy0 = bload<T>(rH, cb1, s0);
+    hz0 = bload<T>(rH, cb2, s0);
+    iAy0 = bload<T>(rG, iab, (uint32_t)(k0 - 1) * rowb);
+    mcur = hasmat ? bload_u8(rM, colm, (uint32_t)(k0 - 1) * (uint32_t)plane) : 0;
+  }
+  HaloPk hp0;
+  halo_load(k0, hp0);
+  Pk A, B;
+  load_pk(k0, A);
+  if (k0 + 1 < k1) load_pk(k0 + 1, B);
+  __syncthreads();
+  T wxp, wyp, wx, wy, wz;
+  {
+    const int s = k0 & 1;
+    const int pm = slot(k0 - 1), p0 = slot(k0), p1 = slot(k0 + 1);
+    const T *mtm = smt + mm * MAT_STRIDE;
+    const T hxn = hxm - (mtm[0] * (fm0 * giAx[jc + ny * (k0 - 1)])) * curl_u(0, pm, p0, li, lj);
+    const T hyn = hym - (mtm[1] * (fm1 * iAym)) * curl_u(1, pm, p0, li, lj);
+    wxp = tLx0m * hxn;
+    wyp = tLy0m * hyn;
+    const T *mt = smt + mcur * MAT_STRIDE;
+    const T hx = hx0 - (mt[0] * (fm0 * giAx[jc + ny * k0])) * curl_u(0, p0, p1, li, lj);
+    const T hy = hy0 - (mt[1] * (fm1 * iAy0)) * curl_u(1, p0, p1, li, lj);
+    const T hz = hz0 - (mt[2] * iAz0m) * curl_u(2, p0, p1, li, lj);
+    if (own) {
+      const uint32_t s0 = (uint32_t)k0 * PB;
+      bstore(rHn, cb0, s0, hx);
+      bstore(rHn, cb1, s0, hy);
+      bstore(rHn, cb2, s0, hz);
+    }
+    wx = own ? tLx0m * hx : T(0);
+    wy = own ? tLy0m * hy : T(0);
+    wz = own ? (fm2 * gtLz[k0]) * hz : T(0);
+    if (!own) wxp = wyp = T(0);
+    sm.w[s][0][lj][li] = wx;
+    sm.w[s][1][lj][li] = wy;
+    sm.w[s][2][lj][li] = wz;
+    if (hrow || hcolm) halo_w(s, p0, p1, hp0);
+  }
+  // the first iteration stages plane k0+2 into the slot of plane k0-1
+  __syncthreads();
+
+  // One plane: E_new(k) and H_new(k+1) from `cur`, then the loads of
+  // iteration k+2 into `cur` and pE.
+  auto iter = [&](const int k, Pk &cur, T(&pE)[RL][4]) __attribute__((always_inline)) {
+    // 1. stage plane k+2 (its loads were issued two iterations ago)
+    if (k + 2 <= k1) stage(k + 2, pE);
+    const T itAx_k = gitAx[jc + ny * k];       // scalar
+    const T iAx_k1 = giAx[jc + ny * (k + 1)];  // scalar
+    const T tLz_k1 = gtLz[k + 1];              // scalar
+    const T jk = beam_on ? jline[k] : T(0);    // scalar
+    // 3.
+    __syncthreads();
+    // 4. E_new(k)
+    const uint32_t se = (uint32_t)k * PB;
+    if (own) {
+      const T(*ww)[TY + 1][TX + 1] = sm.w[k & 1];
+      const T *mt = smt + mcur * MAT_STRIDE;
+      const T eox = cur.ex, eoy = cur.ey, eoz = cur.ez;
+      if (pw >= 0) phist[(int64_t)k * a.hist_nt] = eoz;
+      const T ax = fe0 * itAx_k;
+      const T ay = fe1 * cur.itay;
+      const T cx = ((wz - ww[2][lj - 1][li]) - wy) + wyp;
+      const T cy = ((wx - wxp) - wz) + ww[2][lj][li - 1];
+      const T cz = ((wy - ww[1][lj][li - 1]) - wx) + ww[0][lj - 1][li];
+      T jx, jy, jz;
+      if (JB) {
+        const Rsrc rJ = make_rsrc(a.J, FB);
+        jx = bload<T>(rJ, cb0, se);
+        jy = bload<T>(rJ, cb1, se);
+        jz = beam_col ? jk : bload<T>(rJ, cb2, se);
+      } else {
+        const T js = a.jmode == J_SIGMA_E ? T(1) : T(0);
+        jx = js * (mt[6] * eox);
+        jy = js * (mt[7] * eoy);
+        jz = beam_col ? jk : js * (mt[8] * eoz);
+      }
+      bstore(rEn, cb0, se, eox + mt[3] * (ax * cx - jx));
+      bstore(rEn, cb1, se, eoy + mt[4] * (ay * cy - jy));
+      bstore(rEn, cb2, se, eoz + mt[5] * (az0 * cz - jz));
+    }
+    wxp = wx;
+    wyp = wy;
+    // 5. H_new(k+1)
+    if (k + 1 < k1) {
+      const int kk = k + 1;
+      const int s = kk & 1;
+      const int p0 = slot(kk), p1 = slot(kk + 1);
+      if (own) {
+        const uint32_t sh = se + PB;
+        const T *mt = smt + cur.m * MAT_STRIDE;
+        const T hx = cur.hx - (mt[0] * (fm0 * iAx_k1)) * curl_u(0, p0, p1, li, lj);
+        const T hy = cur.hy - (mt[1] * (fm1 * cur.iay)) * curl_u(1, p0, p1, li, lj);
+        const T hz = cur.hz - (mt[2] * iAz0m) * curl_u(2, p0, p1, li, lj);
+        bstore(rHn, cb0, sh, hx);
+        bstore(rHn, cb1, sh, hy);
+        bstore(rHn, cb2, sh, hz);
+        wx = tLx0m * hx;
+        wy = tLy0m * hy;
+        wz = (fm2 * tLz_k1) * hz;
+      }
+      sm.w[s][0][lj][li] = wx;
+      sm.w[s][1][lj][li] = wy;
+      sm.w[s][2][lj][li] = wz;
+      if (hrow || hcolm) halo_w(s, p0, p1, cur.h);
+    }
+    mcur = cur.m;
+    // 6. prefetch window plane k+4 and the packet of iteration k+2
+    if (k + 4 <= k1) fetch(k + 4, pE);
+    if (k + 2 < k1) load_pk(k + 2, cur);
+  };
+  for (int k = k0; k < k1; k += 2) {
+    iter(k, A, pEa);
+    if (k + 1 < k1) iter(k + 1, B, pEb);
+  }
+}
+
+// Whether tile `tile` of a grid takes the lean path: FP32 (in FP64 its two
+// packets and window buffers would halve the waves a CU holds), every plane
+// its chunk touches (window and halo included) lies in 1 .. nz-2, and a
+// field and the metric tables span less than 2^32 bytes (buffer offsets).
+template <typename T, int TX, int TY, int KC>
+__device__ __forceinline__ bool fused_inner(const FusedArgs<T> &f, int tile) {
+  const int k0 = tile / (f.ntx * f.nty) * KC;
+  return sizeof(T) == 4 && f.a.buf32 && k0 >= 3 && k0 + KC + 4 <= f.a.nz;
+}
+
+// One tile with the code path for its J mode and z position.
+template <typename T, int TX, int TY, int KC>
+__device__ __forceinline__ void fused_dispatch(const FusedArgs<T> &f, int tile,
+                                               FusedSmem<T, TX, TY> &sm, T *smt) {
+  const bool in = fused_inner<T, TX, TY, KC>(f, tile);
+  if constexpr (sizeof(T) == 4) {
+    if (in) {
+      if (f.a.jmode == J_BUFFER) fused_tile_inner<T, TX, TY, KC, true>(f, tile, sm, smt);
+      else fused_tile_inner<T, TX, TY, KC, false>(f, tile, sm, smt);
+      return;
+    }
+  }
+  (void)in;
+  if (f.a.jmode == J_BUFFER) fused_tile<T, TX, TY, KC, true>(f, tile, sm, smt);
+  else fused_tile<T, TX, TY, KC, false>(f, tile, sm, smt);
 }
 
 // Tile order of block b among nb blocks: the hardware dispatches block b to
@@ -546,10 +927,14 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return (b < 8 * q8) ? (b % 8) * q8 + b / 8 : b;
 }
 
+extern __shared__ double fit_dyn_lds[];
+
 template <typename T, int TX, int TY, int KC>
-__global__ void __launch_bounds__(TX *TY) fused_sweep(FusedArgs<T> f) {
+__global__ void __launch_bounds__(TX *TY)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) fused_sweep(FusedArgs<T> f) {
   __shared__ FusedSmem<T, TX, TY> sm;
-  fused_tile<T, TX, TY, KC>(f, xcd_tile(blockIdx.x, f.nb), sm);
+  T *smt = (T *)fit_dyn_lds;
+  fused_dispatch<T, TX, TY, KC>(f, xcd_tile(blockIdx.x, f.nb), sm, smt);
 }
 
 // ---------------- batched launch ----------------
@@ -570,9 +955,10 @@ struct BatchMember {
 // m (ascending, tstart[nmem] = ntiles).  Step = *dstep + g of the batch.
 template <typename T, int TX, int TY, int KC>
 __global__ void __launch_bounds__(TX *TY)
-    fused_batch(const BatchMember<T> *__restrict__ mem, const int *__restrict__ tstart, int nmem,
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) fused_batch(const BatchMember<T> *__restrict__ mem, const int *__restrict__ tstart, int nmem,
                 int ntiles, const int64_t *__restrict__ dstep, int g) {
   __shared__ FusedSmem<T, TX, TY> sm;
+  T *smt = (T *)fit_dyn_lds;
   const int64_t s = *dstep + g;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     int lo = 0, hi = nmem - 1;
@@ -585,7 +971,7 @@ __global__ void __launch_bounds__(TX *TY)
     FusedArgs<T> f = m.f[s & 1];
     f.a.step = m.step0 + s;
     if (m.btab) f.a.jline = m.btab + (int64_t)g * f.a.nz;
-    fused_tile<T, TX, TY, KC>(f, xcd_tile(t - tstart[lo], f.nb), sm);
+    fused_dispatch<T, TX, TY, KC>(f, xcd_tile(t - tstart[lo], f.nb), sm, smt);
     __syncthreads();  // the next tile reuses the LDS
   }
 }
@@ -625,7 +1011,7 @@ enum {
   FUSED_CFG_64x8 = 2,
   FUSED_CFG_256x4 = 3,
   FUSED_CFG_64x4 = 4,
-  FUSED_CFG_256x2_K64 = 5,
+  FUSED_CFG_256x4_K64 = 5,
   FUSED_NCFG = 6
 };
 
@@ -635,7 +1021,7 @@ inline void fused_dims(int cfg, int &tx, int &ty, int &kc) {
     case FUSED_CFG_64x8: tx = 64; ty = 8; kc = 32; break;
     case FUSED_CFG_256x4: tx = 256; ty = 4; kc = 32; break;
     case FUSED_CFG_64x4: tx = 64; ty = 4; kc = 32; break;
-    case FUSED_CFG_256x2_K64: tx = 256; ty = 2; kc = 64; break;
+    case FUSED_CFG_256x4_K64: tx = 256; ty = 4; kc = 64; break;
     default: tx = 256; ty = 2; kc = 32; break;
   }
 }
@@ -643,12 +1029,12 @@ inline void fused_dims(int cfg, int &tx, int &ty, int &kc) {
 // Cost model of a tile configuration for a grid: cells the tiles cover
 // (partial tiles padded) times the configuration's time per covered cell,
 // measured on the 256 x 256 x 768 FP32 pillbox (ms per step per 5.03e7
-// cells: 256x4 0.645, 64x8 0.657, 128x4 0.687, 64x4 0.689, 256x2 K64 0.78,
-// 256x2 1.30).  Used when no tuning was done.
+// cells: 256x4 K64 0.603, 256x4 0.611, 64x8 0.637, 128x4 0.640, 64x4 0.752,
+// 256x2 0.995).  Used when no tuning was done.
 inline double fused_cost(int cfg, int nx, int ny, int nz) {
   int tx, ty, kc;
   fused_dims(cfg, tx, ty, kc);
-  static const double w[FUSED_NCFG] = {1.30, 0.687, 0.657, 0.645, 0.689, 0.78};
+  static const double w[FUSED_NCFG] = {0.995, 0.640, 0.637, 0.611, 0.752, 0.603};
   const double cx = (double)((nx + tx - 1) / tx) * tx;
   const double cy = (double)((ny + ty - 1) / ty) * ty;
   const double cz = (double)((nz + kc - 1) / kc) * kc;
@@ -663,9 +1049,10 @@ inline int fused_default(int nx, bool fp64) {
   return FUSED_CFG_64x8;
 }
 
+// (with a full 256-row material table)
 template <typename T, int TX, int TY>
 constexpr bool cfg_fits() {
-  return sizeof(FusedSmem<T, TX, TY>) <= 160 * 1024;
+  return sizeof(FusedSmem<T, TX, TY>) + MAX_MATERIALS * MAT_STRIDE * sizeof(T) <= 160 * 1024;
 }
 
 // Whether a tile configuration fits the LDS of one workgroup for T.
@@ -676,6 +1063,7 @@ inline bool fused_cfg_ok(int cfg) {
     case FUSED_CFG_64x8: return cfg_fits<T, 64, 8>();
     case FUSED_CFG_256x4: return cfg_fits<T, 256, 4>();
     case FUSED_CFG_64x4: return cfg_fits<T, 64, 4>();
+    case FUSED_CFG_256x4_K64: return cfg_fits<T, 256, 4>();
     default: return cfg_fits<T, 256, 2>();
   }
 }
@@ -712,7 +1100,8 @@ inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
   f.nty = (a.ny + TY - 1) / TY;
   f.nkc = (a.nz + KC - 1) / KC;
   f.nb = f.ntx * f.nty * f.nkc;
-  hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC>), dim3(f.nb), dim3(TX * TY), 0, s, f);
+  hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC>), dim3(f.nb), dim3(TX * TY),
+                     fused_mt_bytes<T>(a.nmat), s, f);
   }
 }
 
@@ -723,7 +1112,7 @@ inline void launch_fused(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s, int 
     case FUSED_CFG_64x8: launch_cfg<T, 64, 8, 32>(a, En, Hn, s); break;
     case FUSED_CFG_256x4: launch_cfg<T, 256, 4, 32>(a, En, Hn, s); break;
     case FUSED_CFG_64x4: launch_cfg<T, 64, 4, 32>(a, En, Hn, s); break;
-    case FUSED_CFG_256x2_K64: launch_cfg<T, 256, 2, 64>(a, En, Hn, s); break;
+    case FUSED_CFG_256x4_K64: launch_cfg<T, 256, 4, 64>(a, En, Hn, s); break;
     default: launch_cfg<T, 256, 2, 32>(a, En, Hn, s); break;
   }
 }
