@@ -100,8 +100,8 @@ def bench_batch(args):
     for s, r in zip(solvers, runs):
         s._wake_device_beam(r)
     cells = sum(s.N for s in solvers)
-    st = wa.BatchStepper(solvers, graph=True, group=16)
-    tune = st.tune(reps=3) if not args.no_tune else None
+    st = wa.BatchStepper(solvers, graph=True, group=16, tile=args.tile)
+    tune = st.tune(reps=3) if not args.no_tune and args.tile is None else None
     cfg, nblk, ntiles = st.tile()
     print(f"[bench] batch {len(solvers)} cavities, {cells} cells, setup {setup_s:.1f}s, "
           f"tile {cfg} blocks {nblk} tiles {ntiles} tune {tune}", file=sys.stderr, flush=True)
