@@ -130,6 +130,10 @@ fit::StepArgs<T> make_args(fit_ctx *c) {
                         2 * ((int64_t)c->ny * c->nz + (int64_t)c->nx * c->nz + (int64_t)c->nx * c->ny);
     a.geo_bytes = geo * sz;
     a.buf32 = 3 * c->N * sz < ((int64_t)1 << 32) - 1 && a.geo_bytes < ((int64_t)1 << 32) - 1;
+    // non-temporal field stores when a step moves far more than the 256 MB
+    // last-level cache (measured: +2.6 % on a 2.4 GB step; -9 % on batched
+    // serving of mid-size grids, which keeps ordinary stores)
+    a.nt = 12 * c->N * sz >= ((int64_t)1 << 30);
   }
   const T *g = (const T *)c->geo;
   const int nx = c->nx, ny = c->ny, nz = c->nz;

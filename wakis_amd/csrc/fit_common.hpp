@@ -58,6 +58,7 @@ struct StepArgs {
   int nx, ny, nz;
   int64_t N;
   bool buf32;      // a field (3N values) and the metric tables span < 2^32 bytes
+  bool nt;         // field stores of the step kernels non-temporal
   int64_t geo_bytes;  // bytes of the packed metric tables (g.Lx first)
   uint32_t pec;    // tangential-E column mask faces
   uint32_t pmc;    // tangential-H row mask faces
@@ -183,14 +184,24 @@ template <>
 __device__ __forceinline__ double bload<double>(Rsrc r, uint32_t v, uint32_t s) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)v, (int)s, 0));
 }
-// Stores are non-temporal (cache policy bit 1, nt): a step's output is read
-// only by the next step, after far more data than the caches hold.
-__device__ __forceinline__ void bstore(Rsrc r, uint32_t v, uint32_t s, float x) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)v, (int)s, 2);
+// nt: non-temporal store (cache policy bit 1), for a step's output that is
+// read only by the next step, after far more data than the caches hold.
+// (nt is a compile-time constant at every call site)
+__device__ __forceinline__ void bstore(Rsrc r, uint32_t v, uint32_t s, float x, const bool nt) {
+  const unsigned u = __builtin_bit_cast(unsigned, x);
+  if (nt) __builtin_amdgcn_raw_buffer_store_b32(u, r, (int)v, (int)s, 2);
+  else __builtin_amdgcn_raw_buffer_store_b32(u, r, (int)v, (int)s, 0);
 }
-__device__ __forceinline__ void bstore(Rsrc r, uint32_t v, uint32_t s, double x) {
+__device__ __forceinline__ void bstore(Rsrc r, uint32_t v, uint32_t s, double x, const bool nt) {
   typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, (int)v, (int)s, 2);
+  const u32x2 u = __builtin_bit_cast(u32x2, x);
+  if (nt) __builtin_amdgcn_raw_buffer_store_b64(u, r, (int)v, (int)s, 2);
+  else __builtin_amdgcn_raw_buffer_store_b64(u, r, (int)v, (int)s, 0);
+}
+template <typename P, typename V>
+__device__ __forceinline__ void gstore(P *p, V x, const bool nt) {
+  if (nt) __builtin_nontemporal_store(x, p);
+  else *p = x;
 }
 __device__ __forceinline__ int bload_u8(Rsrc r, uint32_t v, uint32_t s) {
   return (int)__builtin_amdgcn_raw_buffer_load_b8(r, (int)v, (int)s, 0);

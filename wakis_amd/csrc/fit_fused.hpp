@@ -120,7 +120,7 @@ __device__ __forceinline__ WinCell win_cell(int gi, int gj, int nx, int ny) {
 // JB: the J buffer mode (J read from memory) or not (J = 0 or sigma*E); the
 // two are separate code, so the J loads never meet the other mode's values
 // in a register (a join there would make every wave wait for all its loads).
-template <typename T, int TX, int TY, int KC, bool JB>
+template <typename T, int TX, int TY, int KC, bool JB, bool NTS>
 __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile,
                                            FusedSmem<T, TX, TY> &sm, T *__restrict__ smt) {
   using G = Tile<TX, TY>;
@@ -433,9 +433,9 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
       const T hx = hx0o - (r0 ? T(0) : mt[0] * giAx[jc + ny * k0]) * curl_u(0, p0, p1, li, lj);
       const T hy = hy0o - (r1 ? T(0) : mt[1] * iAy0) * curl_u(1, p0, p1, li, lj);
       const T hz = hz0o - (r2 ? T(0) : mt[2] * iAz0) * curl_u(2, p0, p1, li, lj);
-      __builtin_nontemporal_store(hx, &Hnx[n0]);
-      __builtin_nontemporal_store(hy, &Hny[n0]);
-      __builtin_nontemporal_store(hz, &Hnz[n0]);
+      gstore(&Hnx[n0], hx, NTS);
+      gstore(&Hny[n0], hy, NTS);
+      gstore(&Hnz[n0], hz, NTS);
       wx = r0 ? T(0) : tLx0 * hx;
       wy = r1 ? T(0) : tLy0 * hy;
       wz = r2 ? T(0) : gtLz[k0] * hz;
@@ -478,17 +478,17 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
         const T jx = Jx[n];
         const T jy = (Jx + N)[n];
         const T jz = beam_col ? jk : (Jx + 2 * (int64_t)N)[n];
-        __builtin_nontemporal_store(eox + mt[3] * (ax * cx - jx), &Enx[n]);
-        __builtin_nontemporal_store(eoy + mt[4] * (ay * cy - jy), &Eny[n]);
-        __builtin_nontemporal_store(eoz + mt[5] * (az * cz - jz), &Enz[n]);
+        gstore(&Enx[n], eox + mt[3] * (ax * cx - jx), NTS);
+        gstore(&Eny[n], eoy + mt[4] * (ay * cy - jy), NTS);
+        gstore(&Enz[n], eoz + mt[5] * (az * cz - jz), NTS);
       } else {
         const T js = a.jmode == J_SIGMA_E ? T(1) : T(0);
         const T jx = js * (mt[6] * eox);
         const T jy = js * (mt[7] * eoy);
         const T jz = beam_col ? jk : js * (mt[8] * eoz);
-        __builtin_nontemporal_store(eox + mt[3] * (ax * cx - jx), &Enx[n]);
-        __builtin_nontemporal_store(eoy + mt[4] * (ay * cy - jy), &Eny[n]);
-        __builtin_nontemporal_store(eoz + mt[5] * (az * cz - jz), &Enz[n]);
+        gstore(&Enx[n], eox + mt[3] * (ax * cx - jx), NTS);
+        gstore(&Eny[n], eoy + mt[4] * (ay * cy - jy), NTS);
+        gstore(&Enz[n], eoz + mt[5] * (az * cz - jz), NTS);
       }
     }
     wxp = wx;
@@ -507,9 +507,9 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
         const T hx = cur.hx - (r0 ? T(0) : mt[0] * iAx_k1) * curl_u(0, p0, p1, li, lj);
         const T hy = cur.hy - (r1 ? T(0) : mt[1] * cur.iay) * curl_u(1, p0, p1, li, lj);
         const T hz = cur.hz - (r2 ? T(0) : mt[2] * iAz0) * curl_u(2, p0, p1, li, lj);
-        __builtin_nontemporal_store(hx, &Hnx[n1]);
-        __builtin_nontemporal_store(hy, &Hny[n1]);
-        __builtin_nontemporal_store(hz, &Hnz[n1]);
+        gstore(&Hnx[n1], hx, NTS);
+        gstore(&Hny[n1], hy, NTS);
+        gstore(&Hnz[n1], hz, NTS);
         wx = r0 ? T(0) : tLx0 * hx;
         wy = r1 ? T(0) : tLy0 * hy;
         wz = r2 ? T(0) : tLz_k1 * hz;
@@ -537,7 +537,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
 // per-lane byte offset fixed over the march plus a wave-uniform plane
 // offset, so no address costs vector arithmetic.  Needs the fields and the
 // metric tables each to span less than 2^32 bytes.
-template <typename T, int TX, int TY, int KC, bool JB>
+template <typename T, int TX, int TY, int KC, bool JB, bool NTS>
 __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const int tile,
                                                  FusedSmem<T, TX, TY> &sm, T *__restrict__ smt) {
   using G = Tile<TX, TY>;
@@ -801,9 +801,9 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
     const T hz = hz0 - (mt[2] * iAz0m) * curl_u(2, p0, p1, li, lj);
     if (own) {
       const uint32_t s0 = (uint32_t)k0 * PB;
-      bstore(rHn, cb0, s0, hx);
-      bstore(rHn, cb1, s0, hy);
-      bstore(rHn, cb2, s0, hz);
+      bstore(rHn, cb0, s0, hx, NTS);
+      bstore(rHn, cb1, s0, hy, NTS);
+      bstore(rHn, cb2, s0, hz, NTS);
     }
     wx = own ? tLx0m * hx : T(0);
     wy = own ? tLy0m * hy : T(0);
@@ -852,9 +852,9 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
         jy = js * (mt[7] * eoy);
         jz = beam_col ? jk : js * (mt[8] * eoz);
       }
-      bstore(rEn, cb0, se, eox + mt[3] * (ax * cx - jx));
-      bstore(rEn, cb1, se, eoy + mt[4] * (ay * cy - jy));
-      bstore(rEn, cb2, se, eoz + mt[5] * (az0 * cz - jz));
+      bstore(rEn, cb0, se, eox + mt[3] * (ax * cx - jx), NTS);
+      bstore(rEn, cb1, se, eoy + mt[4] * (ay * cy - jy), NTS);
+      bstore(rEn, cb2, se, eoz + mt[5] * (az0 * cz - jz), NTS);
     }
     wxp = wx;
     wyp = wy;
@@ -869,9 +869,9 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
         const T hx = cur.hx - (mt[0] * (fm0 * iAx_k1)) * curl_u(0, p0, p1, li, lj);
         const T hy = cur.hy - (mt[1] * (fm1 * cur.iay)) * curl_u(1, p0, p1, li, lj);
         const T hz = cur.hz - (mt[2] * iAz0m) * curl_u(2, p0, p1, li, lj);
-        bstore(rHn, cb0, sh, hx);
-        bstore(rHn, cb1, sh, hy);
-        bstore(rHn, cb2, sh, hz);
+        bstore(rHn, cb0, sh, hx, NTS);
+        bstore(rHn, cb1, sh, hy, NTS);
+        bstore(rHn, cb2, sh, hz, NTS);
         wx = tLx0m * hx;
         wy = tLy0m * hy;
         wz = (fm2 * tLz_k1) * hz;
@@ -902,21 +902,22 @@ __device__ __forceinline__ bool fused_inner(const FusedArgs<T> &f, int tile) {
   return sizeof(T) == 4 && f.a.buf32 && k0 >= 3 && k0 + KC + 4 <= f.a.nz;
 }
 
-// One tile with the code path for its J mode and z position.
-template <typename T, int TX, int TY, int KC>
+// One tile with the code path for its J mode and z position (NTS:
+// non-temporal field stores).
+template <typename T, int TX, int TY, int KC, bool NTS>
 __device__ __forceinline__ void fused_dispatch(const FusedArgs<T> &f, int tile,
                                                FusedSmem<T, TX, TY> &sm, T *smt) {
   const bool in = fused_inner<T, TX, TY, KC>(f, tile);
   if constexpr (sizeof(T) == 4) {
     if (in) {
-      if (f.a.jmode == J_BUFFER) fused_tile_inner<T, TX, TY, KC, true>(f, tile, sm, smt);
-      else fused_tile_inner<T, TX, TY, KC, false>(f, tile, sm, smt);
+      if (f.a.jmode == J_BUFFER) fused_tile_inner<T, TX, TY, KC, true, NTS>(f, tile, sm, smt);
+      else fused_tile_inner<T, TX, TY, KC, false, NTS>(f, tile, sm, smt);
       return;
     }
   }
   (void)in;
-  if (f.a.jmode == J_BUFFER) fused_tile<T, TX, TY, KC, true>(f, tile, sm, smt);
-  else fused_tile<T, TX, TY, KC, false>(f, tile, sm, smt);
+  if (f.a.jmode == J_BUFFER) fused_tile<T, TX, TY, KC, true, NTS>(f, tile, sm, smt);
+  else fused_tile<T, TX, TY, KC, false, NTS>(f, tile, sm, smt);
 }
 
 // Tile order of block b among nb blocks: the hardware dispatches block b to
@@ -929,12 +930,12 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 
 extern __shared__ double fit_dyn_lds[];
 
-template <typename T, int TX, int TY, int KC>
+template <typename T, int TX, int TY, int KC, bool NTS>
 __global__ void __launch_bounds__(TX *TY)
     __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) fused_sweep(FusedArgs<T> f) {
   __shared__ FusedSmem<T, TX, TY> sm;
   T *smt = (T *)fit_dyn_lds;
-  fused_dispatch<T, TX, TY, KC>(f, xcd_tile(blockIdx.x, f.nb), sm, smt);
+  fused_dispatch<T, TX, TY, KC, NTS>(f, xcd_tile(blockIdx.x, f.nb), sm, smt);
 }
 
 // ---------------- batched launch ----------------
@@ -971,7 +972,7 @@ __global__ void __launch_bounds__(TX *TY)
     FusedArgs<T> f = m.f[s & 1];
     f.a.step = m.step0 + s;
     if (m.btab) f.a.jline = m.btab + (int64_t)g * f.a.nz;
-    fused_dispatch<T, TX, TY, KC>(f, xcd_tile(t - tstart[lo], f.nb), sm, smt);
+    fused_dispatch<T, TX, TY, KC, false>(f, xcd_tile(t - tstart[lo], f.nb), sm, smt);
     __syncthreads();  // the next tile reuses the LDS
   }
 }
@@ -1069,13 +1070,15 @@ inline bool fused_cfg_ok(int cfg) {
 }
 
 // The configuration of least modelled cost over n grids (one grid, or the
-// members of a batch) among those that fit the LDS for T.
+// members of a batch) among those that fit the LDS for T (and, when given,
+// that allowed[cfg] admits).
 template <typename T>
-inline int fused_pick(const int *nx, const int *ny, const int *nz, int n) {
+inline int fused_pick(const int *nx, const int *ny, const int *nz, int n,
+                      const bool *allowed = nullptr) {
   int best = FUSED_CFG_128x4;
   double bc = 1e300;
   for (int cfg = 0; cfg < FUSED_NCFG; ++cfg) {
-    if (!fused_cfg_ok<T>(cfg)) continue;
+    if (!fused_cfg_ok<T>(cfg) || (allowed && !allowed[cfg])) continue;
     double c = 0;
     for (int i = 0; i < n; ++i) c += fused_cost(cfg, nx[i], ny[i], nz[i]);
     if (c < bc) {
@@ -1100,8 +1103,12 @@ inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
   f.nty = (a.ny + TY - 1) / TY;
   f.nkc = (a.nz + KC - 1) / KC;
   f.nb = f.ntx * f.nty * f.nkc;
-  hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC>), dim3(f.nb), dim3(TX * TY),
-                     fused_mt_bytes<T>(a.nmat), s, f);
+  if (a.nt)
+    hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC, true>), dim3(f.nb), dim3(TX * TY),
+                       fused_mt_bytes<T>(a.nmat), s, f);
+  else
+    hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC, false>), dim3(f.nb), dim3(TX * TY),
+                       fused_mt_bytes<T>(a.nmat), s, f);
   }
 }
 
