@@ -153,6 +153,26 @@ int fit_wake_probe(fit_ctx *ctx, int32_t ixt, int32_t iyt, int32_t h,
 /* Copy the probe history (window-major, then z, then time; ctx dtype). */
 int fit_wake_history(fit_ctx *ctx, void *host, int64_t count);
 
+/* Frequency-domain field monitor (wakis/field_monitors.py:14-131: the
+ * FieldMonitor that wakesolve updates after every step, wakis/routines.py:
+ * 450-451).  The context holds complex128 accumulators acc[f][c][n] (nf
+ * frequencies x 3 components x N cells, flat Field order, re/im
+ * interleaved); one update adds E_c * phase_f to each, with the real field
+ * promoted to (E, 0) and numpy's complex product rounding.
+ *   fit_monitor_set:        allocate nf frequencies' accumulators, zeroed
+ *                           (nf = 0 frees them).
+ *   fit_monitor_accumulate: one update with the current E; phase = nf
+ *                           complex values (re, im interleaved).
+ *   fit_monitor_phases:     stage nsteps rows of nf phases: the next
+ *                           fit_step call updates after each of its first
+ *                           nsteps steps with row s (the stage is consumed by
+ *                           that call).
+ *   fit_monitor_read:       copy acc[f][comp] (2N doubles) to the host. */
+int fit_monitor_set(fit_ctx *ctx, int32_t nf);
+int fit_monitor_accumulate(fit_ctx *ctx, const double *phase);
+int fit_monitor_phases(fit_ctx *ctx, const double *phases, int64_t nsteps);
+int fit_monitor_read(fit_ctx *ctx, int32_t f, int32_t comp, double *host);
+
 /* Advance nsteps leapfrog steps (wakis/solverFIT3D.py:504-525):
  *   H -= dt*M_H E;  E += dt*(M_E H - ieps*J);  J = sigma*E if conductive.
  * conductive != 0 selects J := sigma*E after each step.  Asynchronous. */

@@ -30,6 +30,10 @@ What is restated (citations are reference files, path:line):
                         file replaced by an in-memory dict using the same
                         ``'#%05d'`` keys and ``x,y,z,dx,dy,dz,t`` entries.
 * ``OBeam``             wakis/sources.py:22-110.
+* ``OFieldMonitor``      wakis/field_monitors.py:14-131 (pinned by the
+                        reference's own closed-form checks of
+                        tests/test_012_field_monitors.py: zero frequency
+                        sums linearly, f*dt = 1/2 sums to zero).
 * ``OPlaneWave``        wakis/sources.py:142-248.
 * ``OWake``             wakis/wakeSolver.py:133-209 (fmax, ti), 311-421
                         ``calc_long_WP``, 423-551 ``calc_long_WP_3d`` (the
@@ -688,6 +692,42 @@ class OSolver:
         if solve:
             self.wake.solve(compute_plane=compute_plane)
         return hf
+
+
+# --------------------------------------------------------------------------
+# Field monitor  (wakis/field_monitors.py)
+# --------------------------------------------------------------------------
+class OFieldMonitor:
+    """wakis/field_monitors.py:14-131: complex128 accumulators (nf, Nx, Ny,
+    Nz) per E component; update adds E_c * exp(-2j*pi*f*t), t = time_index *
+    dt with dt fixed at the first update (96-104)."""
+
+    def __init__(self, frequencies):
+        self.frequencies = np.array(frequencies)
+        self.time_index = 0
+        self.dt = None
+        self.Ex_acc = self.Ey_acc = self.Ez_acc = None
+        self.shape = None
+
+    def update(self, E, dt):
+        if self.dt is None:
+            self.dt = dt
+            self.shape = (E.Nx, E.Ny, E.Nz)
+            shape = (len(self.frequencies), *self.shape)
+            self.Ex_acc = np.zeros(shape, dtype=np.complex128)
+            self.Ey_acc = np.zeros(shape, dtype=np.complex128)
+            self.Ez_acc = np.zeros(shape, dtype=np.complex128)
+        t = self.time_index * self.dt
+        Ex, Ey, Ez = E.to_matrix("x"), E.to_matrix("y"), E.to_matrix("z")
+        for i, f in enumerate(self.frequencies):
+            phase = np.exp(-2j * np.pi * f * t)
+            self.Ex_acc[i] += Ex * phase
+            self.Ey_acc[i] += Ey * phase
+            self.Ez_acc[i] += Ez * phase
+        self.time_index += 1
+
+    def get_components(self):
+        return {"Ex": self.Ex_acc, "Ey": self.Ey_acc, "Ez": self.Ez_acc}
 
 
 # --------------------------------------------------------------------------

@@ -3,7 +3,7 @@ checker) and as a wakis_amd solver (HIP), with identical initial fields."""
 
 import numpy as np
 
-from oracle.fit_oracle import OBeam, OGrid, OPlaneWave, OSolver, OWake  # noqa: F401
+from oracle.fit_oracle import OBeam, OFieldMonitor, OGrid, OPlaneWave, OSolver, OWake  # noqa: F401
 
 
 def axes(Nx, Ny, Nz, nonuniform=False, seed=0, ext=(0.1, 0.08, 0.2)):
@@ -53,8 +53,12 @@ def make_pair(Nx, Ny, Nz, bc_low, bc_high, dtype=np.float64, bg=(1.0, 1.0), soli
 
 
 def rel(a, b):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    """Relative L2 distance ||a - b|| / ||b|| (complex arrays compared as
+    complex, real ones in float64)."""
+    a, b = np.asarray(a), np.asarray(b)
+    dt = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a = a.astype(dt)
+    b = b.astype(dt)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 

@@ -8,8 +8,9 @@ wake potential and impedance DFT run in hand-written HIP kernels
 (``wakis_amd/csrc``) behind the C ABI of ``include/wakis_fit.h``.
 """
 
-from . import field, materials, sources  # noqa: F401
+from . import field, field_monitors, materials, sources  # noqa: F401
 from .field import Field  # noqa: F401
+from .field_monitors import FieldMonitor  # noqa: F401
 from .grid import GridFIT3D  # noqa: F401
 from .solver import SolverFIT3D  # noqa: F401
 from .sources import Beam, PlaneWave  # noqa: F401

@@ -66,6 +66,10 @@ SIGNATURES = {
     "fit_apply_abc": [_P, _U32],
     "fit_wake_probe": [_P, _I32, _I32, _I32, _I64],
     "fit_wake_history": [_P, _P, _I64],
+    "fit_monitor_set": [_P, _I32],
+    "fit_monitor_accumulate": [_P, _PD],
+    "fit_monitor_phases": [_P, _PD, _I64],
+    "fit_monitor_read": [_P, _I32, _I32, _PD],
     "fit_step": [_P, _I32, _I32],
     "fit_step_timed": [_P, _I32, _I32, _I32, _PD],
     "fit_copy_bench": [_I32, _I64, _I32, _PD],
@@ -311,6 +315,27 @@ class Context:
     def wake_probe(self, ixt: int, iyt: int, h: int, nt_cap: int):
         check(self.lib.fit_wake_probe(self.h, ixt, iyt, h, int(nt_cap)), "fit_wake_probe")
         self.probe = (ixt, iyt, h, int(nt_cap))
+
+    # -- field monitor ---------------------------------------------------------
+    def monitor_set(self, nf: int):
+        check(self.lib.fit_monitor_set(self.h, int(nf)), "fit_monitor_set")
+
+    def monitor_accumulate(self, phase: np.ndarray):
+        p = np.ascontiguousarray(np.asarray(phase, dtype=np.complex128)).view(np.float64)
+        check(self.lib.fit_monitor_accumulate(self.h, dptr(p)), "fit_monitor_accumulate")
+
+    def monitor_phases(self, phases: np.ndarray):
+        """phases: (nsteps, nf) complex128, one row per step of the next step() call."""
+        p = np.ascontiguousarray(np.asarray(phases, dtype=np.complex128))
+        n = p.shape[0]
+        check(self.lib.fit_monitor_phases(self.h, dptr(p.view(np.float64).reshape(-1)), n),
+              "fit_monitor_phases")
+
+    def monitor_read(self, f: int, comp: int) -> np.ndarray:
+        """Accumulator of frequency f, component comp as an (nx, ny, nz) complex array."""
+        out = np.empty(2 * self.N, dtype=np.float64)
+        check(self.lib.fit_monitor_read(self.h, int(f), int(comp), dptr(out)), "fit_monitor_read")
+        return out.view(np.complex128).reshape((self.nx, self.ny, self.nz), order="F")
 
     def wake_history(self) -> np.ndarray:
         ixt, iyt, h, nt = self.probe

@@ -137,6 +137,32 @@ __global__ void materialize_j(StepArgs<T> a) {
   }
 }
 
+// FieldMonitor update (wakis/field_monitors.py:96-102): acc[f][c][n] +=
+// E_c[n] * phase_f in complex128, E promoted to (E, 0) and the product
+// rounded as numpy's complex multiply, (e*pr - 0*pi, e*pi + 0*pr), then
+// the two adds (no contraction into an FMA), so the sums match numpy's.
+template <typename T>
+__global__ void monitor_accumulate(const T *__restrict__ E, int64_t N, int nf,
+                                   const double *__restrict__ phase,
+                                   double2 *__restrict__ acc) {
+#pragma clang fp contract(off)
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  for (int c = 0; c < 3; ++c) {
+    const double e = (double)E[c * N + n];
+    for (int f = 0; f < nf; ++f) {
+      const double pr = phase[2 * f], pi = phase[2 * f + 1];
+      const double re = e * pr - 0.0 * pi;
+      const double im = e * pi + 0.0 * pr;
+      double2 *p = acc + ((int64_t)f * 3 + c) * N + n;
+      double2 v = *p;
+      v.x = v.x + re;
+      v.y = v.y + im;
+      *p = v;
+    }
+  }
+}
+
 // Longitudinal wake potential (wakis/wakeSolver.py:492-534): thread n owns
 // sample n and sums over k in order, with separate multiply and add roundings
 // (no contraction into an FMA) exactly as the reference's WP[n] = WP[n] +

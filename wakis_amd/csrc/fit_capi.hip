@@ -69,6 +69,13 @@ struct fit_ctx {
   int fused_cfg = -1;  // tile configuration of the fused sweep (-1: default)
   void *btab = nullptr;  // beam line table of the steps of one fit_step call
   int64_t btab_steps = 0;
+  // field monitor: complex128 accumulators (3 * mon_nf * N) and the staged
+  // phases (mon_ph_steps rows of mon_nf complex values) of the next fit_step
+  void *mon = nullptr;
+  int mon_nf = 0;
+  double *mon_ph = nullptr;
+  int64_t mon_ph_cap = 0;  // rows
+  int64_t mon_ph_steps = 0;
 };
 
 namespace {
@@ -205,6 +212,14 @@ int want_fused(fit_ctx *c, bool *fused) {
   return 0;
 }
 
+// One field-monitor update of the current E with nf phases at `phase`
+// (device memory).
+template <typename T>
+void monitor_launch(fit_ctx *c, const double *phase) {
+  hipLaunchKernelGGL(fit::monitor_accumulate<T>, dim3(nblocks(c->N, 256)), dim3(256), 0,
+                     c->stream, (const T *)c->E, c->N, c->mon_nf, phase, (double2 *)c->mon);
+}
+
 // ev (optional): 3 events per step [before first kernel, between, after].
 template <typename T>
 int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) {
@@ -248,6 +263,7 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
       hipLaunchKernelGGL(fit::e_update_split<T>, grid, block, 0, c->stream, a);
     }
     if (ev) HIPCHK(hipEventRecord(ev[3 * s + 2], c->stream));
+    if (c->mon && s < c->mon_ph_steps) monitor_launch<T>(c, c->mon_ph + (int64_t)s * 2 * c->mon_nf);
     HIPCHK(hipGetLastError());
     c->step += 1;
     if (conductive) {
@@ -255,6 +271,7 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
       if (c->beam_mode == FIT_BEAM_VALUES) c->beam_mode = FIT_BEAM_OFF;
     }
   }
+  c->mon_ph_steps = 0;  // the staged monitor phases belong to this call
   return 0;
 }
 
@@ -376,8 +393,9 @@ int fit_destroy(fit_ctx *c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void *ps[] = {c->E,    c->H,   c->J,     c->E_alt, c->H_alt, c->mat,   c->mtab,
-                c->geo,  c->jline, c->zc, c->hist,  c->staging, c->btab};
+  void *ps[] = {c->E,   c->H,     c->J,  c->E_alt, c->H_alt,   c->mat,  c->mtab,
+                c->geo, c->jline, c->zc, c->hist,  c->staging, c->btab, c->mon,
+                c->mon_ph};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -647,6 +665,83 @@ int fit_wake_history(fit_ctx *c, void *host, int64_t count) {
   if (count != (int64_t)nw * c->nz * c->hist_nt) return fail(FIT_EINVAL, "history count mismatch");
   if (int rc = set_device(c)) return rc;
   HIPCHK(hipMemcpyAsync(host, c->hist, (size_t)count * c->esz, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fit_monitor_set(fit_ctx *c, int32_t nf) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (nf < 0) return fail(FIT_EINVAL, "nf < 0");
+  if (int rc = set_device(c)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->mon) HIPCHK(hipFree(c->mon));
+  c->mon = nullptr;
+  c->mon_nf = 0;
+  c->mon_ph_steps = 0;
+  if (nf == 0) return 0;
+  const size_t bytes = (size_t)3 * nf * c->N * 16;
+  if (hipMalloc(&c->mon, bytes) != hipSuccess) {
+    c->mon = nullptr;
+    (void)hipGetLastError();
+    return fail(FIT_ENOMEM, "field monitor accumulators do not fit");
+  }
+  HIPCHK(hipMemsetAsync(c->mon, 0, bytes, c->stream));
+  c->mon_nf = nf;
+  return 0;
+}
+
+namespace {
+int monitor_stage(fit_ctx *c, const double *rows, int64_t nrows) {
+  if (c->mon_ph_cap < nrows) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->mon_ph) HIPCHK(hipFree(c->mon_ph));
+    c->mon_ph = nullptr;
+    c->mon_ph_cap = 0;
+    if (hipMalloc((void **)&c->mon_ph, (size_t)nrows * 2 * c->mon_nf * 8) != hipSuccess) {
+      c->mon_ph = nullptr;
+      (void)hipGetLastError();
+      return fail(FIT_ENOMEM, "field monitor phase table allocation failed");
+    }
+    c->mon_ph_cap = nrows;
+  }
+  // the previous use of the table (launches of an earlier call) must be done
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->mon_ph, rows, (size_t)nrows * 2 * c->mon_nf * 8, hipMemcpyHostToDevice));
+  return 0;
+}
+}  // namespace
+
+int fit_monitor_accumulate(fit_ctx *c, const double *phase) {
+  if (!c || !phase) return fail(FIT_EINVAL, "null argument");
+  if (!c->mon) return fail(FIT_ESTATE, "no field monitor");
+  if (int rc = set_device(c)) return rc;
+  if (int rc = monitor_stage(c, phase, 1)) return rc;
+  c->mon_ph_steps = 0;
+  if (c->dtype == FIT_F64) monitor_launch<double>(c, c->mon_ph);
+  else monitor_launch<float>(c, c->mon_ph);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int fit_monitor_phases(fit_ctx *c, const double *phases, int64_t nsteps) {
+  if (!c || (!phases && nsteps > 0)) return fail(FIT_EINVAL, "null argument");
+  if (!c->mon) return fail(FIT_ESTATE, "no field monitor");
+  if (nsteps < 0) return fail(FIT_EINVAL, "nsteps < 0");
+  if (int rc = set_device(c)) return rc;
+  if (nsteps > 0)
+    if (int rc = monitor_stage(c, phases, nsteps)) return rc;
+  c->mon_ph_steps = nsteps;
+  return 0;
+}
+
+int fit_monitor_read(fit_ctx *c, int32_t f, int32_t comp, double *host) {
+  if (!c || !host) return fail(FIT_EINVAL, "null argument");
+  if (!c->mon) return fail(FIT_ESTATE, "no field monitor");
+  if (f < 0 || f >= c->mon_nf || comp < 0 || comp > 2) return fail(FIT_EINVAL, "bad monitor index");
+  if (int rc = set_device(c)) return rc;
+  const size_t off = ((size_t)f * 3 + comp) * c->N * 16;
+  HIPCHK(hipMemcpyAsync(host, (char *)c->mon + off, (size_t)c->N * 16, hipMemcpyDeviceToHost,
+                        c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }

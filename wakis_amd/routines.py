@@ -67,9 +67,15 @@ class RoutinesMixin:
         """wakis/routines.py:231-485 on the device (see module docstring)."""
         if plot:
             raise NotImplementedError("plotting is not part of this build")
+        from .field_monitors import FieldMonitor
+
         run = self._wake_setup(wakelength, wake, compute_plane, add_space)
         t0 = time.time()
-        per_step = callback is not None or (use_field_monitor and field_monitor is not None)
+        monitor = use_field_monitor and field_monitor is not None
+        # this package's FieldMonitor accumulates on the device inside the
+        # launch sequence; any other monitor object is updated from the host
+        dev_monitor = monitor and callback is None and isinstance(field_monitor, FieldMonitor)
+        per_step = callback is not None or (monitor and not dev_monitor)
         if per_step:
             beam = run["beam"]
             for n in range(run["Nt"]):
@@ -84,6 +90,8 @@ class RoutinesMixin:
             done = 0
             while done < run["Nt"]:
                 n = min(chunk, run["Nt"] - done)
+                if dev_monitor:
+                    field_monitor._stage(self.E, self.dt, n)
                 self._advance(n)
                 done += n
             self.ctx.sync()
