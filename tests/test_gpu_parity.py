@@ -326,3 +326,25 @@ def test_stratified_box_tensor_block_beam(dtype):
     err = compare_fields(osv, psv)
     print(err)
     assert max(err.values()) <= (TOL64 if dtype == np.float64 else TOL32), err
+
+
+def test_save_reset_load_state(tmp_path):
+    """save_state / reset_fields / load_state (wakis/solverFIT3D.py:908-1035)
+    round trip: the restored run continues exactly like an uninterrupted one."""
+    shape = (8, 7, 10)
+    print("[case] save/reset/load state", flush=True)
+    _, a = make_pair(*shape, ["pec"] * 3, ["pec", "pec", "pml"], seed=17)
+    _, b = make_pair(*shape, ["pec"] * 3, ["pec", "pec", "pml"], seed=17)
+    for _ in range(5):
+        a.one_step()
+        b.one_step()
+    fn = str(tmp_path / "state.h5")
+    b.save_state(fn)
+    b.reset_fields()
+    assert not np.any(b.E.toarray()) and not np.any(b.H.toarray()) and not np.any(b.J.toarray())
+    b.load_state(fn)
+    for _ in range(5):
+        a.one_step()
+        b.one_step()
+    for n in "EHJ":
+        assert np.array_equal(getattr(a, n).toarray(), getattr(b, n).toarray()), n

@@ -216,3 +216,32 @@ def test_stratified_box_host_setup_is_lean():
     assert all(isinstance(r, tuple) for r, _ in wl["grid"]["solids"])
     assert len(wl["grid"]["solids"]) == 17
     assert wl["grid"]["x"].size == 1025
+
+
+def test_result_files_roundtrip(tmp_path):
+    """save_txt / read_txt / load_results with the reference's file layout
+    (wakis/wakeSolver.py:1366-1511): header keys "name[unit]", complex
+    columns read back as complex."""
+    import wakis_amd as wa
+
+    w = wa.WakeSolver(save=True, results_folder=str(tmp_path) + "/")
+    rng = np.random.default_rng(3)
+    s = np.linspace(-0.1, 1.0, 50)
+    f = np.linspace(0, 5e9, 30)
+    cols = {"WP": rng.standard_normal(50), "WPx": rng.standard_normal(50),
+            "WPy": rng.standard_normal(50), "lambda": rng.standard_normal(50),
+            "Z": rng.standard_normal(30) + 1j * rng.standard_normal(30),
+            "Zx": rng.standard_normal(30) + 1j * rng.standard_normal(30),
+            "Zy": rng.standard_normal(30) + 1j * rng.standard_normal(30),
+            "spectrum": rng.standard_normal(30) + 1j * rng.standard_normal(30)}
+    for name, v in cols.items():
+        x = s if v.size == 50 else f
+        w.save_txt(str(tmp_path / name), x, v, "s [m]" if v.size == 50 else "f [Hz]", name + " [-]")
+    d = w.read_txt(str(tmp_path / "WP.txt"))
+    assert list(d.keys()) == ["s[m]", "WP[-]"]
+    w2 = wa.WakeSolver(save=False)
+    w2.load_results(str(tmp_path))
+    assert np.array_equal(w2.s, s) and np.array_equal(w2.WP, cols["WP"])
+    assert np.array_equal(w2.Z, cols["Z"]) and np.array_equal(w2.f, f)
+    assert np.array_equal(w2.lambdaf, cols["spectrum"]) and np.array_equal(w2.WPy, cols["WPy"])
+    assert w2.wakelength == s[-1] and w2.Ez_file.endswith("Ez.h5")

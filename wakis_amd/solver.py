@@ -371,6 +371,59 @@ class SolverFIT3D(RoutinesMixin):
     sigma = property(lambda s: s._material_field("sigma"),
                      lambda s, v: s._set_material_field("sigma", v))
 
+    # -- state files (wakis/solverFIT3D.py:908-1035) ---------------------------
+    def save_state(self, filename="solver_state.h5", close=True):
+        """Write H, E, J (flat 3N arrays) to `filename`: an HDF5 file with
+        datasets "H", "E", "J" as in the reference when h5py is importable,
+        otherwise a numpy .npz archive with the same keys under the same file
+        name.  close=False returns the open h5py file (or the dict of arrays)."""
+        state = {"H": self.H.toarray(), "E": self.E.toarray(), "J": self.J.toarray()}
+        try:
+            import h5py
+        except ImportError:
+            h5py = None
+        if h5py is not None:
+            f = h5py.File(filename, "w")
+            for k, v in state.items():
+                f.create_dataset(k, data=v)
+            if close:
+                f.close()
+                return None
+            return f
+        with open(filename, "wb") as fh:
+            np.savez(fh, **state)
+        return None if close else state
+
+    def read_state(self, filename="solver_state.h5"):
+        """Open a state file read-only (wakis/solverFIT3D.py:1005-1023): an
+        h5py.File, or the npz archive when h5py is not importable."""
+        try:
+            import h5py
+            return h5py.File(filename, "r")
+        except ImportError:
+            return np.load(filename, allow_pickle=False)
+        except OSError:
+            return np.load(filename, allow_pickle=False)
+
+    def load_state(self, filename="solver_state.h5"):
+        """Restore H, E, J from a state file (wakis/solverFIT3D.py:959-1003)."""
+        st = self.read_state(filename)
+        try:
+            E, H, J = (np.asarray(st[k][:]) for k in ("E", "H", "J"))
+        finally:
+            if hasattr(st, "close"):
+                st.close()
+        self.E.fromarray(E)
+        self.H.fromarray(H)
+        self.J.fromarray(J)
+
+    def reset_fields(self):
+        """E, H, J := 0 (wakis/solverFIT3D.py:1025-1035)."""
+        for d in ["x", "y", "z"]:
+            self.E[:, :, :, d] = 0.0
+            self.H[:, :, :, d] = 0.0
+            self.J[:, :, :, d] = 0.0
+
     def update_tensors(self, tensor="all"):
         """Re-derive the material ids from the (possibly edited) ieps, imu,
         sigma Fields and upload them (wakis/solverFIT3D.py:378-436); like the

@@ -267,3 +267,57 @@ class WakeSolver:
         if self.save:
             np.savetxt(self.folder + name, np.c_[a, b],
                        header="   " + ha + " " * 20 + hb + "\n" + "-" * 48)
+
+    # -- result files (wakis/wakeSolver.py:1366-1511) --------------------------
+    def save_txt(self, f_name, x_data=None, y_data=None, x_name="X [-]", y_name="Y [-]"):
+        """Two-column text file with the reference's header
+        (wakis/wakeSolver.py:1422-1475); nothing is written without data."""
+        if f_name.endswith(".txt"):
+            f_name = f_name[:-4]
+        if x_data is not None and y_data is not None:
+            np.savetxt(f_name + ".txt", np.c_[x_data, y_data],
+                       header="   " + x_name + " " * 20 + y_name + "\n" + "-" * 48)
+
+    def read_txt(self, txt, skiprows=2, delimiter=None, usecols=None):
+        """Columns of a text file keyed by the header names "name[unit]"
+        (integer keys when the header does not parse), complex when a float
+        read fails (wakis/wakeSolver.py:1366-1420)."""
+        try:
+            load = np.loadtxt(txt, skiprows=skiprows, delimiter=delimiter, usecols=usecols)
+        except Exception:
+            load = np.loadtxt(txt, skiprows=skiprows, delimiter=delimiter, usecols=usecols,
+                              dtype=np.complex128)
+        try:
+            with open(txt) as f:
+                header = f.readline()
+            header = header.replace(" ", "").replace("#", "").replace("\n", "").split("]")
+            d = {}
+            for i in range(len(load[0, :])):
+                d[header[i] + "]"] = load[:, i]
+        except Exception:
+            d = {}
+            for i in range(len(load[0, :])):
+                d[i] = load[:, i]
+        return d
+
+    def load_results(self, folder=None):
+        """Read the result files of a saved run back into the attributes
+        (wakis/wakeSolver.py:1477-1511)."""
+        if folder is None and self.folder is None:
+            raise Exception("[!] Please provide a folder path")
+        elif folder is None:
+            folder = self.folder
+        if not folder.endswith("/"):
+            folder = folder + "/"
+        _, self.lambdas = self.read_txt(folder + "lambda.txt").values()
+        _, self.WPx = self.read_txt(folder + "WPx.txt").values()
+        _, self.WPy = self.read_txt(folder + "WPy.txt").values()
+        self.s, self.WP = self.read_txt(folder + "WP.txt").values()
+        _, self.lambdaf = self.read_txt(folder + "spectrum.txt").values()
+        _, self.Zx = self.read_txt(folder + "Zx.txt").values()
+        _, self.Zy = self.read_txt(folder + "Zy.txt").values()
+        self.f, self.Z = self.read_txt(folder + "Z.txt").values()
+        self.f = np.abs(self.f)
+        self.wakelength = self.s[-1]
+        self.folder = folder
+        self.Ez_file = folder + "Ez.h5"
