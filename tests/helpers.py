@@ -20,6 +20,18 @@ def axes(Nx, Ny, Nz, nonuniform=False, seed=0, ext=(0.1, 0.08, 0.2)):
     return tuple(out)
 
 
+def write_ascii_stl(path, tri):
+    """Triangles (n, 3, 3) as an ASCII STL file."""
+    with open(path, "w") as f:
+        f.write("solid t\n")
+        for a, b, c in np.asarray(tri, dtype=np.float64):
+            f.write(" facet normal 0 0 0\n  outer loop\n")
+            for v in (a, b, c):
+                f.write("   vertex %r %r %r\n" % (float(v[0]), float(v[1]), float(v[2])))
+            f.write("  endloop\n endfacet\n")
+        f.write("endsolid t\n")
+
+
 def block_mask(shape, lo, hi):
     m = np.zeros(shape, bool)
     m[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = True

@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import OGrid, OSolver, OWake, block_mask, make_pair, rel
+from helpers import OGrid, OSolver, OWake, block_mask, make_pair, rel, write_ascii_stl
 from oracle.fit_oracle import _wp_accumulate
 
 pytestmark = pytest.mark.gpu
@@ -158,3 +158,63 @@ def _cubic_cavity_mask(geo):
             for d in (0, 1):
                 c |= inside[a:a + 50, b:b + 50, d:d + 150]
     return c
+
+
+def _smart_grid(tmp_path):
+    """The reference's tests/test_011_smart_mesh.py grid: the cubic cavity
+    STL on 50 x 50 x 80 cells refined around its snap points."""
+    import wakis_amd as wa
+
+    geo = json.load(open(os.path.join(HERE, "golden", "001_cubic_cavity_box.json")))
+    path = str(tmp_path / "001_cubic_cavity.stl")
+    write_ascii_stl(path, geo["triangles"])
+    lo, hi = geo["bounds_lo"], geo["bounds_hi"]
+    return wa.GridFIT3D(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], 50, 50, 80,
+                        stl_solids={"cavity": path}, stl_materials={"cavity": "vacuum"},
+                        use_mesh_refinement=True, snap_tol=1e-2, refinement_tol=1e-8)
+
+
+def test_smart_mesh_wakesolve_fp32(tmp_path):
+    """tests/test_011_smart_mesh.py:21-103, 149-153: FP32 wakesolve on the
+    refined non-uniform grid, no charge accumulation in Z (DC bin below 1 %
+    of the peak); the run log lands in the results folder."""
+    import wakis_amd as wa
+
+    grid = _smart_grid(tmp_path)
+    print(f"[case] smart mesh wakesolve fp32 shape={(grid.Nx, grid.Ny, grid.Nz)}", flush=True)
+    folder = str(tmp_path / "results")
+    wake = wa.WakeSolver(q=1e-9, sigmaz=18.5e-3, beta=1.0, xsource=0.0, ysource=0.0,
+                         xtest=0.0, ytest=0.0, save=False, skip_cells=12, results_folder=folder)
+    solver = wa.SolverFIT3D(grid, wake, bc_low=["pec"] * 3, bc_high=["pec"] * 3, use_stl=True,
+                            bg="pec", dtype=np.float32)
+    solver.wakesolve(wakelength=1.0, save_J=False)
+    Z = wake.Z
+    assert np.all(np.isfinite(Z)) and np.all(np.isfinite(wake.WP))
+    assert np.abs(Z[0]) < 0.01 * np.abs(np.max(Z))
+    log = open(os.path.join(folder, "wakis.log"), encoding="utf-8").read()
+    assert '"wakelength": 1.0' in log and '"use_mesh_refinement": true' in log
+
+
+def test_smart_mesh_parity_fp64(tmp_path):
+    """Device vs oracle on the refined grid with its STL mask (FP64, short wake)."""
+    grid = _smart_grid(tmp_path)
+    import wakis_amd as wa
+
+    shape = (grid.Nx, grid.Ny, grid.Nz)
+    print(f"[case] smart mesh parity fp64 shape={shape}", flush=True)
+    mask = grid.solids[0][1]
+    kw = dict(wakelength=0.05, q=1e-9, sigmaz=18.5e-3, beta=1.0, skip_cells=12)
+    og = OGrid(x=grid.x, y=grid.y, z=grid.z)
+    osv = OSolver(og, OWake(**kw), bc_low=["pec"] * 3, bc_high=["pec"] * 3,
+                  solids=[(mask, "vacuum")], bg="pec", dtype=np.float64)
+    pg = wa.GridFIT3D(x=grid.x, y=grid.y, z=grid.z, solids=[(mask, "vacuum")])
+    psv = wa.SolverFIT3D(pg, wa.WakeSolver(save=False, results_folder=str(tmp_path / "r"), **kw),
+                         bc_low=["pec"] * 3, bc_high=["pec"] * 3, use_stl=True, bg="pec",
+                         dtype=np.float64)
+    osv.wakesolve(compute_plane="both")
+    psv.wakesolve(compute_plane="both")
+    errs = {"E": rel(psv.E.toarray(), osv.E.toarray()), "H": rel(psv.H.toarray(), osv.H.toarray()),
+            "WP": rel(psv.wake.WP, osv.wake.WP), "Z": rel(psv.wake.Z, osv.wake.Z),
+            "WPx": rel(psv.wake.WPx, osv.wake.WPx)}
+    print(errs)
+    assert max(errs.values()) <= 1e-10, errs

@@ -103,3 +103,104 @@ def voxelize_nodes(tri, x, y, z) -> np.ndarray:
             above = len(cz) - np.searchsorted(cz, z, side="right")
             inside[i, j, :] = (above % 2) == 1
     return inside
+
+
+def merge_points(tri):
+    """Exactly coincident vertices merged: (points (m, 3), triangles (n, 3)
+    of point ids), triangles with a repeated id dropped (the STL reader's
+    point merging that the edge topology below relies on)."""
+    v = np.asarray(tri, dtype=np.float64).reshape(-1, 3)
+    pts, ids = np.unique(v, axis=0, return_inverse=True)
+    ids = ids.reshape(-1, 3)
+    ok = (ids[:, 0] != ids[:, 1]) & (ids[:, 1] != ids[:, 2]) & (ids[:, 0] != ids[:, 2])
+    return pts, ids[ok]
+
+
+def feature_edge_points(tri, feature_angle=30.0):
+    """Points of the surface's boundary edges (one triangle), non-manifold
+    edges (more than two) and feature edges (two triangles whose unit normals
+    n1, n2 have n1.n2 <= cos(feature_angle)); manifold edges are left out.
+    This is what wakis/gridFIT3D.py:681 asks of pyvista's
+    ``extract_feature_edges(boundary_edges=True, manifold_edges=False)``."""
+    pts, ids = merge_points(tri)
+    if len(ids) == 0:
+        return np.zeros((0, 3))
+    p0, p1, p2 = pts[ids[:, 0]], pts[ids[:, 1]], pts[ids[:, 2]]
+    nrm = np.cross(p1 - p0, p2 - p0)
+    ln = np.linalg.norm(nrm, axis=1)
+    nrm = np.divide(nrm, ln[:, None], out=np.zeros_like(nrm), where=ln[:, None] > 0)
+    e = np.concatenate([ids[:, [0, 1]], ids[:, [1, 2]], ids[:, [2, 0]]])
+    owner = np.tile(np.arange(len(ids)), 3)
+    e = np.sort(e, axis=1)
+    key, inv, cnt = np.unique(e, axis=0, return_inverse=True, return_counts=True)
+    inv = inv.reshape(-1)
+    keep = (cnt == 1) | (cnt > 2)
+    two = np.nonzero(cnt == 2)[0]
+    if len(two):
+        order = np.argsort(inv, kind="stable")
+        first = np.searchsorted(inv[order], two)
+        ta, tb = owner[order[first]], owner[order[first + 1]]
+        dot = np.einsum("ij,ij->i", nrm[ta], nrm[tb])
+        keep[two] = dot <= np.cos(np.deg2rad(feature_angle))
+    used = np.unique(key[keep].reshape(-1))
+    return pts[used]
+
+
+def _closest_dist2(P, a, b, c):
+    """Squared distance of points P (m, 3) to the triangle (a, b, c), by the
+    Voronoi-region method for the closest point on a triangle (Ericson,
+    Real-Time Collision Detection, 5.1.5)."""
+    ab, ac = b - a, c - a
+    ap, bp, cp = P - a, P - b, P - c
+    d1, d2 = ap @ ab, ap @ ac
+    d3, d4 = bp @ ab, bp @ ac
+    d5, d6 = cp @ ab, cp @ ac
+    va = d3 * d6 - d5 * d4
+    vb = d5 * d2 - d1 * d6
+    vc = d1 * d4 - d3 * d2
+    with np.errstate(divide="ignore", invalid="ignore"):
+        den = va + vb + vc
+        v, w = vb / den, vc / den
+        q = a + v[:, None] * ab + w[:, None] * ac
+        # regions in increasing priority: edge BC, edge AC, vertex C, edge AB,
+        # vertex B, vertex A
+        m = (va <= 0) & (d4 - d3 >= 0) & (d5 - d6 >= 0)
+        t = (d4 - d3) / ((d4 - d3) + (d5 - d6))
+        q = np.where(m[:, None], b + t[:, None] * (c - b), q)
+        m = (vb <= 0) & (d2 >= 0) & (d6 <= 0)
+        t = d2 / (d2 - d6)
+        q = np.where(m[:, None], a + t[:, None] * ac, q)
+        m = (d6 >= 0) & (d5 <= d6)
+        q = np.where(m[:, None], c, q)
+        m = (vc <= 0) & (d1 >= 0) & (d3 <= 0)
+        t = d1 / (d1 - d3)
+        q = np.where(m[:, None], a + t[:, None] * ab, q)
+        m = (d3 >= 0) & (d4 <= d3)
+        q = np.where(m[:, None], b, q)
+        m = (d1 <= 0) & (d2 <= 0)
+        q = np.where(m[:, None], a, q)
+    r = P - q
+    return np.einsum("ij,ij->i", r, r)
+
+
+def node_distance(tri, x, y, z, band):
+    """Unsigned distance of every node of the (x, y, z) tensor grid to the
+    triangle surface, exact where it is below `band` and +inf elsewhere."""
+    x, y, z = (np.asarray(a, dtype=np.float64) for a in (x, y, z))
+    d2 = np.full((len(x), len(y), len(z)), np.inf)
+    for a, b, c in np.asarray(tri, dtype=np.float64):
+        lo = np.minimum(np.minimum(a, b), c) - band
+        hi = np.maximum(np.maximum(a, b), c) + band
+        i0, i1 = np.searchsorted(x, lo[0]), np.searchsorted(x, hi[0], side="right")
+        j0, j1 = np.searchsorted(y, lo[1]), np.searchsorted(y, hi[1], side="right")
+        k0, k1 = np.searchsorted(z, lo[2]), np.searchsorted(z, hi[2], side="right")
+        if i0 >= i1 or j0 >= j1 or k0 >= k1:
+            continue
+        X, Y, Z = np.meshgrid(x[i0:i1], y[j0:j1], z[k0:k1], indexing="ij")
+        P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+        dd = _closest_dist2(P, a, b, c).reshape(X.shape)
+        blk = d2[i0:i1, j0:j1, k0:k1]
+        np.minimum(blk, dd, out=blk)
+    d = np.sqrt(d2)
+    d[d > band] = np.inf
+    return d

@@ -8,18 +8,34 @@ solver keeps them as 1-D and 2-D tables (``metric_tables``) instead of four
 first access.
 
 Solids: ``stl_solids``/``stl_materials`` as in the reference (an own STL
-reader and ray-casting voxeliser replace pyvista; a cell belongs to a solid
-if at least one of its 8 corner nodes is inside, as the reference's
-``point_data_to_cell_data() > tol`` test, wakis/gridFIT3D.py:502-510), or
+reader and ray-casting voxeliser replace pyvista), or
 ``solids=[(mask, material), ...]`` with ready (Nx, Ny, Nz) boolean masks.
+``stl_method`` picks the cell rule of wakis/gridFIT3D.py:475-593 from the
+node classification: "legacy" (default) marks a cell if at least one of its
+8 corner nodes is inside (``point_data_to_cell_data() > tol``), and
+"interior_points" if the corner mean exceeds 0.5 (5 or more corners);
+"implicit_distance" marks a cell whose mean corner signed distance to the
+surface is at most half the mean cell size; "voxelize_rectilinear" tests one
+point per cell, (xmin + i*dx, ...) on the uniform spacing of the extents.
+
+Mesh refinement (``use_mesh_refinement``, wakis/gridFIT3D.py:202-208,
+654-870): snap points are the feature-edge points of the STL solids that
+lie within ``snap_tol`` of the planes x=0, y=0 or z=0; each axis is then
+rebuilt around its snap coordinates and relaxed by scipy's ``least_squares``
+on the reference's three penalties (snap displacement, gaps below the
+uniform gap, spread of the gaps).
 """
 
 from __future__ import annotations
 
 import numpy as np
 
+import time
+
 from .field import Field
-from .geometry import read_stl, transform_triangles, voxelize_nodes
+from .geometry import (feature_edge_points, node_distance, read_stl, transform_triangles,
+                       voxelize_nodes)
+from .logger import Logger
 from .materials import material_lib
 
 
@@ -28,10 +44,14 @@ class GridFIT3D:
                  Nx=None, Ny=None, Nz=None, x=None, y=None, z=None, use_mpi=False,
                  stl_solids=None, stl_materials=None, stl_rotate=(0.0, 0.0, 0.0),
                  stl_translate=(0.0, 0.0, 0.0), stl_scale=1.0, stl_colors=None,
-                 stl_tol=1e-3, solids=None, verbose=1, **kwargs):
+                 stl_tol=1e-3, stl_method="legacy", use_mesh_refinement=False,
+                 refinement_method="insert", refinement_tol=1e-8, snap_points=None,
+                 snap_tol=1e-5, snap_solids=None, solids=None, verbose=1, **kwargs):
         if use_mpi:
             raise NotImplementedError(
                 "use_mpi: multi-GPU z-slab decomposition is not part of this build")
+        t0 = time.time()
+        self.logger = Logger()
         self.verbose = verbose
         self.use_mpi = False
         if x is not None and y is not None and z is not None:
@@ -55,15 +75,43 @@ class GridFIT3D:
         self.dx = np.diff(self.x)
         self.dy = np.diff(self.y)
         self.dz = np.diff(self.z)
+        self.update_logger(["Nx", "Ny", "Nz", "dx", "dy", "dz"])
+        self.update_logger(["xmin", "xmax", "ymin", "ymax", "zmin", "zmax"])
 
         self.stl_solids = stl_solids
         self.stl_materials = stl_materials
         self.stl_colors = stl_colors
         self.stl_tol = stl_tol
+        self.stl_method = stl_method
+        self.update_logger(["stl_solids", "stl_materials"])
+        for name, val, default in (("stl_rotate", stl_rotate, (0.0, 0.0, 0.0)),
+                                   ("stl_translate", stl_translate, (0.0, 0.0, 0.0)),
+                                   ("stl_scale", stl_scale, 1.0)):
+            setattr(self, name, val)
+            if isinstance(val, dict) or np.any(np.asarray(val) != np.asarray(default)):
+                self.update_logger([name])
+        if stl_solids is not None:
+            self._prepare_stl_dicts(stl_rotate, stl_translate, stl_scale)
+
+        # axes refined around the solids' snap points (gridFIT3D.py:202-208)
+        self.use_mesh_refinement = use_mesh_refinement
+        self.refinement_method = refinement_method
+        self.snap_points = snap_points
+        self.snap_tol = snap_tol
+        self.snap_solids = snap_solids
+        self.update_logger(["use_mesh_refinement"])
+        if use_mesh_refinement:
+            if snap_points is None:
+                if stl_solids is None:
+                    raise ValueError("use_mesh_refinement needs stl_solids or snap_points")
+                self._compute_snap_points(snap_solids=snap_solids, snap_tol=snap_tol)
+            else:
+                self._snaps_from_points(np.asarray(snap_points, dtype=np.float64).reshape(-1, 3))
+            self._refine_xyz_axis(method=refinement_method, tol=refinement_tol)
+
         # list of (name, mask, material) in application order
         self.solids = []
         if stl_solids is not None:
-            self._prepare_stl_dicts(stl_rotate, stl_translate, stl_scale)
             for key in self.stl_solids:
                 self.solids.append((key, self._mark_cells_in_stl(key), self.stl_materials[key]))
         if solids is not None:
@@ -78,6 +126,13 @@ class GridFIT3D:
                     raise ValueError("solid mask must have shape (Nx, Ny, Nz)")
                 self.solids.append((f"solid{n}", mask, mat))
         self._fields = None
+        self.gridInitializationTime = time.time() - t0
+        self.update_logger(["gridInitializationTime"])
+
+    def update_logger(self, attrs):
+        """Copy grid attributes into ``self.logger.grid`` (wakis/gridFIT3D.py:1088-1098)."""
+        for a in attrs:
+            self.logger.grid[a] = getattr(self, a)
 
     # -- STL (wakis/gridFIT3D.py:397-652) --------------------------------------
     def _prepare_stl_dicts(self, rotate, translate, scale):
@@ -111,15 +166,140 @@ class GridFIT3D:
         return transform_triangles(tri, self.stl_rotate[key], self.stl_translate[key],
                                    self.stl_scale[key])
 
-    def _mark_cells_in_stl(self, key):
-        inside = voxelize_nodes(self.read_stl(key), self.x, self.y, self.z)
-        # cell (i,j,k) is in the solid if any of its 8 corner nodes is inside
-        c = np.zeros((self.Nx, self.Ny, self.Nz), dtype=bool)
+    def _corner_sum(self, node):
+        """Sum over each cell's 8 corner nodes of a (Nx+1, Ny+1, Nz+1) array."""
+        c = np.zeros((self.Nx, self.Ny, self.Nz), dtype=np.result_type(node, np.int32))
         for di in (0, 1):
             for dj in (0, 1):
                 for dk in (0, 1):
-                    c |= inside[di:di + self.Nx, dj:dj + self.Ny, dk:dk + self.Nz]
+                    c = c + node[di:di + self.Nx, dj:dj + self.Ny, dk:dk + self.Nz]
         return c
+
+    def _mark_cells_in_stl(self, key):
+        """Cell mask of one solid by the rule ``self.stl_method`` names."""
+        tri = self.read_stl(key)
+        method = self.stl_method.lower()
+        if method == "voxelize_rectilinear":
+            xs = self.xmin + np.arange(self.Nx) * ((self.xmax - self.xmin) / self.Nx)
+            ys = self.ymin + np.arange(self.Ny) * ((self.ymax - self.ymin) / self.Ny)
+            zs = self.zmin + np.arange(self.Nz) * ((self.zmax - self.zmin) / self.Nz)
+            return voxelize_nodes(tri, xs, ys, zs)
+        inside = voxelize_nodes(tri, self.x, self.y, self.z)
+        if method == "legacy":
+            return self._corner_sum(inside.astype(np.int32)) > 0
+        if method == "interior_points":
+            return self._corner_sum(inside.astype(np.int32)) > 4
+        if method == "implicit_distance":
+            thr = 0.5 * np.mean([np.mean(self.dx), np.mean(self.dy), np.mean(self.dz)])
+            diag = np.sqrt(np.max(self.dx) ** 2 + np.max(self.dy) ** 2 + np.max(self.dz) ** 2)
+            # exact distances within the band decide every cell: a corner
+            # farther than thr + diag from the surface fixes its cell's answer
+            d = node_distance(tri, self.x, self.y, self.z, band=thr + 1.01 * diag)
+            d = np.where(inside, -d, d)
+            return self._corner_sum(d) / 8.0 <= thr
+        raise ValueError(f"[!] Error: stl_method {self.stl_method} not recognized. Use 'legacy', "
+                         "'interior_points', 'implicit_distance', or 'voxelize_rectilinear'.")
+
+    def _mark_cells_in_surface(self, key):
+        """Keep only the surface layer of solid `key`: the cells with a face
+        neighbour of the other mask value (the nonzero-gradient cells of
+        wakis/gridFIT3D.py:606-622, which the reference's SIBC branch
+        would call; that branch is unreachable there, solverFIT3D.py:476)."""
+        for n, (name, mask, mat) in enumerate(self.solids):
+            if name != key:
+                continue
+            m = np.asarray(self.cell_mask(mask), dtype=bool)
+            edge = np.zeros_like(m)
+            for ax in range(3):
+                d = np.diff(m.astype(np.int8), axis=ax) != 0
+                lo = [slice(None)] * 3
+                hi = [slice(None)] * 3
+                lo[ax] = slice(0, -1)
+                hi[ax] = slice(1, None)
+                edge[tuple(lo)] |= d
+                edge[tuple(hi)] |= d
+            self.solids[n] = (name, edge, mat)
+            return edge
+        raise KeyError(key)
+
+    def cell_mask(self, mask):
+        """(Nx, Ny, Nz) boolean array of a solid's mask (slice boxes expanded)."""
+        if isinstance(mask, tuple):
+            m = np.zeros((self.Nx, self.Ny, self.Nz), dtype=bool)
+            m[mask] = True
+            return m
+        return mask
+
+    # -- mesh refinement (wakis/gridFIT3D.py:654-870) --------------------------
+    def _compute_snap_points(self, snap_solids=None, snap_tol=1e-8):
+        """Feature-edge points of the union of the solids lying within
+        `snap_tol` of the planes y=0, x=0 and z=0 (gridFIT3D.py:654-698)."""
+        keys = list(self.stl_solids.keys()) if snap_solids is None else list(snap_solids)
+        tri = np.concatenate([self.read_stl(k) for k in keys])
+        p = feature_edge_points(tri)
+        self.snap_points = np.r_[p[np.abs(p[:, 1]) < snap_tol], p[np.abs(p[:, 0]) < snap_tol],
+                                 p[np.abs(p[:, 2]) < snap_tol]]
+        self._snaps_from_points(self.snap_points)
+
+    def _snaps_from_points(self, pts):
+        """Per-axis snap coordinates: unique values rounded to 5 decimals, with
+        the domain bounds added."""
+        self.snap_points = pts
+        for ax, (lo, hi) in enumerate(((self.xmin, self.xmax), (self.ymin, self.ymax),
+                                       (self.zmin, self.zmax))):
+            v = np.unique(np.round(pts[:, ax], 5))
+            setattr(self, "xyz"[ax] + "_snaps", np.unique(np.concatenate(([lo], v, [hi]))))
+
+    def refine_axis(self, xmin, xmax, Nx, x_snaps, method="insert", tol=1e-12):
+        """An axis of about `Nx` nodes on [xmin, xmax] through the snap points
+        (gridFIT3D.py:771-842).  Start: "insert" adds the snaps to a uniform
+        axis of Nx - len(snaps) nodes, "neighbor" moves the nearest uniform
+        node (within half a gap) onto each snap, "subdivision" starts from
+        the snaps and halves the longest gap until there are Nx nodes (the
+        reference's branch raises UnboundLocalError before it starts).  Then
+        ``least_squares`` (dogbox, arctan loss, 3-point Jacobian) on the
+        normalised nodes in [0, 1] minimises the three penalties."""
+        from scipy.optimize import least_squares
+
+        def penalties(u, u0, fixed):
+            moved = np.sum((u[fixed] - u0[fixed]) ** 2) * 1000
+            gap = np.diff(u)
+            h = 1 / (len(u) - 1)
+            small = np.sum((h - gap[gap < h]) ** 2)
+            spread = np.std(np.diff(u)) * 10
+            return np.hstack([moved, small, spread])
+
+        s = (np.asarray(x_snaps, dtype=np.float64) - xmin) / (xmax - xmin)
+        if method == "insert":
+            u0 = np.unique(np.append(s, np.linspace(0, 1, Nx - len(s))))
+        elif method == "neighbor":
+            u0 = np.linspace(0, 1, Nx)
+            h = np.diff(u0)[0]
+            for v in s:
+                near = np.isclose(u0, v, rtol=0.0, atol=h / 2)
+                if np.sum(near) > 0:
+                    u0[np.argmax(near)] = v
+        elif method == "subdivision":
+            u0 = np.unique(s)
+            while len(u0) < Nx:
+                i = np.argsort(np.diff(u0))[-1]
+                u0 = np.unique(np.insert(u0, i + 1, u0[i] + (u0[i + 1] - u0[i]) / 2))
+        else:
+            raise ValueError(f"Method {method} not supported. Use 'insert', 'neighbor' or "
+                             "'subdivision'.")
+        fixed = np.isin(u0, s)
+        res = least_squares(penalties, x0=u0.copy(), bounds=(0, 1), jac="3-point",
+                            method="dogbox", loss="arctan", gtol=tol, ftol=tol, xtol=tol,
+                            verbose=0, args=(u0.copy(), fixed.copy()))
+        return res.x * (xmax - xmin) + xmin
+
+    def _refine_xyz_axis(self, method="insert", tol=1e-6):
+        """gridFIT3D.py:844-906: every axis refined, then counts and spacings."""
+        self.x = self.refine_axis(self.xmin, self.xmax, self.Nx + 1, self.x_snaps, method, tol)
+        self.y = self.refine_axis(self.ymin, self.ymax, self.Ny + 1, self.y_snaps, method, tol)
+        self.z = self.refine_axis(self.zmin, self.zmax, self.Nz + 1, self.z_snaps, method, tol)
+        self.Nx, self.Ny, self.Nz = len(self.x) - 1, len(self.y) - 1, len(self.z) - 1
+        self.dx, self.dy, self.dz = np.diff(self.x), np.diff(self.y), np.diff(self.z)
 
     # -- metric Fields, built on demand (wakis/gridFIT3D.py:263-305) ----------
     def _metric_fields(self):

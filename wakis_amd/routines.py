@@ -127,7 +127,7 @@ class RoutinesMixin:
         t_axis = np.arange(0, Nt * self.dt, self.dt)
         self.ctx.wake_probe(self.ixt, self.iyt, h, max(Nt, 1))
         self.ctx.set_step_index(0)
-        return {"Nt": Nt, "beam": beam, "h": h, "t": t_axis}
+        return {"Nt": Nt, "beam": beam, "h": h, "t": t_axis, "t0": time.time()}
 
     def _wake_device_beam(self, run):
         """The Gaussian bunch evaluated on the device at every step."""
@@ -149,7 +149,7 @@ class RoutinesMixin:
         else:
             self._set_beam_line(int(beam.ixs), int(beam.iys), beam.line(self, (Nt - 1) * self.dt))
 
-    def _wake_finish(self, run, compute_plane):
+    def _wake_finish(self, run, compute_plane, save_logs=True):
         """Hand the device E_z history to the WakeSolver and solve
         (wakis/routines.py:456-485)."""
         h = run["h"]
@@ -165,3 +165,9 @@ class RoutinesMixin:
         }
         self.wake.attach_device_history(self.ctx, meta)
         self.wake.solve(compute_plane=compute_plane)
+        # run metadata next to the results (wakis/routines.py:481-485)
+        self.logger.wakeSolver = self.wake.logger.wakeSolver
+        self.logger.wakeSolver["wakelength"] = self.wake.wakelength
+        self.logger.wakeSolver["simulationTime"] = time.time() - run["t0"]
+        if save_logs:
+            self.logger.save_logs()

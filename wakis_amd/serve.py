@@ -260,7 +260,7 @@ class Server:
                         s = it["solver"]
                         tf = time.monotonic()
                         s._wake_device_end(it["run"])
-                        s._wake_finish(it["run"], self.compute_plane)
+                        s._wake_finish(it["run"], self.compute_plane, save_logs=False)
                         w = it["wake"]
                         it["req"].result = {"WP": np.asarray(w.WP).copy(), "Z": np.asarray(w.Z).copy(),
                                             "f": np.asarray(w.f).copy(), "s": np.asarray(w.s).copy(),

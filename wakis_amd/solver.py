@@ -25,6 +25,7 @@ from scipy.constants import mu_0
 from . import _lib
 from .field import DeviceField, Field
 from .grid import metric_tables
+from .logger import Logger
 from .materials import material_lib
 from .routines import RoutinesMixin
 
@@ -125,6 +126,7 @@ class SolverFIT3D(RoutinesMixin):
         if use_conductors:
             raise NotImplementedError("use_conductors: legacy conductor masks are not supported")
         self.verbose = verbose
+        self.logger = Logger()
         self.step_0 = True
         self.nstep = 0
         self.use_stl = use_stl
@@ -136,6 +138,7 @@ class SolverFIT3D(RoutinesMixin):
         self.activate_pml = False
         self.use_conductivity = False
         self.one_step = self._one_step
+        self.update_logger(["use_gpu", "use_mpi"])
 
         self.grid = grid
         self.background = bg
@@ -145,8 +148,11 @@ class SolverFIT3D(RoutinesMixin):
         self.x = grid.x[:-1] + self.dx / 2
         self.y = grid.y[:-1] + self.dy / 2
         self.z = grid.z[:-1] + self.dz / 2
+        self.update_logger(["grid", "background"])
 
         self.wake = wake
+        if wake is not None:
+            self.logger.wakeSolver = wake.logger.wakeSolver
         if wake is not None and fmax == 1e9:
             self.fmax = wake.fmax
         self.dtype = dtype
@@ -157,6 +163,7 @@ class SolverFIT3D(RoutinesMixin):
         self.cdtype = np.float64 if precision == "mixed" else dtype
         cdtype = self.cdtype
         self.bc_low, self.bc_high = list(bc_low), list(bc_high)
+        self.update_logger(["bc_low", "bc_high"])
 
         # boundaries (wakis/boundaries.py:14-188)
         self.pec_bits, self.pmc_bits = face_bits(self.bc_low, self.bc_high)
@@ -188,6 +195,7 @@ class SolverFIT3D(RoutinesMixin):
         if self.activate_pml:
             self.n_pml = n_pml
             self._initialize_PML()
+            self.update_logger(["n_pml"])
         self._mat_fields = None
 
         # dt (solverFIT3D.py:291-318)
@@ -199,6 +207,7 @@ class SolverFIT3D(RoutinesMixin):
         else:
             self.dt = dt
         self.dt = dt_np(self.dt)
+        self.update_logger(["dt"])
         if self.use_conductivity:
             self._relaxation_clamp()
 
@@ -219,6 +228,16 @@ class SolverFIT3D(RoutinesMixin):
             self.H = DeviceField(self.ctx, _lib.FIELD_H, self)
             self.J = DeviceField(self.ctx, _lib.FIELD_J, self)
         self.solverInitializationTime = time.time() - t0
+        self.update_logger(["solverInitializationTime"])
+
+    def update_logger(self, attrs):
+        """Copy solver attributes into ``self.logger.solver``; "grid" links
+        the grid's own log (wakis/solverFIT3D.py:1037-1051)."""
+        for a in attrs:
+            if a == "grid":
+                self.logger.grid = getattr(self.grid, "logger", Logger()).grid
+            else:
+                self.logger.solver[a] = getattr(self, a)
 
     @property
     def dt(self):

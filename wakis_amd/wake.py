@@ -19,6 +19,8 @@ import os
 import numpy as np
 from scipy.constants import c as c_light
 
+from .logger import Logger
+
 
 class WakeSolver:
     def __init__(self, wakelength=None, q=1e-9, sigmaz=1e-3, beta=1.0, xsource=0.0,
@@ -61,6 +63,16 @@ class WakeSolver:
             os.makedirs(self.folder, exist_ok=True)
         self._ctx = None
         self._meta = None
+        self.logger = Logger()
+        self.assign_logs()
+
+    def assign_logs(self):
+        """The wake parameters in ``self.logger.wakeSolver``
+        (wakis/wakeSolver.py:1713-1727)."""
+        for key in ("ti", "q", "sigmaz", "beta", "xsource", "ysource", "xtest", "ytest",
+                    "chargedist", "skip_cells"):
+            self.logger.wakeSolver[key] = getattr(self, key)
+        self.logger.wakeSolver["results_folder"] = self.folder
 
     def attach_device_history(self, ctx, meta):
         """Use the E_z history of a device context (written by wakesolve)."""
