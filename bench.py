@@ -83,6 +83,260 @@ def cpu_baseline(nz_sample=32, steps=150):
     }
 
 
+class PlainSparseStepper:
+    """The reference's own GPU time step, for the plain-method baseline
+    (wakis/solverFIT3D.py:337-367 and 504-525 as run with cupyx): the
+    operators M_H = iDa iDmu C Ds and M_E = iDeps itDa C^T tDs as FP64
+    sparse matrices on the device (the CSR structure of the reference, zero
+    entries dropped as scipy's products drop them), stepped in the
+    reference's expression order:
+
+        H[:] = H - (dt * M_H) @ E        dt * M_H: a new matrix each step
+                                         (values scaled)
+        E[:] = E + dt * (M_E @ H - ieps * J)
+        J[:] = sigma * E                 (conductive solvers)
+
+    The products run as coalesced COO matrix-vector products through torch:
+    its CSR product on this ROCm build moves about 12 GB/s (measured), the
+    COO one about 30 times more.  The index arrays are shared by dt * M_H
+    rather than copied as cupyx's scalar product copies them, which favours
+    this baseline.
+
+    `wake_step(n)` adds the rest of the reference's wakesolve iteration
+    (wakis/routines.py:438-448): the beam line written into J_z from host
+    values and the 3x3 E_z window read back to the host.  Built from a
+    wakis_amd solver's host data (metric tables, material ids and table,
+    face masks, dt); the fields start at zero."""
+
+    def __init__(self, solver, device="cuda"):
+        import warnings
+
+        import torch
+
+        warnings.filterwarnings("ignore", message="Sparse CSR tensor support is in beta")
+        self.torch = torch
+        dev = torch.device(device)
+        self.dev = dev
+        nx, ny, nz = solver.Nx, solver.Ny, solver.Nz
+        N = nx * ny * nz
+        self.solver, self.N, self.shape = solver, N, (nx, ny, nz)
+        geo = np.asarray(solver.geometry, dtype=np.float64)
+        sizes = [nx, ny, nz, nx, ny, nz, ny * nz, nx * nz, nx * ny, ny * nz, nx * nz, nx * ny]
+        parts, o = [], 0
+        for m in sizes:
+            parts.append(torch.from_numpy(geo[o:o + m].copy()).to(dev))
+            o += m
+        Lt, tLt, iAt, itAt = parts[0:3], parts[3:6], parts[6:9], parts[9:12]
+        n = torch.arange(N, device=dev, dtype=torch.int64)
+        i, j, k = n % nx, (n // nx) % ny, n // (nx * ny)
+        self._ijk = (i, j, k)
+        L = [Lt[0][i], Lt[1][j], Lt[2][k]]
+        tL = [tLt[0][i], tLt[1][j], tLt[2][k]]
+        iA = [iAt[0][j + ny * k], iAt[1][i + nx * k], iAt[2][i + nx * j]]
+        itA = [itAt[0][j + ny * k], itAt[1][i + nx * k], itAt[2][i + nx * j]]
+        tab = torch.from_numpy(solver._mt.table()).to(dev)
+        ids = torch.from_numpy(np.reshape(solver._mt.ids, -1, order="F").astype(np.int64)).to(dev)
+        mat = tab[ids]  # (N, 9): 1/eps, 1/mu, sigma per component
+        keep = lambda bits, c: (~self._masked(bits, c)).to(torch.float64)
+        pec = [keep(solver.pec_bits, c) for c in range(3)]
+        pmc = [keep(solver.pmc_bits, c) for c in range(3)]
+        sx, sy, sz = 1, nx, nx * ny
+        # curl rows: (column component, flat shift, sign) of each of the 4 terms
+        h_terms = {0: [(1, 0, 1.0), (1, sz, -1.0), (2, 0, -1.0), (2, sy, 1.0)],
+                   1: [(2, 0, 1.0), (2, sx, -1.0), (0, 0, -1.0), (0, sz, 1.0)],
+                   2: [(0, 0, 1.0), (0, sy, -1.0), (1, 0, -1.0), (1, sx, 1.0)]}
+        e_terms = {0: [(2, 0, 1.0), (2, -sy, -1.0), (1, 0, -1.0), (1, -sz, 1.0)],
+                   1: [(0, 0, 1.0), (0, -sz, -1.0), (2, 0, -1.0), (2, -sx, 1.0)],
+                   2: [(1, 0, 1.0), (1, -sx, -1.0), (0, 0, -1.0), (0, -sy, 1.0)]}
+        # M_H row (c, n): (iA_c imu_c pmc_c)[n] * sign * (L_p pec_p)[m]
+        rh = [iA[c] * mat[:, 3 + c] * pmc[c] for c in range(3)]
+        self.MH = self._coo(self._csr(h_terms, rh, [L[p] * pec[p] for p in range(3)]))
+        # M_E row (c, n): (ieps_c itA_c pec_c)[n] * sign * (tL_p pmc_p)[m]
+        re = [mat[:, c] * itA[c] * pec[c] for c in range(3)]
+        self.ME = self._coo(self._csr(e_terms, re, [tL[p] * pmc[p] for p in range(3)]))
+        self.ieps = torch.cat([mat[:, c] for c in range(3)]).contiguous()
+        self.sigma = torch.cat([mat[:, 6 + c] for c in range(3)]).contiguous()
+        self.dt = float(solver.dt)
+        self.cond = bool(solver.use_conductivity)
+        self.E = torch.zeros(3 * N, dtype=torch.float64, device=dev)
+        self.H = torch.zeros_like(self.E)
+        self.J = torch.zeros_like(self.E)
+        self.step_0 = True
+        del L, tL, iA, itA, mat, ids, rh, re, pec, pmc
+
+    def _masked(self, bits, c):
+        """Face mask of component c (last face written wins, in the order
+        xlo, xhi, ylo, yhi, zlo, zhi: wakis/boundaries.py:90-100, 130-140)."""
+        torch = self.torch
+        i, j, k = self._ijk
+        nx, ny, nz = self.shape
+        r = torch.zeros(self.N, dtype=torch.bool, device=self.dev)
+        for axis, idx, last in ((0, i, nx - 1), (1, j, ny - 1), (2, k, nz - 1)):
+            if c == axis:
+                continue
+            r = torch.where(idx == 0, bool(bits & (1 << (2 * axis))), r)
+            r = torch.where(idx == last, bool(bits & (1 << (2 * axis + 1))), r)
+        return r
+
+    def _csr(self, terms, rowc, colv):
+        torch = self.torch
+        N = self.N
+        n = torch.arange(N, device=self.dev, dtype=torch.int64)
+        crow_parts, col_parts, val_parts = [], [], []
+        for c in range(3):
+            cols, vals = [], []
+            for p, shift, sign in terms[c]:
+                m = n + shift
+                ok = (m >= 0) & (m < N)
+                mc = m.clamp(0, N - 1)
+                v = (rowc[c] * sign) * colv[p][mc]
+                vals.append(torch.where(ok, v, torch.zeros_like(v)))
+                cols.append(p * N + mc)
+            col = torch.stack(cols, 1)
+            val = torch.stack(vals, 1)
+            del cols, vals
+            nz = val != 0
+            col = torch.where(nz, col, torch.full_like(col, 3 * N))
+            col, order = torch.sort(col, dim=1)
+            val = torch.gather(val, 1, order)
+            nz = col < 3 * N
+            crow_parts.append(nz.sum(1))
+            col_parts.append(col[nz].to(torch.int32))
+            val_parts.append(val[nz])
+            del col, val, nz, order
+        counts = torch.cat(crow_parts)
+        crow = torch.zeros(3 * N + 1, dtype=torch.int64, device=self.dev)
+        crow[1:] = torch.cumsum(counts, 0)
+        return (crow.to(torch.int32), torch.cat(col_parts), torch.cat(val_parts))
+
+    def _coo(self, csr):
+        """(indices (2, nnz) int64, values) of a CSR triple, rows ascending."""
+        torch = self.torch
+        crow, col, val = csr
+        counts = (crow[1:] - crow[:-1]).to(torch.int64)
+        rows = torch.repeat_interleave(torch.arange(len(counts), device=self.dev), counts)
+        return torch.stack([rows, col.to(torch.int64)]), val
+
+    def _zero_ghosts(self):
+        """wakis/solverFIT3D.py:842-862."""
+        i, j, k = self._ijk
+        nx, ny, nz = self.shape
+        N = self.N
+        li, lj, lk = i == nx - 1, j == ny - 1, k == nz - 1
+        for c, m in ((0, lj | lk), (1, li | lk), (2, li | lj)):
+            self.H[c * N:(c + 1) * N][m] = 0.0
+        for c, m in ((0, li), (1, lj), (2, lk)):
+            self.E[c * N:(c + 1) * N][m] = 0.0
+
+    def one_step(self):
+        torch = self.torch
+        if self.step_0:
+            self._zero_ghosts()
+            self.step_0 = False
+        size = (3 * self.N, 3 * self.N)
+        idx, val = self.MH
+        mh_dt = torch.sparse_coo_tensor(idx, self.dt * val, size=size, is_coalesced=True)
+        self.H.copy_(self.H - torch.mv(mh_dt, self.E))
+        me = torch.sparse_coo_tensor(*self.ME, size=size, is_coalesced=True)
+        self.E.copy_(self.E + self.dt * (torch.mv(me, self.H) - self.ieps * self.J))
+        if self.cond:
+            self.J.copy_(self.sigma * self.E)
+
+    def setup_wake(self, beam, ixt, iyt):
+        """Beam line and E_z window positions of a wakesolve loop."""
+        torch = self.torch
+        nx, ny, nz = self.shape
+        N = self.N
+        self.beam = beam
+        beam.line(self.solver, 0.0)  # first update: source indices and zmin
+        self._jline = 2 * N + int(beam.ixs) + nx * int(beam.iys) + nx * ny * torch.arange(
+            nz, device=self.dev)
+        ii = torch.arange(ixt - 1, ixt + 2, device=self.dev)
+        jj = torch.arange(iyt - 1, iyt + 2, device=self.dev)
+        kk = torch.arange(nz, device=self.dev)
+        self._win = (2 * N + ii[:, None, None] + nx * jj[None, :, None]
+                     + nx * ny * kk[None, None, :]).reshape(-1)
+
+    def wake_step(self, n):
+        """One wakesolve iteration: beam.update, the E_z window to the host,
+        one_step (wakis/routines.py:438-448)."""
+        line = self.beam.line(self.solver, n * self.dt)
+        self.J[self._jline] = self.torch.from_numpy(np.asarray(line, dtype=np.float64)).to(self.dev)
+        ez = self.E[self._win].cpu()
+        self.one_step()
+        return ez
+
+
+def plain_ms_per_step(solver, steps=60, warmup=5, wake=True):
+    """Wall milliseconds per wakesolve iteration of the plain FP64 GPU method
+    on `solver`'s grid (the E_z read-back synchronises every iteration)."""
+    import torch
+    from wakis_amd.sources import Beam
+
+    st = PlainSparseStepper(solver)
+    w = solver.wake
+    beam = Beam(q=w.q, sigmaz=w.sigmaz, beta=w.beta, xsource=w.xsource, ysource=w.ysource)
+    ixt = int(np.abs(solver.x - w.xtest).argmin())
+    iyt = int(np.abs(solver.y - w.ytest).argmin())
+    st.setup_wake(beam, ixt, iyt)
+    for n in range(warmup):
+        st.wake_step(n)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for n in range(warmup, warmup + steps):
+        st.wake_step(n)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    nnz = int(st.MH[1].numel() + st.ME[1].numel())
+    del st
+    torch.cuda.empty_cache()
+    return ms, nnz
+
+
+def plain_child(nreq, steps):
+    """ms per wakesolve iteration of the plain FP64 GPU method for every
+    request of the serving workload, measured in a child process."""
+    import subprocess
+
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--workload", "plain", "--nreq",
+           str(nreq), "--plain-steps", str(steps)]
+    out = subprocess.run(cmd, stdout=subprocess.PIPE, timeout=900, check=True).stdout
+    rows = json.loads(out.decode().strip().splitlines()[-1])["plain"]
+    return {r["rid"]: (r["ms"], r["nnz"]) for r in rows}
+
+
+def plain_floor_bytes(n_cells, nnz, conductive=True):
+    """Fewest HBM bytes one step of the reference's sparse method can move
+    (FP64): both matrices read once in CSR with 32-bit column indices
+    (12 B per entry + 4 B row pointer per row), dt * M_H written as a new
+    value array (16 B per entry of M_H, half of nnz), and the 27 vector
+    passes of 3N doubles of the step's expression (H: product output, H - y,
+    the copy back; E: product, ieps * J, the difference, dt * (), E + (),
+    the copy back; J: sigma * E and the copy back), each read and write
+    counted once."""
+    words = 7 + 15 + (5 if conductive else 0)
+    return 12 * nnz + 2 * 3 * n_cells * 4 + 16 * (nnz // 2) + words * 3 * n_cells * 8
+
+
+def bench_plain(args):
+    """Child of the serving benchmark: the plain method on each request's
+    grid (host setup only in this process, no wakis_amd device context)."""
+    import torch  # first: its HIP runtime owns the device in this process
+
+    from wakis_amd.serve import make_requests
+    from wakis_amd.workloads import build
+
+    torch.cuda.init()
+    rows = []
+    for r in make_requests(args.nreq, seed=4, rate=2.0):
+        _, _, hs = build(r.workload, dtype=np.float64, device=None, precision="reference")
+        ms, nnz = plain_ms_per_step(hs, steps=args.plain_steps)
+        print(f"[plain] req {r.rid:3d} N {r.cells:9d} nnz {nnz} {ms:.3f} ms/step",
+              file=sys.stderr, flush=True)
+        rows.append({"rid": r.rid, "ms": ms, "nnz": nnz, "cells": r.cells})
+    return {"plain": rows}
+
+
 def bench_batch(args):
     """64 uneven rectangular PEC cavities (box 60-140 mm, pipe 10-25 mm,
     length 60-160 mm, dx = 1.5 mm, z-end PML, beta in {0.8, 0.9, 1.0}) in
@@ -182,21 +436,60 @@ def bench_serve(args):
     print(f"[bench] device stepping {busy_ms / 1e3:.1f}s of {wall:.1f}s wall, "
           f"{cell_steps / (busy_ms * 1e-3) / 1e9:.1f} Gcell-steps/s while stepping, "
           f"{len(srv.log)} chunks", file=sys.stderr, flush=True)
-    # one at a time in FP64 (the plain path), same arrivals, FIFO
-    durs, errs = [], []
+    # one at a time in FP64, same arrivals, FIFO: (a) this library's FP64
+    # kernel; (b) the reference's plain GPU method (sparse operators), its
+    # stepping timed on each request's grid and scaled to the request's Nt,
+    # plus (a)'s post-processing time
+    durs, errs, durs_plain, durs_floor, nts, post = [], [], [], [], [], []
     for r in done:
         d, ref = solve_alone(r, dtype=np.float64)
         durs.append(d)
         errs.append(z_rel_error(r.result["Z"], ref["Z"], ref["f"], np.inf))
+        nts.append(ref["Nt"])
+        post.append(d - ref["stepping_s"])
         print(f"[bench] req {r.rid:3d} N {r.cells:9d} Nt {r.steps:6d} arr {r.arrival:6.2f} "
               f"adm {r.admitted:6.2f} fin {r.finished:6.2f} prep {r.prep_s:5.2f}s "
               f"post {r.finish_s:5.2f}s fp64 {d:5.2f}s dZ {errs[-1]:.2e} "
               f"finite {bool(np.isfinite(r.result['Z']).all())}/{bool(np.isfinite(ref['Z']).all())}",
               file=sys.stderr, flush=True)
+    if not args.no_plain:
+        # torch carries its own HIP runtime, which cannot start on a device
+        # this process's runtime already uses: the plain leg runs in a child
+        from wakis_amd import _lib
+
+        ms = plain_child(args.nreq, args.plain_steps)
+        copy_gbs = 2 * (2 << 30) / (_lib.copy_bench(2 << 30, reps=10) * 1e-3) / 1e9
+        for r, nt, p in zip(done, nts, post):
+            pms, nnz = ms[r.rid]
+            durs_plain.append(nt * pms * 1e-3 + p)
+            floor_ms = plain_floor_bytes(r.cells, nnz) / (copy_gbs * 1e9) * 1e3
+            durs_floor.append(nt * floor_ms * 1e-3 + p)
+            print(f"[bench] req {r.rid:3d} plain FP64 GPU {pms:8.2f} ms/step (floor "
+                  f"{floor_ms:7.2f} at {copy_gbs:.0f} GB/s) -> {durs_plain[-1]:7.1f}s "
+                  f"(floor {durs_floor[-1]:7.1f}s)", file=sys.stderr, flush=True)
     arr = np.array([r.arrival for r in done])
     lat64 = fifo_latencies(arr, durs)
     span64 = max(arr + lat64) - arr.min()
     per_h64 = 3600.0 * len(done) / span64
+    plain = {}
+    if durs_plain:
+        latp = fifo_latencies(arr, durs_plain)
+        per_hp = 3600.0 * len(done) / (max(arr + latp) - arr.min())
+        latf = fifo_latencies(arr, durs_floor)
+        per_hf = 3600.0 * len(done) / (max(arr + latf) - arr.min())
+        plain = {
+            "plain_fp64_gpu_one_at_a_time_solves_per_h": round(per_hp, 2),
+            "plain_fp64_gpu_p50_s": round(float(np.percentile(latp, 50)), 2),
+            "plain_fp64_gpu_p99_s": round(float(np.percentile(latp, 99)), 2),
+            "speedup_vs_plain_fp64_gpu": round(served_per_h / per_hp, 2),
+            "plain_fp64_gpu_floor_solves_per_h": round(per_hf, 2),
+            "speedup_vs_plain_fp64_gpu_floor": round(served_per_h / per_hf, 2),
+            "plain_floor": "each plain step at its fewest bytes (plain_floor_bytes) moved at "
+                           "the measured device copy rate",
+            "plain_method": "reference's GPU step: FP64 sparse M_H, M_E (torch COO products), "
+                            "dt*M_H rebuilt per step, vector temporaries, beam line H2D and "
+                            f"E_z window D2H per step; {args.plain_steps} steps timed per request",
+        }
     cells = np.array([r.cells for r in done])
     return {
         "metric": "solves/h",
@@ -222,6 +515,7 @@ def bench_serve(args):
             "fp64_one_at_a_time_p99_s": round(float(np.percentile(lat64, 99)), 3),
             "fp64_mean_service_s": round(float(np.mean(durs)), 3),
             "speedup_solves_per_h": round(served_per_h / per_h64, 3),
+            **plain,
             "max_rel_dZ_vs_fp64": float(np.max(errs)),
             "device_busy_frac": round(busy_ms / 1e3 / wall, 3),
             "chunk_steps": args.chunk,
@@ -427,7 +721,7 @@ def main():
     ap.add_argument("--tile", type=int, default=None, help="fused tile configuration id")
     ap.add_argument("--wakelength", type=float, default=None, help="precision: wakelength (m)")
     ap.add_argument("--workload", default="pillbox",
-                    choices=["pillbox", "batch", "serve", "precision", "large"],
+                    choices=["pillbox", "batch", "serve", "precision", "large", "plain"],
                     help="pillbox: one 256x256x768 solve (the headline); batch: 64 cavities in "
                          "one launch per step; serve: Poisson impedance requests; large: "
                          "the 1024^3 stratified box, FP32 vs FP64")
@@ -435,6 +729,10 @@ def main():
     ap.add_argument("--nreq", type=int, default=24, help="serve: number of requests")
     ap.add_argument("--chunk", type=int, default=256, help="serve: steps per batch call")
     ap.add_argument("--n-large", type=int, default=1024, help="large: cells per side")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="serve: skip the plain (sparse-operator) FP64 GPU baseline")
+    ap.add_argument("--plain-steps", type=int, default=60,
+                    help="serve: timed iterations of the plain baseline per request")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 2000 if args.workload == "large" else 1000
@@ -453,6 +751,9 @@ def main():
         return
     if args.workload == "large":
         print(json.dumps(bench_large(args)), flush=True)
+        return
+    if args.workload == "plain":
+        print(json.dumps(bench_plain(args)), flush=True)
         return
 
     from wakis_amd.workloads import build, lossy_pillbox

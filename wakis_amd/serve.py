@@ -294,7 +294,8 @@ def solve_alone(req: Request, dtype=np.float64, tile=None, compute_plane="longit
     solver.ctx.sync()
     dt = time.monotonic() - t0
     res = {"WP": np.asarray(wake.WP).copy(), "Z": np.asarray(wake.Z).copy(),
-           "f": np.asarray(wake.f).copy(), "Nt": solver.Nt, "N": solver.N}
+           "f": np.asarray(wake.f).copy(), "Nt": solver.Nt, "N": solver.N,
+           "stepping_s": solver.simulationTime}
     solver.ctx.close()
     return dt, res
 
