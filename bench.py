@@ -293,13 +293,14 @@ def plain_ms_per_step(solver, steps=60, warmup=5, wake=True):
     return ms, nnz
 
 
-def plain_child(nreq, steps):
+def plain_child(nreq, steps, target="serve", grid="256,256,768"):
     """ms per wakesolve iteration of the plain FP64 GPU method for every
-    request of the serving workload, measured in a child process."""
+    request of the serving workload (or the pillbox grid), measured in a
+    child process."""
     import subprocess
 
     cmd = [sys.executable, "-u", os.path.abspath(__file__), "--workload", "plain", "--nreq",
-           str(nreq), "--plain-steps", str(steps)]
+           str(nreq), "--plain-steps", str(steps), "--plain-target", target, "--grid", grid]
     out = subprocess.run(cmd, stdout=subprocess.PIPE, timeout=900, check=True).stdout
     rows = json.loads(out.decode().strip().splitlines()[-1])["plain"]
     return {r["rid"]: (r["ms"], r["nnz"]) for r in rows}
@@ -328,6 +329,14 @@ def bench_plain(args):
 
     torch.cuda.init()
     rows = []
+    if args.plain_target == "pillbox":
+        from wakis_amd.workloads import lossy_pillbox
+
+        nx, ny, nz = (int(v) for v in args.grid.split(","))
+        _, _, hs = build(lossy_pillbox(nx, ny, nz), dtype=np.float64, device=None)
+        ms, nnz = plain_ms_per_step(hs, steps=args.plain_steps)
+        print(f"[plain] pillbox N {hs.N} nnz {nnz} {ms:.3f} ms/step", file=sys.stderr, flush=True)
+        return {"plain": [{"rid": 0, "ms": ms, "nnz": nnz, "cells": hs.N}]}
     for r in make_requests(args.nreq, seed=4, rate=2.0):
         _, _, hs = build(r.workload, dtype=np.float64, device=None, precision="reference")
         ms, nnz = plain_ms_per_step(hs, steps=args.plain_steps)
@@ -733,6 +742,10 @@ def main():
                     help="serve: skip the plain (sparse-operator) FP64 GPU baseline")
     ap.add_argument("--plain-steps", type=int, default=60,
                     help="serve: timed iterations of the plain baseline per request")
+    ap.add_argument("--plain", action="store_true",
+                    help="pillbox: also time the plain (sparse-operator) FP64 GPU method")
+    ap.add_argument("--plain-target", default="serve", choices=["serve", "pillbox"],
+                    help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 2000 if args.workload == "large" else 1000
@@ -828,6 +841,15 @@ def main():
     copy_bytes = 2 * 12 * esz * N  # same footprint as the fields (read + write)
     copy_ms = _lib.copy_bench(copy_bytes // 2, reps=10)
     copy_gbs = copy_bytes / (copy_ms * 1e-3) / 1e9
+    if fused:
+        # a plain copy of the twelve field streams with the sweep's traversal
+        try:
+            tiled_ms = _lib.copy_bench_tiled(nx, ny, nz, tile[0], dtype=dtype, reps=10)
+            tiled_gbs = 12 * esz * N / (tiled_ms * 1e-3) / 1e9
+            extra["copy_same_traversal_gbs"] = round(tiled_gbs, 1)
+            extra["frac_of_copy_same_traversal"] = round(achieved / tiled_gbs, 4)
+        except _lib.FitError as e:
+            print(f"[bench] tiled copy calibration failed: {e}", file=sys.stderr, flush=True)
 
     out = {
         "metric": "Gcell-steps/s",
@@ -865,6 +887,15 @@ def main():
             **extra,
         },
     }
+    if args.plain:
+        pms, nnz = plain_child(0, args.plain_steps, "pillbox", args.grid)[0]
+        out["config"]["plain_fp64_gpu"] = {
+            "gcell_steps_per_s": round(N / (pms * 1e-3) / 1e9, 3),
+            "ms_per_step": round(pms, 3),
+            "nnz": nnz,
+            "method": "the reference's GPU step (sparse M_H, M_E in FP64, torch COO products) "
+                      "with its wakesolve iteration (beam line H2D, E_z window D2H)",
+        }
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()

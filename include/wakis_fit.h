@@ -191,6 +191,14 @@ int fit_step_timed(fit_ctx *ctx, int32_t nsteps, int32_t conductive, int32_t per
  * achievable HBM bandwidth and the rocprof FETCH/WRITE counters.  No
  * counterpart in the reference. */
 int fit_copy_bench(int32_t device, int64_t nbytes, int32_t reps, double *ms);
+/* Copy of six (nx, ny, nz) arrays of the given dtype into six others, with
+ * the traversal of fused tile configuration `cfg` (a block per tile of
+ * columns marching through its planes), timed over reps launches: *ms =
+ * average milliseconds per launch.  The achievable rate of the sweep's own
+ * access pattern; needs 12 N elements of device memory.  No counterpart in
+ * the reference. */
+int fit_copy_bench_tiled(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t dtype,
+                         int32_t cfg, int32_t reps, double *ms);
 /* Step counter used by the beam time and the probe column. */
 int fit_set_step_index(fit_ctx *ctx, int64_t step);
 int fit_get_step_index(fit_ctx *ctx, int64_t *step);

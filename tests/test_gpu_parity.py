@@ -348,3 +348,15 @@ def test_save_reset_load_state(tmp_path):
         b.one_step()
     for n in "EHJ":
         assert np.array_equal(getattr(a, n).toarray(), getattr(b, n).toarray()), n
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["fp32", "fp64"])
+def test_tiled_copy_calibration_runs(dtype):
+    """The sweep-traversal copy used to calibrate the roofline runs for every
+    tile configuration, ragged grid edges included."""
+    from wakis_amd import _lib
+
+    print(f"[case] tiled copy calibration {np.dtype(dtype).name}", flush=True)
+    for cfg in range(6):
+        ms = _lib.copy_bench_tiled(70, 37, 45, cfg, dtype=dtype, reps=2)
+        assert ms > 0

@@ -73,6 +73,7 @@ SIGNATURES = {
     "fit_step": [_P, _I32, _I32],
     "fit_step_timed": [_P, _I32, _I32, _I32, _PD],
     "fit_copy_bench": [_I32, _I64, _I32, _PD],
+    "fit_copy_bench_tiled": [_I32, _I32, _I32, _I32, _I32, _I32, _I32, _PD],
     "fit_set_step_index": [_P, _I64],
     "fit_get_step_index": [_P, C.POINTER(_I64)],
     "fit_set_kernel": [_P, _I32],
@@ -142,6 +143,17 @@ def copy_bench(nbytes: int, reps: int = 20, device: int = 0) -> float:
     """Average ms of a plain nbytes device copy (reads + writes nbytes each)."""
     out = np.zeros(1, dtype=np.float64)
     check(load().fit_copy_bench(int(device), int(nbytes), int(reps), dptr(out)), "fit_copy_bench")
+    return float(out[0])
+
+
+def copy_bench_tiled(nx: int, ny: int, nz: int, cfg: int, dtype=np.float32, reps: int = 10,
+                     device: int = 0) -> float:
+    """Average ms of copying six (nx, ny, nz) arrays into six others with the
+    traversal of fused tile configuration cfg."""
+    out = np.zeros(1, dtype=np.float64)
+    dt = FIT_F64 if np.dtype(dtype) == np.float64 else FIT_F32
+    check(load().fit_copy_bench_tiled(int(device), int(nx), int(ny), int(nz), dt, int(cfg),
+                                      int(reps), dptr(out)), "fit_copy_bench_tiled")
     return float(out[0])
 
 

@@ -799,6 +799,82 @@ int fit_step_timed(fit_ctx *c, int32_t nsteps, int32_t conductive, int32_t per_k
   return rc;
 }
 
+extern "C++" {
+template <typename T>
+static int copy_tiled_impl(int nx, int ny, int nz, int cfg, int tx, int ty, int kc, int reps,
+                           double *ms, int device) {
+  const int64_t N = (int64_t)nx * ny * nz;
+  const int64_t blocks = (int64_t)((nx + tx - 1) / tx) * ((ny + ty - 1) / ty) * ((nz + kc - 1) / kc);
+  const dim3 grid((unsigned)blocks), block((unsigned)(tx * ty));
+  // unused dynamic LDS that leaves room for as many blocks per CU as the
+  // sweep itself keeps resident, so that both run the same rounds of tiles
+  const int occ = fit::fused_blocks_per_cu<T>(cfg);
+  int lds_cu = 0;
+  if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                            device) != hipSuccess)
+    lds_cu = 0;
+  const size_t dyn = (occ > 0 && lds_cu > 2048) ? ((size_t)(lds_cu / occ - 1024) & ~(size_t)255) : 0;
+  fit::Copy6<T> p{};
+  std::vector<T *> bufs;
+  int rc = 0;
+  for (int c = 0; c < 12 && !rc; ++c) {
+    T *q = nullptr;
+    if (hipMalloc(&q, N * sizeof(T)) != hipSuccess) {
+      rc = fail(FIT_ENOMEM, "tiled copy benchmark buffers");
+      break;
+    }
+    bufs.push_back(q);
+  }
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+    rc = fail(FIT_EHIP, "stream");
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(FIT_EHIP, "events");
+  if (!rc) {
+    for (int c = 0; c < 6; ++c) {
+      p.a[c] = bufs[c];
+      p.b[c] = bufs[6 + c];
+      (void)hipMemsetAsync(bufs[c], 0, N * sizeof(T), st);
+    }
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(fit::copy_zmarch<T>, grid, block, dyn, st, p, nx, ny, nz, tx, ty, kc);
+    if (hipGetLastError() != hipSuccess) rc = fail(FIT_EHIP, "tiled copy launch");
+  }
+  if (!rc) {
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(fit::copy_zmarch<T>, grid, block, dyn, st, p, nx, ny, nz, tx, ty, kc);
+    (void)hipEventRecord(e1, st);
+    float t = 0;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+      rc = fail(FIT_EHIP, "tiled copy benchmark timing");
+    *ms = t / reps;
+  }
+  if (st) (void)hipStreamSynchronize(st);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  for (T *q : bufs) (void)hipFree(q);
+  return rc;
+}
+}  // extern "C++"
+
+int fit_copy_bench_tiled(int32_t device, int32_t nx, int32_t ny, int32_t nz, int32_t dtype,
+                         int32_t cfg, int32_t reps, double *ms) {
+  if (!ms || nx < 1 || ny < 1 || nz < 1 || reps < 1 || cfg < 0 || cfg >= fit::FUSED_NCFG ||
+      (dtype != FIT_F32 && dtype != FIT_F64))
+    return fail(FIT_EINVAL, "bad tiled copy benchmark arguments");
+  if ((int64_t)nx * ny * nz >= ((int64_t)1 << 31))
+    return fail(FIT_EINVAL, "tiled copy benchmark grid too large");
+  int tx, ty, kc;
+  fit::fused_dims(cfg, tx, ty, kc);
+  if (tx * ty > 1024) return fail(FIT_EINVAL, "tile too large");
+  HIPCHK(hipSetDevice(device));
+  return dtype == FIT_F64 ? copy_tiled_impl<double>(nx, ny, nz, cfg, tx, ty, kc, reps, ms, device)
+                          : copy_tiled_impl<float>(nx, ny, nz, cfg, tx, ty, kc, reps, ms, device);
+}
+
 int fit_copy_bench(int32_t device, int64_t nbytes, int32_t reps, double *ms) {
   if (!ms || nbytes < 16 || reps < 1) return fail(FIT_EINVAL, "bad copy benchmark arguments");
   HIPCHK(hipSetDevice(device));

@@ -1112,6 +1112,33 @@ inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
   }
 }
 
+// Resident blocks per CU of the single-grid sweep of a tile configuration
+// (0 if the configuration does not fit this dtype).
+template <typename T, int TX, int TY, int KC>
+inline int occ_cfg() {
+  if constexpr (!cfg_fits<T, TX, TY>()) {
+    return 0;
+  } else {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fused_sweep<T, TX, TY, KC, false>,
+                                                     TX * TY, fused_mt_bytes<T>(1)) != hipSuccess)
+      return 0;
+    return n;
+  }
+}
+
+template <typename T>
+inline int fused_blocks_per_cu(int cfg) {
+  switch (cfg) {
+    case FUSED_CFG_128x4: return occ_cfg<T, 128, 4, 32>();
+    case FUSED_CFG_64x8: return occ_cfg<T, 64, 8, 32>();
+    case FUSED_CFG_256x4: return occ_cfg<T, 256, 4, 32>();
+    case FUSED_CFG_64x4: return occ_cfg<T, 64, 4, 32>();
+    case FUSED_CFG_256x4_K64: return occ_cfg<T, 256, 4, 64>();
+    default: return occ_cfg<T, 256, 2, 32>();
+  }
+}
+
 template <typename T>
 inline void launch_fused(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s, int cfg) {
   switch (cfg) {

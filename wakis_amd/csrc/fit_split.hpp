@@ -154,4 +154,35 @@ __global__ void __launch_bounds__(256) stream_copy(const float4 *__restrict__ sr
   if (t < n4) dst[t] = src[t];
 }
 
+// Copy of six arrays into six others (the twelve field streams of a step)
+// with the fused sweep's traversal: a block per (tx x ty columns, kc planes)
+// tile, marching in z.  Calibrates what a plain copy reaches with the same
+// access pattern as the sweep.
+template <typename T>
+struct Copy6 {
+  const T *a[6];
+  T *b[6];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(1024) copy_zmarch(Copy6<T> p, int nx, int ny, int nz, int tx,
+                                                    int ty, int kc) {
+  const int ntx = (nx + tx - 1) / tx, nty = (ny + ty - 1) / ty;
+  const int t = blockIdx.x;
+  const int bx = t % ntx, r = t / ntx, by = r % nty, bz = r / nty;
+  const int i = bx * tx + (int)threadIdx.x % tx, j = by * ty + (int)threadIdx.x / tx;
+  if (i >= nx || j >= ny) return;
+  const int64_t plane = (int64_t)nx * ny;
+  const int64_t col = i + (int64_t)nx * j;
+  const int k1 = min(nz, (bz + 1) * kc);
+  for (int k = bz * kc; k < k1; ++k) {
+    const int64_t n = col + plane * k;
+    T v[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) v[c] = p.a[c][n];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) p.b[c][n] = v[c];
+  }
+}
+
 }  // namespace fit
