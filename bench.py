@@ -346,6 +346,70 @@ def bench_plain(args):
     return {"plain": rows}
 
 
+def bench_cubic(args):
+    """The reference's own CPU case: the PEC cubic cavity (64 x 64 x 192,
+    FP64, CFL 1.0, wakelength 3 m) solved in full on the device (wakesolve:
+    steps, WP and Z, longitudinal and transverse), against the CPU oracle on
+    the same grid (steps timed on a bounded sample, scaled to the run)."""
+    from wakis_amd.workloads import build, pec_cubic_cavity
+
+    wl = pec_cubic_cavity()
+    grid, wake, solver = build(wl, dtype=np.float64)
+    solver.wakesolve(compute_plane="both")  # warm-up run (builds, tunes, caches)
+    grid, wake, solver = build(wl, dtype=np.float64)
+    t0 = time.perf_counter()
+    solver.wakesolve(compute_plane="both")
+    wall = time.perf_counter() - t0
+    step_s = solver.simulationTime
+    N, Nt = solver.N, solver.Nt
+    print(f"[bench] cubic cavity FP64 on the device: Nt {Nt}, wakesolve {wall:.3f} s "
+          f"(stepping {step_s:.3f} s)", file=sys.stderr, flush=True)
+    out = {
+        "metric": "Gcell-steps/s",
+        "value": round(N * Nt / step_s / 1e9, 3),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": int(Nt),
+        "warmup": 0,
+        "ms_per_step": round(step_s / Nt * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"PEC cubic cavity {wl['name']} ({N} cells), CFL 1.0, q 1 nC, "
+                        "sigma_z 25 mm, wakelength 3 m, full wakesolve (both planes)",
+            "wakesolve_s": round(wall, 4),
+            "stepping_s": round(step_s, 4),
+        },
+    }
+    if not args.no_cpu:
+        from oracle.fit_oracle import OGrid, OSolver, OWake
+
+        g, sv = wl["grid"], wl["solver"]
+        osv = OSolver(OGrid(x=g["x"], y=g["y"], z=g["z"]), OWake(**wl["wake"]), cfln=sv["cfln"],
+                      bc_low=sv["bc_low"], bc_high=sv["bc_high"], solids=g["solids"],
+                      bg=sv["bg"], dtype=np.float64)
+        osv.one_step()
+        steps = 200
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            osv.one_step()
+        cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": N * steps / cpu / 1e9,
+            "unit": "Gcell-steps/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"oracle FP64 scipy.sparse one_step on the full grid, {steps} of the "
+                      f"{Nt} steps, {cpu:.2f} s on 1 core of {cpu_model()}; the full stepping "
+                      f"would take {cpu / steps * Nt:.0f} s",
+        }
+        out["config"]["speedup_vs_cpu_stepping"] = round(cpu / steps * Nt / step_s, 1)
+    return out
+
+
 def bench_batch(args):
     """64 uneven rectangular PEC cavities (box 60-140 mm, pipe 10-25 mm,
     length 60-160 mm, dx = 1.5 mm, z-end PML, beta in {0.8, 0.9, 1.0}) in
@@ -730,10 +794,11 @@ def main():
     ap.add_argument("--tile", type=int, default=None, help="fused tile configuration id")
     ap.add_argument("--wakelength", type=float, default=None, help="precision: wakelength (m)")
     ap.add_argument("--workload", default="pillbox",
-                    choices=["pillbox", "batch", "serve", "precision", "large", "plain"],
+                    choices=["pillbox", "batch", "serve", "precision", "large", "plain", "cubic"],
                     help="pillbox: one 256x256x768 solve (the headline); batch: 64 cavities in "
                          "one launch per step; serve: Poisson impedance requests; large: "
-                         "the 1024^3 stratified box, FP32 vs FP64")
+                         "the 1024^3 stratified box, FP32 vs FP64; cubic: the PEC cubic "
+                         "cavity 64x64x192 FP64 wakesolve against the CPU oracle")
     ap.add_argument("--ncav", type=int, default=64, help="batch: number of cavities")
     ap.add_argument("--nreq", type=int, default=24, help="serve: number of requests")
     ap.add_argument("--chunk", type=int, default=256, help="serve: steps per batch call")
@@ -767,6 +832,9 @@ def main():
         return
     if args.workload == "plain":
         print(json.dumps(bench_plain(args)), flush=True)
+        return
+    if args.workload == "cubic":
+        print(json.dumps(bench_cubic(args)), flush=True)
         return
 
     from wakis_amd.workloads import build, lossy_pillbox

@@ -218,3 +218,27 @@ def test_smart_mesh_parity_fp64(tmp_path):
             "WPx": rel(psv.wake.WPx, osv.wake.WPx)}
     print(errs)
     assert max(errs.values()) <= 1e-10, errs
+
+
+def test_config1_pec_cubic_cavity_fp64():
+    """The reference's own CPU case in full on the device, FP64: the PEC cubic
+    cavity (64 x 64 x 192, CFL 1.0, wakelength 3 m, both planes) against the
+    oracle's full run on the same inputs (tests/golden/make_config1_golden.py
+    wrote the fixture)."""
+    from wakis_amd.workloads import build, pec_cubic_cavity
+
+    g = np.load(os.path.join(HERE, "golden", "config1_pec_cubic_cavity_fp64.npz"))
+    print("[case] config 1 cubic cavity fp64 full wakesolve", flush=True)
+    _, wake, solver = build(pec_cubic_cavity(), dtype=np.float64)
+    solver.wakesolve(compute_plane="both")
+    assert solver.Nt == int(g["Nt"]) and float(solver.dt) == float(g["dt"])
+    assert len(wake.s) == len(g["s"]) and len(wake.Z) == len(g["Z"])
+    errs = {k: rel(getattr(wake, k), g[k]) for k in ("s", "WP", "WPx", "WPy", "f", "Z", "Zx", "Zy")}
+    stride = int(g["sample"])
+    E, H = solver.E.toarray(), solver.H.toarray()
+    errs["E_sample"] = rel(E[::stride], g["E_sample"])
+    errs["H_sample"] = rel(H[::stride], g["H_sample"])
+    errs["E_norm"] = abs(np.linalg.norm(E) / float(g["E_norm"]) - 1)
+    errs["H_norm"] = abs(np.linalg.norm(H) / float(g["H_norm"]) - 1)
+    print(errs)
+    assert max(errs.values()) <= 1e-10, errs
