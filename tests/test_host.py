@@ -245,3 +245,41 @@ def test_result_files_roundtrip(tmp_path):
     assert np.array_equal(w2.Z, cols["Z"]) and np.array_equal(w2.f, f)
     assert np.array_equal(w2.lambdaf, cols["spectrum"]) and np.array_equal(w2.WPy, cols["WPy"])
     assert w2.wakelength == s[-1] and w2.Ez_file.endswith("Ez.h5")
+
+
+def test_wake_impedance_transforms_sine():
+    """tests/test_004_wakes_and_impedances.py:11-52: a sine wake of amplitude
+    A at f_r gives |Z| peaking at A at f_r, and wake -> Z -> wake round trips
+    keep the wake within 1 (twice)."""
+    from wakis_amd import WakeSolver as wk
+
+    fr, A = 0.5e9, 100
+    t = np.linspace(0, 100 * 1e-9, 3000)
+    wake = A * np.sin(2 * np.pi * fr * t)
+    f, Z = wk.calc_impedance_from_wake([t, wake], verbose=False)
+    tt, wwake = wk.calc_wake_from_impedance([f, Z], samples=3000, verbose=False)
+    ff, Zz = wk.calc_impedance_from_wake([tt, wwake], verbose=False)
+    ttt, wwwake = wk.calc_wake_from_impedance([ff, Zz], samples=3000, verbose=False)
+    assert np.allclose(wake, wwake, atol=1)
+    assert np.allclose(wake, wwwake, atol=1)
+    assert np.allclose(np.max(np.abs(Z)), A, atol=1)
+    assert np.allclose(np.max(np.abs(Zz)), A, atol=1)
+    assert np.allclose(f[np.argmax(Z)], fr, atol=1e6)
+    assert np.allclose(ff[np.argmax(Zz)], fr, atol=1e6)
+    # the transform pair by its definition: a DFT bin of the wake, and a
+    # length-preserving inverse of a one-sided spectrum
+    n = len(wake)
+    k = 17
+    ref = 2 / n * np.sum(wake * np.exp(-2j * np.pi * k * np.arange(n) / n))
+    assert abs(Z[k] - ref) <= 1e-9 * abs(ref) + 1e-9
+    assert len(f) == (n + 1) // 2 and np.all(f >= 0) and np.all(np.diff(f) > 0)
+    assert len(tt) == 3000 and tt[0] == 0.0 and tt[-1] == pytest.approx(1 / np.mean(np.diff(f)))
+    # s in metres instead of t; fmax resampling; errors without a time axis
+    f2, Z2 = wk.calc_impedance_from_wake(wake, s=t * 299792458.0, verbose=False)
+    assert np.allclose(f2, f, rtol=1e-9) and np.allclose(Z2, Z)
+    f3, _ = wk.calc_impedance_from_wake([t, wake], fmax=2e9, verbose=False)
+    assert f3.max() < 2e9
+    with pytest.raises(AttributeError):
+        wk.calc_impedance_from_wake(wake, verbose=False)
+    with pytest.raises(AttributeError):
+        wk.calc_wake_from_impedance(Z, verbose=False)

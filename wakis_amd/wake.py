@@ -290,6 +290,69 @@ class WakeSolver:
             np.savetxt(f_name + ".txt", np.c_[x_data, y_data],
                        header="   " + x_name + " " * 20 + y_name + "\n" + "-" * 48)
 
+    # -- stand-alone transforms (wakis/wakeSolver.py:1185-1315) -------------
+    @staticmethod
+    def calc_impedance_from_wake(wake, s=None, t=None, fmax=None, samples=None, verbose=True):
+        """Impedance of a sampled wake function by FFT: ``[f, Z]`` for
+        0 <= f < fmax (default 1/dt), Z = FFT(wake) * 2 / len(wake).
+        ``wake`` may be ``[t, wake]``; ``s`` [m] gives t = s/c.  With
+        ``fmax`` the wake is first resampled (linear interpolation) at
+        1/(2 fmax); ``samples`` zero-pads the FFT to 2*samples points."""
+        if type(wake) is list:
+            t, wake = wake[0], wake[1]
+        if s is not None:
+            t = s / c_light
+        elif t is None:
+            raise AttributeError('Provide time data through parameter "t" [s] or "s" [m]')
+        t = np.asarray(t)
+        wake = np.asarray(wake)
+        dt = np.mean(np.diff(t))
+        if fmax is not None:
+            ts = np.arange(t.min(), t.max(), 1 / fmax / 2)
+            wake = np.interp(ts, t, wake)
+            dt = np.mean(np.diff(ts))
+        else:
+            fmax = 1 / dt
+        W = np.fft.fft(wake, n=None if samples is None else 2 * samples)
+        freqs = np.fft.fftfreq(len(W), dt)
+        keep = (freqs >= 0) & (freqs < fmax)
+        f = freqs[keep]
+        Z = W[keep] / len(wake) * 2
+        if verbose:
+            print(f"    * Number of samples = {len(f)}")
+            print(f"    * Maximum frequency = {f.max()} Hz")
+            print(f"    * Maximum resolution = {np.mean(np.diff(f))} Hz")
+        return [f, Z]
+
+    @staticmethod
+    def calc_wake_from_impedance(impedance, f=None, tmax=None, samples=None, pad=0,
+                                 verbose=True):
+        """Wake function of a sampled impedance: ``[t, wake]`` with
+        wake = Re(-FFT(Z, n=samples)) rotated left by one sample and t
+        spanning [0, tmax] (default 1/df).  ``impedance`` may be ``[f, Z]``
+        (any length-2 sequence is taken as that pair); with ``tmax`` Z is
+        first resampled at 1/tmax on f's range."""
+        if len(impedance) == 2:
+            f, Z = impedance[0], impedance[1]
+        elif f is None:
+            raise AttributeError('Provide frequency data through parameter "f"')
+        else:
+            Z = impedance
+        f = np.asarray(f)
+        Z = np.asarray(Z)
+        df = np.mean(np.diff(f))
+        if tmax is not None:
+            Z = np.interp(np.arange(f.min(), f.max(), 1 / tmax), f, Z)
+        else:
+            tmax = 1 / df
+        wake = np.roll(np.real(-1 * np.fft.fft(Z, n=samples)), -1)
+        t = np.linspace(0, tmax, len(wake))
+        if verbose:
+            print(f"    * Number of samples = {len(t)}")
+            print(f"    * Maximum time = {t.max()} s")
+            print(f"    * Maximum resolution = {np.mean(np.diff(t))} s")
+        return [t, wake]
+
     def read_txt(self, txt, skiprows=2, delimiter=None, usecols=None):
         """Columns of a text file keyed by the header names "name[unit]"
         (integer keys when the header does not parse), complex when a float
