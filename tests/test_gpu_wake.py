@@ -242,3 +242,48 @@ def test_config1_pec_cubic_cavity_fp64():
     errs["H_norm"] = abs(np.linalg.norm(H) / float(g["H_norm"]) - 1)
     print(errs)
     assert max(errs.values()) <= 1e-10, errs
+
+
+def _sphere_case(dtype):
+    """tests/test_003_planewave_sphere.py:19-95: a dielectric sphere (eps_r
+    10, the reference's tests/stl/003_sphere.stl scaled by 1e-3) on 60 x 60 x
+    120 cells, periodic x/y, PEC z low, PML z high, a plane wave from z = 1."""
+    import wakis_amd as wa
+    from wakis_amd.geometry import read_stl, transform_triangles
+
+    stl = os.path.join(HERE, "golden", "003_sphere.stl")
+    tri = transform_triangles(read_stl(stl), (0, 0, 0), (0, 0, 0), 1e-3).reshape(-1, 3)
+    lo, hi = tri.min(0), tri.max(0)
+    pad = (hi - lo) * np.array([0.2, 0.2, 1.0])
+    lo, hi = lo - pad, hi + pad
+    Nx, Ny, Nz = 60, 60, 120
+    grid = wa.GridFIT3D(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], Nx, Ny, Nz,
+                        stl_solids={"Sphere": stl}, stl_rotate=[0, 0, 0], stl_scale=1e-3,
+                        stl_materials={"Sphere": [10.0, 1.0]})
+    bl, bh = ["periodic", "periodic", "pec"], ["periodic", "periodic", "pml"]
+    psv = wa.SolverFIT3D(grid, use_stl=True, bc_low=bl, bc_high=bh, dtype=dtype)
+    mask = grid.solids[0][1]
+    osv = OSolver(OGrid(x=grid.x, y=grid.y, z=grid.z), bc_low=bl, bc_high=bh,
+                  solids=[(mask, [10.0, 1.0])], dtype=dtype)
+    return grid, psv, osv
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["fp64", "fp32"])
+def test_planewave_sphere_emsolve(dtype):
+    """The reference's test_003 simulation on the device against the oracle
+    (fields after the whole emsolve run)."""
+    import wakis_amd as wa
+    from helpers import OPlaneWave
+
+    grid, psv, osv = _sphere_case(dtype)
+    Nx, Ny = grid.Nx, grid.Ny
+    assert 0 < grid.solids[0][1].sum() < grid.solids[0][1].size
+    f = 15 / ((psv.z.max() - psv.z.min()) / 299792458.0)
+    Nt = int(1.0 * (psv.z.max() - psv.z.min()) / 299792458.0 / psv.dt)
+    print(f"[case] planewave sphere {np.dtype(dtype).name} Nt={Nt}", flush=True)
+    assert float(psv.dt) == float(osv.dt)
+    osv.emsolve(Nt, OPlaneWave(xs=slice(1, Nx - 1), ys=slice(1, Ny - 1), zs=1, f=f, beta=1.0))
+    psv.emsolve(Nt, wa.PlaneWave(xs=slice(1, Nx - 1), ys=slice(1, Ny - 1), zs=1, f=f, beta=1.0))
+    errs = {n: rel(getattr(psv, n).toarray(), getattr(osv, n).toarray()) for n in "EHJ"}
+    print(errs)
+    assert max(errs.values()) <= (1e-10 if dtype == np.float64 else 1e-4), errs
