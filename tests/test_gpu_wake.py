@@ -287,3 +287,38 @@ def test_planewave_sphere_emsolve(dtype):
     errs = {n: rel(getattr(psv, n).toarray(), getattr(osv, n).toarray()) for n in "EHJ"}
     print(errs)
     assert max(errs.values()) <= (1e-10 if dtype == np.float64 else 1e-4), errs
+
+
+def test_te101_resonator_device():
+    """tests/test_002_analytic_resonator.py:83-145 on the device: the TE101
+    mode of a 1 m PEC cube on 30^3 cells (FP64), E_y at the centre read
+    after every step for 5000 steps, rings at c/sqrt(2) = 2.121e8 Hz within
+    5 %; the same series as the oracle's within 1e-10."""
+    import wakis_amd as wa
+    from test_oracle import te101_initial_h
+
+    L, N, steps = 1.0, 30, 5000
+    d = L / N
+    lo, hi = -L / 2 + d / 2, L / 2 + d / 2
+    print("[case] TE101 resonator on the device", flush=True)
+    psv = wa.SolverFIT3D(wa.GridFIT3D(lo, hi, lo, hi, lo, hi, N, N, N),
+                         bc_low=["pec"] * 3, bc_high=["pec"] * 3)
+    osv = OSolver(OGrid(lo, hi, lo, hi, lo, hi, N, N, N), bc_low=["pec"] * 3,
+                  bc_high=["pec"] * 3)
+    hx, hy, hz = te101_initial_h(N, L)
+    for s in (psv, osv):
+        s.H.from_matrix(hx, "x")
+        s.H.from_matrix(hy, "y")
+        s.H.from_matrix(hz, "z")
+    ep, eo = [], []
+    for _ in range(steps):
+        ep.append(psv.E[N // 2, N // 2, N // 2, "y"])
+        eo.append(osv.E[N // 2, N // 2, N // 2, "y"])
+        psv.one_step()
+        osv.one_step()
+    ep, eo = np.array(ep, dtype=np.float64), np.array(eo, dtype=np.float64)
+    assert rel(ep, eo) <= 1e-10
+    F = np.abs(np.fft.fft(ep))
+    f = np.fft.fftfreq(len(ep), d=float(psv.dt))
+    half = f >= 0
+    assert f[half][np.argmax(F[half])] == pytest.approx(2.121e8, rel=0.05)
