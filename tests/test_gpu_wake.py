@@ -322,3 +322,36 @@ def test_te101_resonator_device():
     f = np.fft.fftfreq(len(ep), d=float(psv.dt))
     half = f >= 0
     assert f[half][np.argmax(F[half])] == pytest.approx(2.121e8, rel=0.05)
+
+
+@pytest.mark.parametrize("eps_r,sigma", [(1.0, 0.0), (3.0, 1.0), (1.0, 1.0)],
+                         ids=["vacuum", "resistive", "high-resistivity"])
+def test_pml_reflection_device(eps_r, sigma):
+    """tests/test_005_pml_boundaries.py:19-336 on the device (FP32): a plane
+    wave of amplitude 100 run into a 30-cell z PML through a background of
+    (eps_r, sigma); after the round trip the field left on the axis is below
+    10 % of the amplitude, for H and for E / (mu_0 c)."""
+    import wakis_amd as wa
+    from scipy.constants import c, mu_0
+
+    Nx, Ny, Nz = 20, 20, 200
+    print(f"[case] PML reflection eps_r={eps_r} sigma={sigma}", flush=True)
+    grid = wa.GridFIT3D(-1.0, 1.0, -1.0, 1.0, 0.0, 1.0, Nx, Ny, Nz)
+    solver = wa.SolverFIT3D(grid, use_stl=False, bg=[eps_r, 1.0, sigma],
+                            bc_low=["periodic", "periodic", "pec"],
+                            bc_high=["periodic", "periodic", "pml"], n_pml=30, dtype=np.float32)
+    amplitude = 100.0
+    f = 15 / ((solver.z.max() - solver.z.min()) / c)
+    pw = wa.PlaneWave(xs=slice(0, Nx), ys=slice(0, Ny), zs=0, beta=1.0, amplitude=amplitude,
+                      f=f, nodes=7, phase=np.pi / 2)
+    solver.dt = 1 / f / 200
+    z_pml = sum(solver.dz[Nz - 1 - k] for k in range(solver.n_pml))
+    Nt = int(eps_r * 2.0 * (1.0 - 0.0 - z_pml) / c / solver.dt)
+    for n in range(Nt):
+        pw.update(solver, n * solver.dt)
+        solver.one_step()
+    refl_h = solver.H.get_abs()[Nx // 2, Ny // 2, :].max()
+    refl_e = solver.E.get_abs()[Nx // 2, Ny // 2, :].max() / (mu_0 * c)
+    print({"Nt": Nt, "H": float(refl_h), "E": float(refl_e)})
+    assert np.isfinite(refl_h) and np.isfinite(refl_e)
+    assert refl_h <= 10 and refl_e <= 10
