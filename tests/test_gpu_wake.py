@@ -355,3 +355,44 @@ def test_pml_reflection_device(eps_r, sigma):
     print({"Nt": Nt, "H": float(refl_h), "E": float(refl_e)})
     assert np.isfinite(refl_h) and np.isfinite(refl_e)
     assert refl_h <= 10 and refl_e <= 10
+
+
+def test_run_log_keys_like_reference(tmp_path):
+    """tests/test_007_mpi_lossy_cavity.py:88-136, 453-536: after a wakesolve
+    the grid, solver and wake logs hold exactly the reference's keys (a PEC
+    background, PEC faces, STL solids) and wakis.log is in the results
+    folder with the three sections."""
+    import json as js
+
+    import wakis_amd as wa
+
+    geo = json.load(open(os.path.join(HERE, "golden", "001_cubic_cavity_box.json")))
+    stl = str(tmp_path / "cavity.stl")
+    write_ascii_stl(stl, geo["triangles"])
+    lo, hi = geo["bounds_lo"], geo["bounds_hi"]
+    print("[case] run log keys", flush=True)
+    grid = wa.GridFIT3D(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], 16, 16, 40,
+                        stl_solids={"cavity": stl}, stl_materials={"cavity": [1.0, 1.0]})
+    folder = str(tmp_path / "007_results") + "/"
+    wake = wa.WakeSolver(q=1e-9, sigmaz=0.1, beta=1.0, xsource=0.0, ysource=0.0, xtest=0.0,
+                         ytest=0.0, skip_cells=0, results_folder=folder, save=False)
+    solver = wa.SolverFIT3D(grid, wake, bc_low=["pec"] * 3, bc_high=["pec"] * 3, use_stl=True,
+                            bg="pec", dtype=np.float32)
+    solver.wakesolve(wakelength=0.2, compute_plane="longitudinal")
+    assert set(solver.logger.grid) == {
+        "use_mesh_refinement", "Nx", "Ny", "Nz", "dx", "dy", "dz", "xmin", "xmax", "ymin",
+        "ymax", "zmin", "zmax", "stl_solids", "stl_materials", "gridInitializationTime"}
+    assert set(solver.logger.solver) == {
+        "use_gpu", "use_mpi", "background", "bc_low", "bc_high", "dt", "solverInitializationTime"}
+    assert set(solver.logger.wakeSolver) == {
+        "ti", "q", "sigmaz", "beta", "xsource", "ysource", "xtest", "ytest", "chargedist",
+        "skip_cells", "results_folder", "wakelength", "simulationTime"}
+    assert solver.logger.wakeSolver["results_folder"] == folder
+    assert solver.logger.wakeSolver["wakelength"] == 0.2
+    assert solver.logger.solver["dt"] == solver.dt and solver.logger.grid["Nz"] == 40
+    text = open(os.path.join(folder, "wakis.log"), encoding="utf-8").read()
+    parts = text.split("\n## ")
+    assert [p.split(" ##")[0] for p in parts[1:]] == ["WakeSolver Logs", "Solver Logs",
+                                                      "Grid Logs"]
+    grid_log = js.loads(parts[3].split("\n", 1)[1])
+    assert grid_log["stl_materials"] == {"cavity": [1.0, 1.0]} and grid_log["Nx"] == 16
