@@ -413,7 +413,7 @@ def bench_cubic(args):
 def bench_batch(args):
     """64 uneven rectangular PEC cavities (box 60-140 mm, pipe 10-25 mm,
     length 60-160 mm, dx = 1.5 mm, z-end PML, beta in {0.8, 0.9, 1.0}) in
-    FP32, stepped by one persistent launch per step inside hipGraph launch
+    FP32, stepped by one launch per step (a block per tile) inside hipGraph launch
     groups (the wakesolve inner loop of every member: device beam + E_z
     probe + step), against the same cavities stepped one context at a time."""
     import wakis_amd as wa
@@ -474,7 +474,7 @@ def bench_batch(args):
         "data": "synthetic",
         "config": {
             "workload": f"batch of {len(solvers)} rectangular PEC cavities (seed 3), "
-                        f"{sizes[0]}..{sizes[-1]} cells each, {cells} cells in all; one persistent "
+                        f"{sizes[0]}..{sizes[-1]} cells each, {cells} cells in all; one "
                         "launch per step in hipGraph groups of 16; wakesolve inner loop",
             "tile": cfg, "blocks": nblk, "tiles": ntiles,
             "one_at_a_time_gcells": round(seq_value, 3),

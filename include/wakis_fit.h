@@ -244,7 +244,7 @@ int fit_impedance(fit_ctx *ctx, const double *x, const double *lam,
 /* ---------------- batches (no counterpart in the reference) ----------------
  * The reference steps one solver at a time (wakis/routines.py:231-485).  A
  * batch steps many independent contexts (same device and dtype, fused
- * sweep) with ONE persistent launch per step; each member's arithmetic is
+ * sweep) with ONE launch per step (a block per tile of every member); each member's arithmetic is
  * that of its own fit_step with the same tile configuration, so results are
  * the same alone and in a batch.  Members keep their own state (fields,
  * beam, wake probe); after fit_batch_step every member is as if it had run

@@ -382,7 +382,7 @@ class Context:
 
 
 class Batch:
-    """Many contexts stepped by one persistent launch per step (see
+    """Many contexts stepped by one launch per step (see
     ``fit_batch_*`` in include/wakis_fit.h)."""
 
     def __init__(self, contexts, conductive=None):

@@ -2,7 +2,7 @@
 
 The reference runs one ``SolverFIT3D`` at a time (wakis/routines.py:231-485).
 Here many device solvers (same dtype, same device) are stepped together by
-one persistent HIP launch per step (``fit_batch_*`` in include/wakis_fit.h):
+one HIP launch per step (``fit_batch_*`` in include/wakis_fit.h):
 each member runs the same per-tile arithmetic as in its own ``one_step``,
 so its fields, wake potential and impedance are the same alone and in the
 batch (with the same tile configuration).  Launch groups of ``group`` steps

@@ -957,9 +957,11 @@ struct BatchMember {
   const T *btab;
 };
 
-// One step of every member grid in one persistent launch: block b takes the
-// batch's tiles b, b + gridDim.x, ...; tstart[m] is the first tile of member
-// m (ascending, tstart[nmem] = ntiles).  Step = *dstep + g of the batch.
+// One step of every member grid in one launch of one block per tile: block
+// t takes tile t of the batch, of the member m with tstart[m] <= t <
+// tstart[m+1] (tstart ascending, tstart[nmem] = ntiles).  Step = *dstep + g
+// of the batch.  (A persistent grid walking the tiles keeps the walk's state
+// live across the tile body and made the compiler spill it to scratch.)
 template <typename T, int TX, int TY, int KC>
 __global__ void __launch_bounds__(TX *TY)
     __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) fused_batch(const BatchMember<T> *__restrict__ mem, const int *__restrict__ tstart, int nmem,
@@ -967,7 +969,8 @@ __global__ void __launch_bounds__(TX *TY)
   __shared__ FusedSmem<T, TX, TY> sm;
   T *smt = (T *)fit_dyn_lds;
   const int64_t s = *dstep + g;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  const int t = blockIdx.x;
+  if (t < ntiles) {
     int lo = 0, hi = nmem - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -979,7 +982,6 @@ __global__ void __launch_bounds__(TX *TY)
     f.a.step = m.step0 + s;
     if (m.btab) f.a.jline = m.btab + (int64_t)g * f.a.nz;
     fused_dispatch<T, TX, TY, KC, false>(f, xcd_tile(t - tstart[lo], f.nb), sm, smt);
-    __syncthreads();  // the next tile reuses the LDS
   }
 }
 
