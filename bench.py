@@ -451,7 +451,7 @@ def bench_batch(args):
     seq_value = cells * seq_steps / seq_s / 1e9
     ms_step = ms / args.steps
     alg = 12 * 4 * cells
-    dims = {0: (256, 2, 32), 1: (128, 4, 32), 2: (64, 8, 32), 3: (256, 4, 32), 4: (64, 4, 32),
+    dims = {0: (320, 3, 64), 1: (128, 4, 32), 2: (64, 8, 32), 3: (256, 4, 32), 4: (64, 4, 32),
             5: (256, 4, 64)}.get(cfg)
     traffic, tsrc = (None, None)
     if dims and len(solvers) == 64:

@@ -185,9 +185,10 @@ def test_fused_multi_tile_fp64(name, bl, bh, tile):
     solids = [(block_mask(shape, (3, 2, 5), (Nx - 4, Ny - 3, Nz - 6)), "vacuum"),
               (block_mask(shape, (40, 5, 20), (90, 12, 40)), [3.0, 1.2, 20.0]),
               (block_mask(shape, (60, 8, 30), (70, 15, 50)), "pec")]
-    # tiles 3 and 5 (256 x 4) do not fit the LDS in FP64: the default tile runs
+    # tiles 0 (320 x 3), 3 and 5 (256 x 4) do not fit the LDS in FP64: the
+    # default tile runs
     osv, psv = make_pair(*shape, bl, bh, solids=solids, seed=21, n_pml=4, kernel="fused",
-                         tile=None if tile in (3, 5) else tile)
+                         tile=None if tile in (0, 3, 5) else tile)
     for _ in range(6):
         osv.one_step()
         psv.one_step()
@@ -238,6 +239,27 @@ def test_fused_inner_chunks_fp32(tile):
               (block_mask(shape, (90, 2, 33), (100, 11, 70)), "pec")]
     osv, psv = make_pair(*shape, ["periodic", "pmc", "pml"], ["pec", "periodic", "pml"],
                          dtype=np.float32, solids=solids, seed=41, kernel="fused", tile=tile)
+    for _ in range(8):
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    print(err)
+    assert max(err.values()) <= TOL32, err
+
+
+@pytest.mark.parametrize("tile", [0, 5])
+def test_fused_inner_chunks_k64_fp32(tile):
+    """The lean path of the 64-plane tiles (320 x 3, 256 x 4) in FP32: 150
+    planes give a chunk at planes 64..127 away from both z faces; a grid
+    narrower than the tile in x (partial rows) and several tiles in y."""
+    shape = (300, 11, 150)
+    print(f"[case] fused inner chunks K64 fp32 tile={tile} shape={shape}", flush=True)
+    Nx, Ny, Nz = shape
+    solids = [(block_mask(shape, (2, 1, 4), (Nx - 3, Ny - 2, Nz - 5)), "vacuum"),
+              (block_mask(shape, (30, 3, 70), (200, 9, 110)), [4.0, 1.3, 30.0]),
+              (block_mask(shape, (220, 2, 60), (260, 10, 120)), "pec")]
+    osv, psv = make_pair(*shape, ["periodic", "pec", "pml"], ["pec", "pmc", "pml"],
+                         dtype=np.float32, solids=solids, seed=47, kernel="fused", tile=tile)
     for _ in range(8):
         osv.one_step()
         psv.one_step()

@@ -1015,7 +1015,7 @@ inline bool fused_supported(int kernel, int nx, int ny, int nz) {
 
 // Tile configurations (TX, TY, KC) selectable per context.
 enum {
-  FUSED_CFG_256x2 = 0,
+  FUSED_CFG_320x3_K64 = 0,
   FUSED_CFG_128x4 = 1,
   FUSED_CFG_64x8 = 2,
   FUSED_CFG_256x4 = 3,
@@ -1031,19 +1031,20 @@ inline void fused_dims(int cfg, int &tx, int &ty, int &kc) {
     case FUSED_CFG_256x4: tx = 256; ty = 4; kc = 32; break;
     case FUSED_CFG_64x4: tx = 64; ty = 4; kc = 32; break;
     case FUSED_CFG_256x4_K64: tx = 256; ty = 4; kc = 64; break;
-    default: tx = 256; ty = 2; kc = 32; break;
+    default: tx = 320; ty = 3; kc = 64; break;
   }
 }
 
 // Cost model of a tile configuration for a grid: cells the tiles cover
 // (partial tiles padded) times the configuration's time per covered cell,
 // measured on the 256 x 256 x 768 FP32 pillbox (ms per step per 5.03e7
-// cells: 256x4 K64 0.603, 256x4 0.611, 64x8 0.637, 128x4 0.640, 64x4 0.752,
-// 256x2 0.995).  Used when no tuning was done.
+// cells: 256x4 K64 0.603, 256x4 0.611, 64x8 0.637, 128x4 0.640, 64x4 0.752;
+// 320x3 K64 on the same pillbox 320 cells wide: 0.752).  Used when no
+// tuning was done.
 inline double fused_cost(int cfg, int nx, int ny, int nz) {
   int tx, ty, kc;
   fused_dims(cfg, tx, ty, kc);
-  static const double w[FUSED_NCFG] = {0.995, 0.640, 0.637, 0.611, 0.752, 0.603};
+  static const double w[FUSED_NCFG] = {0.752, 0.640, 0.637, 0.611, 0.752, 0.603};
   const double cx = (double)((nx + tx - 1) / tx) * tx;
   const double cy = (double)((ny + ty - 1) / ty) * ty;
   const double cz = (double)((nz + kc - 1) / kc) * kc;
@@ -1073,7 +1074,7 @@ inline bool fused_cfg_ok(int cfg) {
     case FUSED_CFG_256x4: return cfg_fits<T, 256, 4>();
     case FUSED_CFG_64x4: return cfg_fits<T, 64, 4>();
     case FUSED_CFG_256x4_K64: return cfg_fits<T, 256, 4>();
-    default: return cfg_fits<T, 256, 2>();
+    default: return cfg_fits<T, 320, 3>();
   }
 }
 
@@ -1143,7 +1144,7 @@ inline int fused_blocks_per_cu(int cfg) {
     case FUSED_CFG_256x4: return occ_cfg<T, 256, 4, 32>();
     case FUSED_CFG_64x4: return occ_cfg<T, 64, 4, 32>();
     case FUSED_CFG_256x4_K64: return occ_cfg<T, 256, 4, 64>();
-    default: return occ_cfg<T, 256, 2, 32>();
+    default: return occ_cfg<T, 320, 3, 64>();
   }
 }
 
@@ -1155,7 +1156,7 @@ inline void launch_fused(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s, int 
     case FUSED_CFG_256x4: launch_cfg<T, 256, 4, 32>(a, En, Hn, s); break;
     case FUSED_CFG_64x4: launch_cfg<T, 64, 4, 32>(a, En, Hn, s); break;
     case FUSED_CFG_256x4_K64: launch_cfg<T, 256, 4, 64>(a, En, Hn, s); break;
-    default: launch_cfg<T, 256, 2, 32>(a, En, Hn, s); break;
+    default: launch_cfg<T, 320, 3, 64>(a, En, Hn, s); break;
   }
 }
 
