@@ -565,10 +565,9 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
   auto gofs = [&](const T *t) __attribute__((always_inline)) -> uint32_t {
     return (uint32_t)((const char *)t - (const char *)g.Lx);
   };
-  const uint32_t otLz = gofs(g.tLz), oiAy = gofs(g.iAy), oitAy = gofs(g.itAy);
+  const uint32_t oLz = gofs(g.Lz), otLz = gofs(g.tLz), oiAy = gofs(g.iAy), oitAy = gofs(g.itAy);
   const uint32_t rowb = (uint32_t)nx * SZ;  // row stride of iA_y, itA_y
   const CT<T> *gtLz = (const CT<T> *)g.tLz;
-  const CT<T> *gLz = (const CT<T> *)g.Lz;
   const CT<T> *giAx = (const CT<T> *)g.iAx;
   const CT<T> *gitAx = (const CT<T> *)g.itAx;
   const CT<T> *jline = (const CT<T> *)a.jline;
@@ -578,10 +577,8 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
   for (int t = threadIdx.x; t < a.nmat * MAT_STRIDE; t += NT) smt[t] = mtab[t];
 
   // ---- window slots: LDS index, byte offsets (per component) from plane
-  //      kk-1, plane offset of the cell (-1, 0, +1: flat-index wraps),
-  //      mask factors ----
-  uint32_t wld[RL], wob[RL][3];
-  int wdk[RL];
+  //      kk-1, mask factors ----
+  uint32_t wld[RL], wob[RL][3], wlz[RL];
   T fx[RL], fy[RL], fz[RL];
   const int ri_max = min(TX + 1, nx - x0 + 1), rj_max = min(TY + 1, ny - y0 + 1);
 #pragma unroll
@@ -594,7 +591,7 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
     wob[q][0] = (uint32_t)(w.i + nx * w.j + plane * (w.dk + 1)) * SZ;
     wob[q][1] = wob[q][0] + NB;
     wob[q][2] = wob[q][0] + 2 * NB;
-    wdk[q] = w.dk;
+    wlz[q] = oLz + (uint32_t)(w.dk + 1) * SZ;
     fx[q] = (need && !xy_mask(pec, 0, w.i, w.j, nx, ny)) ? g.Lx[w.i] : T(0);
     fy[q] = (need && !xy_mask(pec, 1, w.i, w.j, nx, ny)) ? g.Ly[w.j] : T(0);
     fz[q] = (need && !xy_mask(pec, 2, w.i, w.j, nx, ny)) ? T(1) : T(0);
@@ -671,27 +668,24 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
   const T h_iAzm = (hin && !hmb) ? g.iAz[hcolo] : T(0);
 
   auto slot = [](int k) __attribute__((always_inline)) { return (k + 3) % NR; };
-  auto fetch = [&](int kk, T(&v)[RL][3]) __attribute__((always_inline)) {
-    const uint32_t s0 = (uint32_t)(kk - 1) * PB;
+  auto fetch = [&](int kk, T(&v)[RL][4]) __attribute__((always_inline)) {
+    const uint32_t s0 = (uint32_t)(kk - 1) * PB, sz0 = (uint32_t)(kk - 1) * SZ;
 #pragma unroll
     for (int q = 0; q < RL; ++q) {
       v[q][0] = bload<T>(rE, wob[q][0], s0);
       v[q][1] = bload<T>(rE, wob[q][1], s0);
       v[q][2] = bload<T>(rE, wob[q][2], s0);
+      v[q][3] = bload<T>(rG, wlz[q], sz0);
     }
   };
-  // L_z of a window cell is that of plane kk-1, kk or kk+1: three scalar
-  // loads per plane and a select instead of a vector load per cell
-  auto stage = [&](int kk, const T(&v)[RL][3]) __attribute__((always_inline)) {
+  auto stage = [&](int kk, const T(&v)[RL][4]) __attribute__((always_inline)) {
     T *dst = &sm.u[slot(kk)][0][0][0];
-    const T lzm = gLz[kk - 1], lz0 = gLz[kk], lzp = gLz[kk + 1];
 #pragma unroll
     for (int q = 0; q < RL; ++q) {
       if (q < RL - 1 || threadIdx.x + q * NT < RC) {
-        const T lz = wdk[q] < 0 ? lzm : (wdk[q] > 0 ? lzp : lz0);
         dst[wld[q]] = fx[q] * v[q][0];
         dst[RC + wld[q]] = fy[q] * v[q][1];
-        dst[2 * RC + wld[q]] = fz[q] * (lz * v[q][2]);
+        dst[2 * RC + wld[q]] = fz[q] * (v[q][3] * v[q][2]);
       }
     }
   };
@@ -763,7 +757,7 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
   };
 
   // ---- prologue: planes k0-1, k0, k0+1 staged; w of planes k0-1 and k0 ----
-  T pEa[RL][3], pEb[RL][3];
+  T pEa[RL][4], pEb[RL][4];
   fetch(k0 - 1, pEa);
   stage(k0 - 1, pEa);
   fetch(k0, pEa);
@@ -825,7 +819,7 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
 
   // One plane: E_new(k) and H_new(k+1) from `cur`, then the loads of
   // iteration k+2 into `cur` and pE.
-  auto iter = [&](const int k, Pk &cur, T(&pE)[RL][3]) __attribute__((always_inline)) {
+  auto iter = [&](const int k, Pk &cur, T(&pE)[RL][4]) __attribute__((always_inline)) {
     // 1. stage plane k+2 (its loads were issued two iterations ago)
     if (k + 2 <= k1) stage(k + 2, pE);
     const T itAx_k = gitAx[jc + ny * k];       // scalar
