@@ -38,7 +38,34 @@ struct FusedArgs {
   T *En, *Hn;     // new buffers
   int ntx, nty, nkc;
   int nb;
+  int kh = 0, kt = 0, nmid = 0;  // z chunk layout (chunk_span)
+  int nremap = 0;                // blocks under the XCD remap
 };
+
+// Planes [k0, k1) of z chunk c of a tile column.  With kh > 0 the first
+// nmid chunks take KC planes each from plane kh, the last of them ending at
+// nz - kt, chunk nmid is [0, kh) and chunk nmid + 1 is [nz - kt, nz): every
+// plane but the kh + kt at the z faces lies in a chunk the lean path takes,
+// and the blocks of the two short chunks come last (outside the XCD remap,
+// so every XCD gets the same number of full chunks).  With kh = 0 chunk c
+// is [c KC, (c + 1) KC).
+template <int KC, typename T>
+__host__ __device__ inline void chunk_span(const FusedArgs<T> &f, int c, int &k0, int &k1) {
+  const int nz = f.a.nz;
+  if (f.kh == 0) {
+    k0 = c * KC;
+    k1 = k0 + KC < nz ? k0 + KC : nz;
+  } else if (c < f.nmid) {
+    k0 = f.kh + c * KC;
+    k1 = k0 + KC < nz - f.kt ? k0 + KC : nz - f.kt;
+  } else if (c == f.nmid) {
+    k0 = 0;
+    k1 = f.kh;
+  } else {
+    k0 = nz - f.kt;
+    k1 = nz;
+  }
+}
 
 // Boundary mask of component c at true cell coordinates, split into the part
 // fixed by (i, j) and the z faces (last writer wins: z faces are written
@@ -134,10 +161,9 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
   const int tx = tile % f.ntx;
   const int tr = tile / f.ntx;
   const int ty = tr % f.nty;
-  const int kc = tr / f.nty;
   const int x0 = tx * TX, y0 = ty * TY;
-  const int k0 = kc * KC;
-  const int k1 = min(k0 + KC, nz);  // planes k0 .. k1-1
+  int k0, k1;  // planes k0 .. k1-1 of the tile's z chunk
+  chunk_span<KC>(f, tr / f.nty, k0, k1);
 
   // Field and table pointers typed as global memory (address space 1): in
   // the batched launch they are read from memory, and a generic pointer
@@ -551,8 +577,8 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
   const int tr = tile / f.ntx;
   const int ty = tr % f.nty;
   const int x0 = tx * TX, y0 = ty * TY;
-  const int k0 = (tr / f.nty) * KC;
-  const int k1 = min(k0 + KC, nz);
+  int k0, k1;
+  chunk_span<KC>(f, tr / f.nty, k0, k1);
   const uint32_t PB = (uint32_t)plane * SZ;    // bytes per plane
   const uint32_t NB = (uint32_t)a.N * SZ;      // bytes per component
   const int64_t FB = (int64_t)3 * a.N * SZ;    // bytes per field
@@ -892,22 +918,40 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
   }
 }
 
-// Whether tile `tile` of a grid takes the lean path: FP32 (in FP64 its two
-// packets and window buffers would halve the waves a CU holds), every plane
-// its chunk touches (window and halo included) lies in 1 .. nz-2, and a
-// field and the metric tables span less than 2^32 bytes (buffer offsets).
-template <typename T, int TX, int TY, int KC>
-__device__ __forceinline__ bool fused_inner(const FusedArgs<T> &f, int tile) {
-  const int k0 = tile / (f.ntx * f.nty) * KC;
-  return sizeof(T) == 4 && f.a.buf32 && k0 >= 3 && k0 + KC + 4 <= f.a.nz;
+// Host: the chunk layout of the grid (f.kh, f.kt, f.nmid, f.nkc, f.nb and
+// f.nremap, the blocks under the XCD remap).  A 4-plane chunk at each z face
+// when the grid has room for at least one full chunk between them.
+template <int KC, typename T>
+inline void chunk_layout(FusedArgs<T> &f, int nz) {
+  constexpr int kh = 4, kt = 4;
+  const int ncol = f.ntx * f.nty;
+  if (nz - kh - kt >= KC) {
+    f.kh = kh;
+    f.kt = kt;
+    f.nmid = (nz - kh - kt + KC - 1) / KC;
+    f.nkc = f.nmid + 2;
+    f.nremap = ncol * f.nmid;
+  } else {
+    f.kh = 0;
+    f.kt = 0;
+    f.nmid = 0;
+    f.nkc = (nz + KC - 1) / KC;
+    f.nremap = ncol * f.nkc;
+  }
+  f.nb = ncol * f.nkc;
 }
 
-// One tile with the code path for its J mode and z position (NTS:
-// non-temporal field stores).
+// One tile with the code path for its J mode and z position: the lean path
+// when every plane the chunk touches (window and halo included) lies in
+// 1 .. nz-2, in FP32 (in FP64 its two packets and window buffers would halve
+// the waves a CU holds) and when a field and the metric tables span less
+// than 2^32 bytes (buffer offsets).  NTS: non-temporal field stores.
 template <typename T, int TX, int TY, int KC, bool NTS>
 __device__ __forceinline__ void fused_dispatch(const FusedArgs<T> &f, int tile,
                                                FusedSmem<T, TX, TY> &sm, T *smt) {
-  const bool in = fused_inner<T, TX, TY, KC>(f, tile);
+  int k0, k1;
+  chunk_span<KC>(f, tile / (f.ntx * f.nty), k0, k1);
+  const bool in = sizeof(T) == 4 && f.a.buf32 && k0 >= 3 && k1 + 4 <= f.a.nz;
   if constexpr (sizeof(T) == 4) {
     if (in) {
       if (f.a.jmode == J_BUFFER) fused_tile_inner<T, TX, TY, KC, true, NTS>(f, tile, sm, smt);
@@ -935,7 +979,8 @@ __global__ void __launch_bounds__(TX *TY)
     __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) fused_sweep(FusedArgs<T> f) {
   __shared__ FusedSmem<T, TX, TY> sm;
   T *smt = (T *)fit_dyn_lds;
-  fused_dispatch<T, TX, TY, KC, NTS>(f, xcd_tile(blockIdx.x, f.nb), sm, smt);
+  const int b = blockIdx.x;
+  fused_dispatch<T, TX, TY, KC, NTS>(f, b < f.nremap ? xcd_tile(b, f.nremap) : b, sm, smt);
 }
 
 // ---------------- batched launch ----------------
@@ -975,7 +1020,8 @@ __global__ void __launch_bounds__(TX *TY)
     FusedArgs<T> f = m.f[s & 1];
     f.a.step = m.step0 + s;
     if (m.btab) f.a.jline = m.btab + (int64_t)g * f.a.nz;
-    fused_dispatch<T, TX, TY, KC, false>(f, xcd_tile(t - tstart[lo], f.nb), sm, smt);
+    const int b = t - tstart[lo];
+    fused_dispatch<T, TX, TY, KC, false>(f, b < f.nremap ? xcd_tile(b, f.nremap) : b, sm, smt);
   }
 }
 
@@ -1104,8 +1150,7 @@ inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
   f.Hn = Hn;
   f.ntx = (a.nx + TX - 1) / TX;
   f.nty = (a.ny + TY - 1) / TY;
-  f.nkc = (a.nz + KC - 1) / KC;
-  f.nb = f.ntx * f.nty * f.nkc;
+  chunk_layout<KC>(f, a.nz);
   if (a.nt)
     hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC, true>), dim3(f.nb), dim3(TX * TY),
                        fused_mt_bytes<T>(a.nmat), s, f);
