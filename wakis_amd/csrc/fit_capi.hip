@@ -804,15 +804,20 @@ template <typename T>
 static int copy_tiled_impl(int nx, int ny, int nz, int cfg, int tx, int ty, int kc, int reps,
                            double *ms, int device) {
   const int64_t N = (int64_t)nx * ny * nz;
-  const int64_t blocks = (int64_t)((nx + tx - 1) / tx) * ((ny + ty - 1) / ty) * ((nz + kc - 1) / kc);
-  const dim3 grid((unsigned)blocks), block((unsigned)(tx * ty));
   // unused dynamic LDS that leaves room for as many blocks per CU as the
-  // sweep itself keeps resident, so that both run the same rounds of tiles
+  // sweep itself keeps resident, so that both run the same rounds of tiles,
+  // and the sweep's choice of planes per chunk for this grid
   const int occ = fit::fused_blocks_per_cu<T>(cfg);
-  int lds_cu = 0;
+  int lds_cu = 0, ncu = 0;
   if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
                             device) != hipSuccess)
     lds_cu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    ncu = 256;
+  const int ncol = ((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
+  kc = fit::pick_kc(kc, ncol, nz, ncu * (occ > 0 ? occ : 1));
+  const int64_t blocks = (int64_t)ncol * ((nz + kc - 1) / kc);
+  const dim3 grid((unsigned)blocks), block((unsigned)(tx * ty));
   const size_t dyn = (occ > 0 && lds_cu > 2048) ? ((size_t)(lds_cu / occ - 1024) & ~(size_t)255) : 0;
   fit::Copy6<T> p{};
   std::vector<T *> bufs;

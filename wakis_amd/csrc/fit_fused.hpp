@@ -38,26 +38,27 @@ struct FusedArgs {
   T *En, *Hn;     // new buffers
   int ntx, nty, nkc;
   int nb;
+  int kc = 0;                    // planes per z chunk
   int kh = 0, kt = 0, nmid = 0;  // z chunk layout (chunk_span)
   int nremap = 0;                // blocks under the XCD remap
 };
 
-// Planes [k0, k1) of z chunk c of a tile column.  With kh > 0 the first
-// nmid chunks take KC planes each from plane kh, the last of them ending at
-// nz - kt, chunk nmid is [0, kh) and chunk nmid + 1 is [nz - kt, nz): every
-// plane but the kh + kt at the z faces lies in a chunk the lean path takes,
-// and the blocks of the two short chunks come last (outside the XCD remap,
-// so every XCD gets the same number of full chunks).  With kh = 0 chunk c
-// is [c KC, (c + 1) KC).
-template <int KC, typename T>
+// Planes [k0, k1) of z chunk c of a tile column (chunks of f.kc planes).
+// With kh > 0 the first nmid chunks take kc planes each from plane kh, the
+// last of them ending at nz - kt, chunk nmid is [0, kh) and chunk nmid + 1
+// is [nz - kt, nz): every plane but the kh + kt at the z faces lies in a
+// chunk the lean path takes, and the blocks of the two short chunks come
+// last (outside the XCD remap, so every XCD gets the same number of full
+// chunks).  With kh = 0 chunk c is [c kc, (c + 1) kc).
+template <typename T>
 __host__ __device__ inline void chunk_span(const FusedArgs<T> &f, int c, int &k0, int &k1) {
-  const int nz = f.a.nz;
+  const int nz = f.a.nz, kc = f.kc;
   if (f.kh == 0) {
-    k0 = c * KC;
-    k1 = k0 + KC < nz ? k0 + KC : nz;
+    k0 = c * kc;
+    k1 = k0 + kc < nz ? k0 + kc : nz;
   } else if (c < f.nmid) {
-    k0 = f.kh + c * KC;
-    k1 = k0 + KC < nz - f.kt ? k0 + KC : nz - f.kt;
+    k0 = f.kh + c * kc;
+    k1 = k0 + kc < nz - f.kt ? k0 + kc : nz - f.kt;
   } else if (c == f.nmid) {
     k0 = 0;
     k1 = f.kh;
@@ -163,7 +164,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
   const int ty = tr % f.nty;
   const int x0 = tx * TX, y0 = ty * TY;
   int k0, k1;  // planes k0 .. k1-1 of the tile's z chunk
-  chunk_span<KC>(f, tr / f.nty, k0, k1);
+  chunk_span(f, tr / f.nty, k0, k1);
 
   // Field and table pointers typed as global memory (address space 1): in
   // the batched launch they are read from memory, and a generic pointer
@@ -578,7 +579,7 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
   const int ty = tr % f.nty;
   const int x0 = tx * TX, y0 = ty * TY;
   int k0, k1;
-  chunk_span<KC>(f, tr / f.nty, k0, k1);
+  chunk_span(f, tr / f.nty, k0, k1);
   const uint32_t PB = (uint32_t)plane * SZ;    // bytes per plane
   const uint32_t NB = (uint32_t)a.N * SZ;      // bytes per component
   const int64_t FB = (int64_t)3 * a.N * SZ;    // bytes per field
@@ -918,27 +919,57 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
   }
 }
 
-// Host: the chunk layout of the grid (f.kh, f.kt, f.nmid, f.nkc, f.nb and
-// f.nremap, the blocks under the XCD remap).  A 4-plane chunk at each z face
-// when the grid has room for at least one full chunk between them.
-template <int KC, typename T>
-inline void chunk_layout(FusedArgs<T> &f, int nz) {
+// Host: the chunk layout of the grid for chunks of kc planes (f.kc, f.kh,
+// f.kt, f.nmid, f.nkc, f.nb and f.nremap, the blocks under the XCD
+// remap).  A 4-plane chunk at each z face when the grid has room for at
+// least one full chunk between them.
+template <typename T>
+inline void chunk_layout(FusedArgs<T> &f, int nz, int kc) {
   constexpr int kh = 4, kt = 4;
   const int ncol = f.ntx * f.nty;
-  if (nz - kh - kt >= KC) {
+  f.kc = kc;
+  if (nz - kh - kt >= kc) {
     f.kh = kh;
     f.kt = kt;
-    f.nmid = (nz - kh - kt + KC - 1) / KC;
+    f.nmid = (nz - kh - kt + kc - 1) / kc;
     f.nkc = f.nmid + 2;
     f.nremap = ncol * f.nmid;
   } else {
     f.kh = 0;
     f.kt = 0;
     f.nmid = 0;
-    f.nkc = (nz + KC - 1) / KC;
+    f.nkc = (nz + kc - 1) / kc;
     f.nremap = ncol * f.nkc;
   }
   f.nb = ncol * f.nkc;
+}
+
+// Planes per chunk of a single-grid sweep over ncol tile columns and nz
+// planes, with `slots` blocks resident on the device at once: the blocks run
+// in rounds (the XCD remap gives each of the 8 XCDs an eighth of them and of
+// the slots), a round lasting as long as a chunk; modelled time rounds x
+// (kc + 4 planes of prologue) over kc = KC, 1.5, 2, 3, 4 and 6 KC (the first
+// least).  On the 256 x 256 x 768 pillbox with 256 x 4 tiles: 192 planes,
+// one round of 256 blocks (0.546 ms per step against 0.577 for 64).
+inline int pick_kc(int KC, int ncol, int nz, int slots) {
+  const int mult[6] = {2, 3, 4, 6, 8, 12};  // x KC / 2
+  int best = KC;
+  double bt = 1e300;
+  const int xslots = slots / 8 > 0 ? slots / 8 : 1;
+  for (int q = 0; q < 6; ++q) {
+    const int kc = KC * mult[q] / 2;
+    if (q > 0 && kc > nz - 8) break;
+    const int nmid = nz - 8 >= kc ? (nz - 8 + kc - 1) / kc : (nz + kc - 1) / kc;
+    const long long blocks = (long long)ncol * nmid;
+    const long long per_xcd = (blocks + 7) / 8;
+    const long long rounds = (per_xcd + xslots - 1) / xslots;
+    const double t = (double)rounds * (kc + 4);
+    if (t < bt) {
+      bt = t;
+      best = kc;
+    }
+  }
+  return best;
 }
 
 // One tile with the code path for its J mode and z position: the lean path
@@ -950,7 +981,7 @@ template <typename T, int TX, int TY, int KC, bool NTS>
 __device__ __forceinline__ void fused_dispatch(const FusedArgs<T> &f, int tile,
                                                FusedSmem<T, TX, TY> &sm, T *smt) {
   int k0, k1;
-  chunk_span<KC>(f, tile / (f.ntx * f.nty), k0, k1);
+  chunk_span(f, tile / (f.ntx * f.nty), k0, k1);
   const bool in = sizeof(T) == 4 && f.a.buf32 && k0 >= 3 && k1 + 4 <= f.a.nz;
   if constexpr (sizeof(T) == 4) {
     if (in) {
@@ -1150,7 +1181,21 @@ inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
   f.Hn = Hn;
   f.ntx = (a.nx + TX - 1) / TX;
   f.nty = (a.ny + TY - 1) / TY;
-  chunk_layout<KC>(f, a.nz);
+  {
+    // resident blocks on the device (cached per device and configuration)
+    static int slots_cache[16] = {0};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int &slots = slots_cache[dev & 15];
+    if (slots == 0) {
+      int ncu = 0, occ = 0;
+      (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fused_sweep<T, TX, TY, KC, false>,
+                                                         TX * TY, fused_mt_bytes<T>(a.nmat));
+      slots = (ncu > 0 ? ncu : 256) * (occ > 0 ? occ : 1);
+    }
+    chunk_layout(f, a.nz, pick_kc(KC, f.ntx * f.nty, a.nz, slots));
+  }
   if (a.nt)
     hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC, true>), dim3(f.nb), dim3(TX * TY),
                        fused_mt_bytes<T>(a.nmat), s, f);
