@@ -921,14 +921,16 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
 
 // Host: the chunk layout of the grid for chunks of kc planes (f.kc, f.kh,
 // f.kt, f.nmid, f.nkc, f.nb and f.nremap, the blocks under the XCD
-// remap).  A 4-plane chunk at each z face when the grid has room for at
-// least one full chunk between them.
+// remap).  With `faces`, a 4-plane chunk at each z face when the grid has
+// room for at least one full chunk between them (the batched launch, whose
+// members are mostly small, keeps uniform chunks: their face chunks' extra
+// blocks cost more there than the planes they move to the lean path gain).
 template <typename T>
-inline void chunk_layout(FusedArgs<T> &f, int nz, int kc) {
+inline void chunk_layout(FusedArgs<T> &f, int nz, int kc, bool faces = true) {
   constexpr int kh = 4, kt = 4;
   const int ncol = f.ntx * f.nty;
   f.kc = kc;
-  if (nz - kh - kt >= kc) {
+  if (faces && nz - kh - kt >= kc) {
     f.kh = kh;
     f.kt = kt;
     f.nmid = (nz - kh - kt + kc - 1) / kc;
