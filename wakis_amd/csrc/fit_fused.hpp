@@ -164,7 +164,14 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
   const int ty = tr % f.nty;
   const int x0 = tx * TX, y0 = ty * TY;
   int k0, k1;  // planes k0 .. k1-1 of the tile's z chunk
-  chunk_span(f, tr / f.nty, k0, k1);
+  if constexpr (sizeof(T) == 8) {
+    // FP64 grids keep uniform chunks of KC planes (no lean path to serve);
+    // the compile-time length keeps this path as fast as it was
+    k0 = (tr / f.nty) * KC;
+    k1 = min(k0 + KC, nz);
+  } else {
+    chunk_span(f, tr / f.nty, k0, k1);
+  }
 
   // Field and table pointers typed as global memory (address space 1): in
   // the batched launch they are read from memory, and a generic pointer
@@ -952,12 +959,20 @@ inline void chunk_layout(FusedArgs<T> &f, int nz, int kc, bool faces = true) {
 // the slots), a round lasting as long as a chunk; modelled time rounds x
 // (kc + 4 planes of prologue) over kc = KC, 1.5, 2, 3, 4 and 6 KC (the first
 // least).  On the 256 x 256 x 768 pillbox with 256 x 4 tiles: 192 planes,
-// one round of 256 blocks (0.546 ms per step against 0.577 for 64).
+// one round of 256 blocks (0.546 ms per step against 0.577 for 64).  When
+// the tile's own KC already needs more than 8 rounds the rounding matters
+// little and KC stays (the 1024^3 grid: 13.74 ms per step with 128 planes
+// against 13.43 with 32).
 inline int pick_kc(int KC, int ncol, int nz, int slots) {
   const int mult[6] = {2, 3, 4, 6, 8, 12};  // x KC / 2
   int best = KC;
   double bt = 1e300;
   const int xslots = slots / 8 > 0 ? slots / 8 : 1;
+  {
+    const int nmid = (nz + KC - 1) / KC;
+    const long long per_xcd = ((long long)ncol * nmid + 7) / 8;
+    if ((per_xcd + xslots - 1) / xslots > 8) return KC;
+  }
   for (int q = 0; q < 6; ++q) {
     const int kc = KC * mult[q] / 2;
     if (q > 0 && kc > nz - 8) break;
@@ -982,17 +997,15 @@ inline int pick_kc(int KC, int ncol, int nz, int slots) {
 template <typename T, int TX, int TY, int KC, bool NTS>
 __device__ __forceinline__ void fused_dispatch(const FusedArgs<T> &f, int tile,
                                                FusedSmem<T, TX, TY> &sm, T *smt) {
-  int k0, k1;
-  chunk_span(f, tile / (f.ntx * f.nty), k0, k1);
-  const bool in = sizeof(T) == 4 && f.a.buf32 && k0 >= 3 && k1 + 4 <= f.a.nz;
   if constexpr (sizeof(T) == 4) {
-    if (in) {
+    int k0, k1;
+    chunk_span(f, tile / (f.ntx * f.nty), k0, k1);
+    if (f.a.buf32 && k0 >= 3 && k1 + 4 <= f.a.nz) {
       if (f.a.jmode == J_BUFFER) fused_tile_inner<T, TX, TY, KC, true, NTS>(f, tile, sm, smt);
       else fused_tile_inner<T, TX, TY, KC, false, NTS>(f, tile, sm, smt);
       return;
     }
   }
-  (void)in;
   if (f.a.jmode == J_BUFFER) fused_tile<T, TX, TY, KC, true, NTS>(f, tile, sm, smt);
   else fused_tile<T, TX, TY, KC, false, NTS>(f, tile, sm, smt);
 }
@@ -1196,7 +1209,12 @@ inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
                                                          TX * TY, fused_mt_bytes<T>(a.nmat));
       slots = (ncu > 0 ? ncu : 256) * (occ > 0 ? occ : 1);
     }
-    chunk_layout(f, a.nz, pick_kc(KC, f.ntx * f.nty, a.nz, slots));
+    // the face chunks and other chunk lengths serve the lean path (FP32,
+    // buffer-addressed fields); FP64 keeps uniform chunks of KC planes
+    if (sizeof(T) == 4)
+      chunk_layout(f, a.nz, pick_kc(KC, f.ntx * f.nty, a.nz, slots), a.buf32);
+    else
+      chunk_layout(f, a.nz, KC, false);
   }
   if (a.nt)
     hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC, true>), dim3(f.nb), dim3(TX * TY),
