@@ -50,9 +50,10 @@ tests/test_001_pec_cubic_cavity.py:18-67 (WP, Z, cumsums), TE101 frequency of
 tests/test_002_analytic_resonator.py:144, PML reflection bound of
 tests/test_005_pml_boundaries.py:99, discrete div(B) conservation (S.C = 0),
 brute-force loop vs vectorised WP, closed-form WP of a travelling Ez, and
-calc_long_Z against an independent direct DFT.  ``emsolve``'s ABC aliasing
-and ``calc_trans_WP`` edge semantics are pinned only by restatement from
-source text: parity unpinned for those two.
+calc_long_Z and calc_trans_Z against an independent direct DFT,
+calc_trans_WP against the Panofsky-Wenzel closed form of a field linear in
+x and y, and ``emsolve``'s ABC view aliasing against input/output vectors
+of the reference's own Field and BCsMixin (tests/golden/abc_aliasing.npz).
 """
 
 from __future__ import annotations

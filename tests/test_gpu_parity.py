@@ -143,6 +143,20 @@ def test_emsolve_abc_fp64():
     assert max(err.values()) <= TOL64, err
 
 
+def test_emsolve_abc_both_faces_one_axis_raises_like_reference():
+    """ABC on both z faces: the reference's update_abc writes the low x face,
+    then fails on the z axis' missing 'xlo' key after the first step; the
+    device run stops at the same point with the same fields."""
+    print("[case] emsolve abc low+high z", flush=True)
+    shape = (6, 5, 9)
+    osv, psv = make_pair(*shape, ["abc", "pec", "abc"], ["pec", "pec", "abc"], seed=12)
+    for sv in (osv, psv):
+        with pytest.raises(KeyError, match="xlo"):
+            sv.emsolve(5)
+    err = compare_fields(osv, psv)
+    assert max(err.values()) <= TOL64, err
+
+
 def test_wrap_large_nx_fp64():
     """A grid whose rows exceed one workgroup, periodic (helical wrap)."""
     print("[case] wrap large nx", flush=True)

@@ -283,3 +283,27 @@ def test_wake_impedance_transforms_sine():
         wk.calc_impedance_from_wake(wake, verbose=False)
     with pytest.raises(AttributeError):
         wk.calc_wake_from_impedance(Z, verbose=False)
+
+
+@pytest.mark.parametrize("n_cells", [1, 2])
+def test_trans_wp_host_closed_form(n_cells):
+    """WakeSolver.calc_trans_WP (host numpy, wakis/wakeSolver.py:553-626) on a
+    field with a closed-form Panofsky-Wenzel transverse wake, and equal to the
+    oracle's per-n loop to rounding on random data."""
+    from test_oracle import _linear_transverse_wake
+    from oracle.fit_oracle import OWake
+    from wakis_amd.wake import WakeSolver
+
+    s, xf, yf, wp3d, wpx, wpy = _linear_transverse_wake(n_cells, 120, 21 + n_cells)
+    w = WakeSolver(save=False, verbose=0)
+    w.calc_trans_WP(s=s, xf=xf, yf=yf, WP_3d=wp3d, n_transverse_cells=n_cells)
+    scale = np.abs(wpx).max()
+    assert np.allclose(w.WPx, wpx, rtol=0, atol=1e-11 * scale)
+    assert np.allclose(w.WPy, wpy, rtol=0, atol=1e-11 * scale)
+    rnd = np.random.default_rng(5).standard_normal(wp3d.shape)
+    o = OWake()
+    o.s, o.xf, o.yf, o.WP_3d, o.n_transverse_cells = s, xf, yf, rnd, n_cells
+    o.calc_trans_WP()
+    w.calc_trans_WP(WP_3d=rnd)
+    assert np.allclose(w.WPx, o.WPx, rtol=1e-12, atol=1e-12 * np.abs(o.WPx).max())
+    assert np.allclose(w.WPy, o.WPy, rtol=1e-12, atol=1e-12 * np.abs(o.WPy).max())

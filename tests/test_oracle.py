@@ -217,3 +217,102 @@ def test_long_z_matches_direct_dft():
     W = np.array([(x * np.exp(-2j * np.pi * j * n / N)).sum() for j in range(K)]) * ds
     Lf = np.array([(lam * np.exp(-2j * np.pi * j * n / N)).sum() for j in range(K)]) * ds
     assert np.allclose(w.Z, -W / Lf, rtol=1e-9, atol=1e-12 * np.abs(w.Z).max())
+
+
+def _linear_transverse_wake(n_cells, nt, seed):
+    """WP_3d(x, y, s) = a x + b y + f (x - x0)(y - y0) + w(s) on a
+    non-uniform transverse grid.  Panofsky-Wenzel fixes the transverse wake of
+    this field: the s-integral of the x-gradient along the centre row is
+    a * (left Riemann sum of ds over s[0:n]) = a * n * ds, so
+    WPx[n] = -a n ds and WPy[n] = -b n ds exactly.  The cross term vanishes on
+    the centre row and column, and w(s) has no transverse gradient."""
+    rng = np.random.default_rng(seed)
+    m = 2 * n_cells + 1
+    xf = np.cumsum(rng.uniform(0.5e-3, 2e-3, m))
+    yf = np.cumsum(rng.uniform(0.5e-3, 2e-3, m))
+    a, b, f = 3.0, -1.7, 250.0
+    s = np.linspace(-0.02, 0.1, nt)
+    w = rng.standard_normal(nt)
+    X, Y = xf[:, None, None], yf[None, :, None]
+    wp3d = (a * X + b * Y + f * (X - xf[n_cells]) * (Y - yf[n_cells]) + w[None, None, :])
+    ds = s[2] - s[1]
+    return s, xf, yf, wp3d, -a * ds * np.arange(nt), -b * ds * np.arange(nt)
+
+
+@pytest.mark.parametrize("n_cells", [1, 2])
+def test_trans_wp_panofsky_wenzel_closed_form(n_cells):
+    """calc_trans_WP (wakeSolver.py:553-626) on a field whose transverse wake
+    is known in closed form: pins the sign, the exclusive prefix s[0:n], the
+    x/y assignment, the centre indices and the non-uniform denominator."""
+    s, xf, yf, wp3d, wpx, wpy = _linear_transverse_wake(n_cells, 120, 11 + n_cells)
+    w = OWake()
+    w.s, w.xf, w.yf, w.WP_3d = s, xf, yf, wp3d
+    w.n_transverse_cells = n_cells
+    w.calc_trans_WP()
+    scale = np.abs(wpx).max()
+    assert np.allclose(w.WPx, wpx, rtol=0, atol=1e-11 * scale)
+    assert np.allclose(w.WPy, wpy, rtol=0, atol=1e-11 * scale)
+
+
+def test_trans_z_matches_direct_dft():
+    """calc_trans_Z (wakeSolver.py:722-785): Zx = 1j DFT_N(WPx 1e12) ds /
+    DFT_N(lambda v) ds on the kept bins, against an O(K N) direct DFT."""
+    w = OWake(sigmaz=0.01)
+    w.s = np.arange(-0.085, 0.3, 3e-4)
+    w.calc_lambdas_analytic()
+    w.WPx = np.sin(w.s * 30) * np.exp(-w.s * 2)
+    w.WPy = np.cos(w.s * 55) * np.exp(-w.s * 4)
+    w.calc_trans_Z()
+    n = np.arange(len(w.s))
+    ds = np.mean(np.diff(w.s))
+    N = int((w.v / ds) // w.fmax * 1001)
+    K = len(w.fx)
+    E = np.exp(-2j * np.pi * np.outer(np.arange(K), n) / N)
+    Lf = E @ (w.lambdas * w.v) * ds
+    for got, wp in ((w.Zx, w.WPx), (w.Zy, w.WPy)):
+        want = 1j * (E @ (wp * 1e12) * ds) / Lf
+        assert np.allclose(got, want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
+
+
+def test_abc_aliasing_matches_reference_vectors():
+    """The oracle's emsolve ABC lag (get_abc / update_abc on views of the
+    live field) reproduces the reference's own Field + BCsMixin run on the
+    same inputs bit for bit (tests/golden/make_abc_golden.py wrote the
+    vectors), including low+high ABC on one axis raising KeyError('xlo')
+    after the first step."""
+    g = np.load(os.path.join(HERE, "golden", "abc_aliasing.npz"))
+    nx, ny, nz, nt = (int(v) for v in g["dims"])
+    a_e, a_h = (float(v) for v in g["a"])
+    for name in g["cases"]:
+        name = str(name)
+        bl, bh = {
+            "low_x": (("abc", "pec", "pec"), ("pec",) * 3),
+            "low_y": (("pec", "abc", "pec"), ("pec",) * 3),
+            "low_z": (("pec", "pec", "abc"), ("pec",) * 3),
+            "high_z": (("pec",) * 3, ("pec", "pec", "abc")),
+            "high_x_low_y": (("pec", "abc", "pec"), ("abc", "pec", "pec")),
+            "low_xyz": (("abc",) * 3, ("pec",) * 3),
+            "low_high_z": (("pec", "pec", "abc"), ("pec", "pec", "abc")),
+            "low_x_low_high_z": (("abc", "pec", "abc"), ("pec", "pec", "abc")),
+        }[name]
+        s = OSolver(OGrid(0, 1, 0, 1, 0, 1, nx, ny, nz), bc_low=bl, bc_high=bh)
+        assert s.activate_abc
+        s.E.fromarray(g["E0"])
+        s.H.fromarray(g["H0"])
+        step = [0]
+
+        def one_step(s=s):
+            n = step[0]
+            s.E.fromarray(a_e * s.E.toarray() + g["dE"][n])
+            s.H.fromarray(a_h * s.H.toarray() + g["dH"][n])
+            step[0] += 1
+
+        s.one_step = one_step
+        err = ""
+        try:
+            s.emsolve(nt)
+        except KeyError as e:
+            err = f"KeyError:{e.args[0]}:step{step[0] - 1}"
+        assert err == str(g[name + "/error"]), name
+        assert np.array_equal(s.E.toarray(), g[name + "/E"]), name
+        assert np.array_equal(s.H.toarray(), g[name + "/H"]), name

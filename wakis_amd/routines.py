@@ -46,7 +46,7 @@ class RoutinesMixin:
                 xx, yy, zz = subdomain
             else:
                 xx, yy, zz = slice(0, self.Nx), slice(0, self.Ny), slice(0, self.Nz)
-        abc_mask = self._abc_low_mask() if self.activate_abc else 0
+        abc_mask, abc_conflict = self._abc_low_mask() if self.activate_abc else (0, False)
         for n in range(Nt):
             if source is not None:
                 source.update(self, n * self.dt)
@@ -57,6 +57,9 @@ class RoutinesMixin:
             self.one_step()
             if abc_mask:
                 self.ctx.apply_abc(abc_mask)
+            if abc_conflict:
+                # low+high ABC on one axis: update_abc fails after the first step
+                raise KeyError("xlo")
             if callback is not None:
                 callback(self, n * self.dt)
 

@@ -521,25 +521,29 @@ class SolverFIT3D(RoutinesMixin):
 
     # -- ABC (wakis/boundaries.py:307-404, CPU aliasing semantics) -----------
     def _abc_low_mask(self):
+        """(mask, conflict): the low ABC faces update_abc applies, in axis
+        order, and whether an axis with ABC on both faces stops it there."""
         mask = 0
         for d in range(3):
             lo = self.bc_low[d].lower() == "abc"
             hi = self.bc_high[d].lower() == "abc"
             if lo and hi:
-                # get_abc rebuilds the axis dict for the high face, so the
-                # low-face key is missing in update_abc
-                raise KeyError("xyz"[d] + "lo")
+                # get_abc's high-face branch rebuilds this axis' dictionary,
+                # so update_abc's first read of it ('xlo') fails here
+                return mask, True
             if lo:
                 mask |= 1 << d
-        return mask
+        return mask, False
 
     def get_abc(self):
-        return self._abc_low_mask(), None
+        return self._abc_low_mask()[0], None
 
     def update_abc(self, E_abc=None, H_abc=None):
-        mask = self._abc_low_mask()
+        mask, conflict = self._abc_low_mask()
         if mask:
             self.ctx.apply_abc(mask)
+        if conflict:
+            raise KeyError("xlo")
 
     def close(self):
         self.ctx.close()
