@@ -261,6 +261,29 @@ def test_fused_inner_chunks_fp32(tile):
     assert max(err.values()) <= TOL32, err
 
 
+@pytest.mark.parametrize("tile", list(range(6)))
+def test_fused_chunk_resources_fp32(tile):
+    """The lean path with a buffer resource per field component based at each
+    chunk (the addressing of fields of 2^32 bytes or more), forced on a small
+    grid: face chunks, interior chunks, periodic/PMC/PEC/PML faces,
+    conductive and PEC solids, initial J; against the oracle."""
+    shape = (300, 11, 150)
+    print(f"[case] fused chunk resources fp32 tile={tile} shape={shape}", flush=True)
+    Nx, Ny, Nz = shape
+    solids = [(block_mask(shape, (2, 1, 4), (Nx - 3, Ny - 2, Nz - 5)), "vacuum"),
+              (block_mask(shape, (30, 3, 20), (200, 9, 110)), [4.0, 1.3, 30.0]),
+              (block_mask(shape, (220, 2, 60), (260, 10, 140)), "pec")]
+    osv, psv = make_pair(*shape, ["periodic", "pmc", "pml"], ["pec", "periodic", "pml"],
+                         dtype=np.float32, solids=solids, seed=43, kernel="fused", tile=tile)
+    psv.ctx.set_chunk_rsrc(True)
+    for _ in range(8):
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    print(err)
+    assert max(err.values()) <= TOL32, err
+
+
 @pytest.mark.parametrize("tile", [0, 5])
 def test_fused_inner_chunks_k64_fp32(tile):
     """The lean path of the 64-plane tiles (320 x 3, 256 x 4) in FP32: 150

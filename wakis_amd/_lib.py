@@ -79,6 +79,7 @@ SIGNATURES = {
     "fit_set_kernel": [_P, _I32],
     "fit_kernel_in_use": [_P, _PI32],
     "fit_set_tile": [_P, _I32],
+    "fit_set_chunk_rsrc": [_P, _I32],
     "fit_get_tile": [_P, _PI32, _PI32, _PI32, _PI32],
     "fit_tune": [_P, _I32, _PD],
     "fit_sync": [_P],
@@ -307,6 +308,9 @@ class Context:
 
     def set_tile(self, cfg: int):
         check(self.lib.fit_set_tile(self.h, int(cfg)), "fit_set_tile")
+
+    def set_chunk_rsrc(self, on: bool):
+        check(self.lib.fit_set_chunk_rsrc(self.h, int(bool(on))), "fit_set_chunk_rsrc")
 
     def get_tile(self):
         v = [C.c_int32() for _ in range(4)]

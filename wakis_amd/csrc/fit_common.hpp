@@ -59,6 +59,7 @@ struct StepArgs {
   int64_t N;
   bool buf32;      // a field (3N values) and the metric tables span < 2^32 bytes
   bool nt;         // field stores of the step kernels non-temporal
+  bool pcr;        // lean path: buffer resources per field component and chunk
   int64_t geo_bytes;  // bytes of the packed metric tables (g.Lx first)
   uint32_t pec;    // tangential-E column mask faces
   uint32_t pmc;    // tangential-H row mask faces

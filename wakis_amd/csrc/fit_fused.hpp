@@ -569,9 +569,12 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
 // term is a product with 0 instead of a selected 0: the same value for
 // finite fields), and every global access is a buffer instruction: a
 // per-lane byte offset fixed over the march plus a wave-uniform plane
-// offset, so no address costs vector arithmetic.  Needs the fields and the
-// metric tables each to span less than 2^32 bytes.
-template <typename T, int TX, int TY, int KC, bool JB, bool NTS>
+// offset, so no address costs vector arithmetic.  Needs the metric tables
+// and the material map each to span less than 2^32 bytes, and either the
+// fields too (PCR false: one resource per field, component c at byte offset
+// c N sizeof(T)) or the planes one chunk touches (PCR true: one resource per
+// field component, based at the chunk's lowest plane k0-2).
+template <typename T, int TX, int TY, int KC, bool JB, bool NTS, bool PCR>
 __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const int tile,
                                                  FusedSmem<T, TX, TY> &sm, T *__restrict__ smt) {
   using G = Tile<TX, TY>;
@@ -591,8 +594,23 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
   const uint32_t NB = (uint32_t)a.N * SZ;      // bytes per component
   const int64_t FB = (int64_t)3 * a.N * SZ;    // bytes per field
   const Geo<T> &g = a.g;
-  const Rsrc rE = make_rsrc(a.E, FB), rH = make_rsrc(a.H, FB);
-  const Rsrc rEn = make_rsrc(f.En, FB), rHn = make_rsrc(f.Hn, FB);
+  // field component c: resource rX<c>, byte offset co<c> + cell offset +
+  // (k - kb) PB for plane k
+  const int kb = PCR ? k0 - 2 : 0;
+  const int64_t eb = (int64_t)kb * plane;  // elements below the base plane
+  const int64_t cspan = PCR ? ((int64_t)a.N - eb) * SZ : FB;
+  const int64_t rspan = cspan < 0xFFFFFFFFll ? cspan : 0xFFFFFFFFll;
+  auto comp_rsrc = [&](const T *p, int c) __attribute__((always_inline)) -> Rsrc {
+    return PCR ? make_rsrc(p + (int64_t)c * a.N + eb, rspan) : make_rsrc(p, FB);
+  };
+  const uint32_t co1 = PCR ? 0u : NB, co2 = PCR ? 0u : 2 * NB;
+  const Rsrc rE0 = comp_rsrc(a.E, 0), rE1 = comp_rsrc(a.E, 1), rE2 = comp_rsrc(a.E, 2);
+  const Rsrc rH0 = comp_rsrc(a.H, 0), rH1 = comp_rsrc(a.H, 1), rH2 = comp_rsrc(a.H, 2);
+  const Rsrc rEn0 = comp_rsrc(f.En, 0), rEn1 = comp_rsrc(f.En, 1), rEn2 = comp_rsrc(f.En, 2);
+  const Rsrc rHn0 = comp_rsrc(f.Hn, 0), rHn1 = comp_rsrc(f.Hn, 1), rHn2 = comp_rsrc(f.Hn, 2);
+  auto pofs = [&](int k) __attribute__((always_inline)) -> uint32_t {
+    return (uint32_t)(k - kb) * PB;
+  };
   const Rsrc rG = make_rsrc(g.Lx, a.geo_bytes);
   const bool hasmat = a.mat != nullptr;
   const Rsrc rM = make_rsrc(hasmat ? (const void *)a.mat : (const void *)g.Lx, hasmat ? a.N : 1);
@@ -623,8 +641,8 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
     const WinCell w = need ? win_cell(x0 - 1 + ri, y0 - 1 + rj, nx, ny) : WinCell{x0, y0, 0, false};
     wld[q] = c < RC ? (uint32_t)c : 0u;
     wob[q][0] = (uint32_t)(w.i + nx * w.j + plane * (w.dk + 1)) * SZ;
-    wob[q][1] = wob[q][0] + NB;
-    wob[q][2] = wob[q][0] + 2 * NB;
+    wob[q][1] = wob[q][0] + co1;
+    wob[q][2] = wob[q][0] + co2;
     wlz[q] = oLz + (uint32_t)(w.dk + 1) * SZ;
     fx[q] = (need && !xy_mask(pec, 0, w.i, w.j, nx, ny)) ? g.Lx[w.i] : T(0);
     fy[q] = (need && !xy_mask(pec, 1, w.i, w.j, nx, ny)) ? g.Ly[w.j] : T(0);
@@ -641,7 +659,7 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
   const int col = ic + nx * jc;
   // own cell byte offsets: field components; itA_y of row k (from row k);
   // iA_y of row k+1 (from row k)
-  const uint32_t cb0 = (uint32_t)col * SZ, cb1 = cb0 + NB, cb2 = cb0 + 2 * NB;
+  const uint32_t cb0 = (uint32_t)col * SZ, cb1 = cb0 + co1, cb2 = cb0 + co2;
   const uint32_t itb = oitAy + (uint32_t)ic * SZ, iab = oiAy + (uint32_t)ic * SZ + rowb;
   const uint32_t colm = (uint32_t)(col + plane);  // material of plane k+1 from plane k
   const T fe0 = xy_mask(pec, 0, ic, jc, nx, ny) ? T(0) : T(1);
@@ -688,8 +706,9 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
   const bool hma = xy_mask(pmc, hc0, hci, hcj, nx, ny), hmb = xy_mask(pmc, 2, hci, hcj, nx, ny);
   // halo offsets from plane kk-1 (kk: the halo's plane)
   const uint32_t hom = (uint32_t)(hcolo + plane * (hdk + 1));
-  const uint32_t hoa = hom * SZ + (hrow ? 0u : NB);  // H_x (rows) or H_y (column)
-  const uint32_t hob = hom * SZ + 2 * NB;            // H_z
+  const uint32_t hoa = hom * SZ + (hrow ? 0u : co1);  // H_x (rows) or H_y (column)
+  const uint32_t hob = hom * SZ + co2;                 // H_z
+  const Rsrc rHa = hrow ? rH0 : rH1;
   const uint32_t has = (uint32_t)__builtin_amdgcn_readfirstlane(hrow ? ny : nx) * SZ;
   const uint32_t oiAx = gofs(g.iAx);
   const uint32_t hab = (hrow ? oiAx : oiAy) +
@@ -703,12 +722,12 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
 
   auto slot = [](int k) __attribute__((always_inline)) { return (k + 3) % NR; };
   auto fetch = [&](int kk, T(&v)[RL][4]) __attribute__((always_inline)) {
-    const uint32_t s0 = (uint32_t)(kk - 1) * PB, sz0 = (uint32_t)(kk - 1) * SZ;
+    const uint32_t s0 = pofs(kk - 1), sz0 = (uint32_t)(kk - 1) * SZ;
 #pragma unroll
     for (int q = 0; q < RL; ++q) {
-      v[q][0] = bload<T>(rE, wob[q][0], s0);
-      v[q][1] = bload<T>(rE, wob[q][1], s0);
-      v[q][2] = bload<T>(rE, wob[q][2], s0);
+      v[q][0] = bload<T>(rE0, wob[q][0], s0);
+      v[q][1] = bload<T>(rE1, wob[q][1], s0);
+      v[q][2] = bload<T>(rE2, wob[q][2], s0);
       v[q][3] = bload<T>(rG, wlz[q], sz0);
     }
   };
@@ -744,9 +763,9 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
     T ha = 0, hb = 0, tlz = 0, area = 0;
     int m = 0;
     if (hwave) {
-      const uint32_t s0 = (uint32_t)(kk - 1) * PB;
-      ha = bload<T>(rH, hoa, s0);
-      hb = bload<T>(rH, hob, s0);
+      const uint32_t s0 = pofs(kk - 1);
+      ha = bload<T>(rHa, hoa, s0);
+      hb = bload<T>(rH2, hob, s0);
       m = hasmat ? bload_u8(rM, hom, (uint32_t)(kk - 1) * (uint32_t)plane) : 0;
       tlz = bload<T>(rG, hlz, (uint32_t)(kk - 2) * SZ);
       area = bload<T>(rG, hab, (uint32_t)(kk - 1) * has);
@@ -777,14 +796,14 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
     HaloPk h;            // halo of plane k+1
   };
   auto load_pk = [&](int k, Pk &p) __attribute__((always_inline)) {
-    const uint32_t se = (uint32_t)k * PB, sh = se + PB, sr = (uint32_t)k * rowb;
-    p.ex = bload<T>(rE, cb0, se);
-    p.ey = bload<T>(rE, cb1, se);
-    p.ez = bload<T>(rE, cb2, se);
+    const uint32_t se = pofs(k), sh = se + PB, sr = (uint32_t)k * rowb;
+    p.ex = bload<T>(rE0, cb0, se);
+    p.ey = bload<T>(rE1, cb1, se);
+    p.ez = bload<T>(rE2, cb2, se);
     p.itay = bload<T>(rG, itb, sr);
-    p.hx = bload<T>(rH, cb0, sh);
-    p.hy = bload<T>(rH, cb1, sh);
-    p.hz = bload<T>(rH, cb2, sh);
+    p.hx = bload<T>(rH0, cb0, sh);
+    p.hy = bload<T>(rH1, cb1, sh);
+    p.hz = bload<T>(rH2, cb2, sh);
     p.iay = bload<T>(rG, iab, sr);
     p.m = hasmat ? bload_u8(rM, colm, (uint32_t)k * (uint32_t)plane) : 0;
     halo_load(k + 1, p.h);
@@ -803,14 +822,14 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
   T hxm, hym, iAym, hx0, hy0, hz0, iAy0;
   int mm, mcur;
   {
-    const uint32_t sm1 = (uint32_t)(k0 - 1) * PB, s0 = sm1 + PB;
-    hxm = bload<T>(rH, cb0, sm1);
-    hym = bload<T>(rH, cb1, sm1);
+    const uint32_t sm1 = pofs(k0 - 1), s0 = sm1 + PB;
+    hxm = bload<T>(rH0, cb0, sm1);
+    hym = bload<T>(rH1, cb1, sm1);
     iAym = bload<T>(rG, iab, (uint32_t)(k0 - 2) * rowb);
     mm = hasmat ? bload_u8(rM, colm, (uint32_t)(k0 - 2) * (uint32_t)plane) : 0;
-    hx0 = bload<T>(rH, cb0, s0);
-    hy0 = bload<T>(rH, cb1, s0);
-    hz0 = bload<T>(rH, cb2, s0);
+    hx0 = bload<T>(rH0, cb0, s0);
+    hy0 = bload<T>(rH1, cb1, s0);
+    hz0 = bload<T>(rH2, cb2, s0);
     iAy0 = bload<T>(rG, iab, (uint32_t)(k0 - 1) * rowb);
     mcur = hasmat ? bload_u8(rM, colm, (uint32_t)(k0 - 1) * (uint32_t)plane) : 0;
   }
@@ -834,10 +853,10 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
     const T hy = hy0 - (mt[1] * (fm1 * iAy0)) * curl_u(1, p0, p1, li, lj);
     const T hz = hz0 - (mt[2] * iAz0m) * curl_u(2, p0, p1, li, lj);
     if (own) {
-      const uint32_t s0 = (uint32_t)k0 * PB;
-      bstore(rHn, cb0, s0, hx, NTS);
-      bstore(rHn, cb1, s0, hy, NTS);
-      bstore(rHn, cb2, s0, hz, NTS);
+      const uint32_t s0 = pofs(k0);
+      bstore(rHn0, cb0, s0, hx, NTS);
+      bstore(rHn1, cb1, s0, hy, NTS);
+      bstore(rHn2, cb2, s0, hz, NTS);
     }
     wx = own ? tLx0m * hx : T(0);
     wy = own ? tLy0m * hy : T(0);
@@ -863,7 +882,7 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
     // 3.
     __syncthreads();
     // 4. E_new(k)
-    const uint32_t se = (uint32_t)k * PB;
+    const uint32_t se = pofs(k);
     if (own) {
       const T(*ww)[TY + 1][TX + 1] = sm.w[k & 1];
       const T *mt = smt + mcur * MAT_STRIDE;
@@ -876,19 +895,18 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
       const T cz = ((wy - ww[1][lj][li - 1]) - wx) + ww[0][lj - 1][li];
       T jx, jy, jz;
       if (JB) {
-        const Rsrc rJ = make_rsrc(a.J, FB);
-        jx = bload<T>(rJ, cb0, se);
-        jy = bload<T>(rJ, cb1, se);
-        jz = beam_col ? jk : bload<T>(rJ, cb2, se);
+        jx = bload<T>(comp_rsrc(a.J, 0), cb0, se);
+        jy = bload<T>(comp_rsrc(a.J, 1), cb1, se);
+        jz = beam_col ? jk : bload<T>(comp_rsrc(a.J, 2), cb2, se);
       } else {
         const T js = a.jmode == J_SIGMA_E ? T(1) : T(0);
         jx = js * (mt[6] * eox);
         jy = js * (mt[7] * eoy);
         jz = beam_col ? jk : js * (mt[8] * eoz);
       }
-      bstore(rEn, cb0, se, eox + mt[3] * (ax * cx - jx), NTS);
-      bstore(rEn, cb1, se, eoy + mt[4] * (ay * cy - jy), NTS);
-      bstore(rEn, cb2, se, eoz + mt[5] * (az0 * cz - jz), NTS);
+      bstore(rEn0, cb0, se, eox + mt[3] * (ax * cx - jx), NTS);
+      bstore(rEn1, cb1, se, eoy + mt[4] * (ay * cy - jy), NTS);
+      bstore(rEn2, cb2, se, eoz + mt[5] * (az0 * cz - jz), NTS);
     }
     wxp = wx;
     wyp = wy;
@@ -903,9 +921,9 @@ __device__ __forceinline__ void fused_tile_inner(const FusedArgs<T> &f, const in
         const T hx = cur.hx - (mt[0] * (fm0 * iAx_k1)) * curl_u(0, p0, p1, li, lj);
         const T hy = cur.hy - (mt[1] * (fm1 * cur.iay)) * curl_u(1, p0, p1, li, lj);
         const T hz = cur.hz - (mt[2] * iAz0m) * curl_u(2, p0, p1, li, lj);
-        bstore(rHn, cb0, sh, hx, NTS);
-        bstore(rHn, cb1, sh, hy, NTS);
-        bstore(rHn, cb2, sh, hz, NTS);
+        bstore(rHn0, cb0, sh, hx, NTS);
+        bstore(rHn1, cb1, sh, hy, NTS);
+        bstore(rHn2, cb2, sh, hz, NTS);
         wx = tLx0m * hx;
         wy = tLy0m * hy;
         wz = (fm2 * tLz_k1) * hz;
@@ -992,17 +1010,19 @@ inline int pick_kc(int KC, int ncol, int nz, int slots) {
 // One tile with the code path for its J mode and z position: the lean path
 // when every plane the chunk touches (window and halo included) lies in
 // 1 .. nz-2, in FP32 (in FP64 its two packets and window buffers would halve
-// the waves a CU holds) and when a field and the metric tables span less
-// than 2^32 bytes (buffer offsets).  NTS: non-temporal field stores.
-template <typename T, int TX, int TY, int KC, bool NTS>
+// the waves a CU holds) and when its buffer offsets fit 32 bits: a field and
+// the metric tables span less than 2^32 bytes, or (PCR, set by the host only
+// when it holds) the planes of every chunk do, per field component.  NTS:
+// non-temporal field stores.
+template <typename T, int TX, int TY, int KC, bool NTS, bool PCR = false>
 __device__ __forceinline__ void fused_dispatch(const FusedArgs<T> &f, int tile,
                                                FusedSmem<T, TX, TY> &sm, T *smt) {
   if constexpr (sizeof(T) == 4) {
     int k0, k1;
     chunk_span(f, tile / (f.ntx * f.nty), k0, k1);
-    if (f.a.buf32 && k0 >= 3 && k1 + 4 <= f.a.nz) {
-      if (f.a.jmode == J_BUFFER) fused_tile_inner<T, TX, TY, KC, true, NTS>(f, tile, sm, smt);
-      else fused_tile_inner<T, TX, TY, KC, false, NTS>(f, tile, sm, smt);
+    if ((PCR || f.a.buf32) && k0 >= 3 && k1 + 4 <= f.a.nz) {
+      if (f.a.jmode == J_BUFFER) fused_tile_inner<T, TX, TY, KC, true, NTS, PCR>(f, tile, sm, smt);
+      else fused_tile_inner<T, TX, TY, KC, false, NTS, PCR>(f, tile, sm, smt);
       return;
     }
   }
@@ -1020,13 +1040,13 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 
 extern __shared__ double fit_dyn_lds[];
 
-template <typename T, int TX, int TY, int KC, bool NTS>
+template <typename T, int TX, int TY, int KC, bool NTS, bool PCR = false>
 __global__ void __launch_bounds__(TX *TY)
     __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) fused_sweep(FusedArgs<T> f) {
   __shared__ FusedSmem<T, TX, TY> sm;
   T *smt = (T *)fit_dyn_lds;
   const int b = blockIdx.x;
-  fused_dispatch<T, TX, TY, KC, NTS>(f, b < f.nremap ? xcd_tile(b, f.nremap) : b, sm, smt);
+  fused_dispatch<T, TX, TY, KC, NTS, PCR>(f, b < f.nremap ? xcd_tile(b, f.nremap) : b, sm, smt);
 }
 
 // ---------------- batched launch ----------------
@@ -1196,6 +1216,7 @@ inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
   f.Hn = Hn;
   f.ntx = (a.nx + TX - 1) / TX;
   f.nty = (a.ny + TY - 1) / TY;
+  bool pcr = false;
   {
     // resident blocks on the device (cached per device and configuration)
     static int slots_cache[16] = {0};
@@ -1211,10 +1232,29 @@ inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
     }
     // the face chunks and other chunk lengths serve the lean path (FP32,
     // buffer-addressed fields); FP64 keeps uniform chunks of KC planes
-    if (sizeof(T) == 4)
-      chunk_layout(f, a.nz, pick_kc(KC, f.ntx * f.nty, a.nz, slots), a.buf32);
-    else
+    if (sizeof(T) == 4) {
+      const int kc = pick_kc(KC, f.ntx * f.nty, a.nz, slots);
+      // fields of 2^32 bytes or more: buffer resources per field component
+      // based at each chunk, when the planes a chunk touches (at most kc+6
+      // from k0-2), the material map and the metric tables fit 32-bit offsets
+      constexpr int64_t LIM = 0xFFFFFFFFll;
+      pcr = (!a.buf32 || a.pcr) && a.geo_bytes < LIM && a.N < LIM &&
+            (int64_t)(kc + 8) * a.nx * a.ny * (int64_t)sizeof(T) < LIM;
+      chunk_layout(f, a.nz, kc, a.buf32 || pcr);
+    } else {
       chunk_layout(f, a.nz, KC, false);
+    }
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (pcr) {
+      if (a.nt)
+        hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC, true, true>), dim3(f.nb), dim3(TX * TY),
+                           fused_mt_bytes<T>(a.nmat), s, f);
+      else
+        hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC, false, true>), dim3(f.nb), dim3(TX * TY),
+                           fused_mt_bytes<T>(a.nmat), s, f);
+      return;
+    }
   }
   if (a.nt)
     hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC, true>), dim3(f.nb), dim3(TX * TY),
