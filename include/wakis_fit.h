@@ -210,9 +210,10 @@ enum { FIT_NTILE = 12 };
 int fit_set_tile(fit_ctx *ctx, int32_t cfg);
 /* Addressing of the fused sweep's lean path (no reference counterpart: a
  * layout choice of this library).  0 (default): one buffer resource per field
- * while a field spans less than 2^32 bytes, else one per field component
- * based at each chunk (fields up to 2^32 cells); 1: per component and chunk
- * on every grid (the same results; lets small grids exercise that path). */
+ * while a field spans less than 2^32 bytes, else, in FP32, one per field
+ * component based at each chunk (grids up to 2^32 cells); 1: per component
+ * and chunk on every FP32 grid (the same results; lets small grids exercise
+ * that path).  FP64 contexts ignore it. */
 int fit_set_chunk_rsrc(fit_ctx *ctx, int32_t on);
 int fit_get_tile(fit_ctx *ctx, int32_t *cfg, int32_t *tx, int32_t *ty, int32_t *kc);
 /* Autotuner: time every tile configuration of the fused sweep on this

@@ -284,6 +284,28 @@ def test_fused_chunk_resources_fp32(tile):
     assert max(err.values()) <= TOL32, err
 
 
+@pytest.mark.parametrize("tile", [1, 2, 4])
+def test_fused_inner_chunks_fp64(tile):
+    """The lean path in FP64 (80 planes in chunks of 32: the chunk at planes
+    32..63 touches neither z end) on every tile that fits FP64: periodic x/y
+    wrap, PMC/PEC/PML faces, conductive and PEC solids, initial J; against
+    the FP64 oracle at 1e-10."""
+    shape = (137, 13, 80)
+    print(f"[case] fused inner chunks fp64 tile={tile} shape={shape}", flush=True)
+    Nx, Ny, Nz = shape
+    solids = [(block_mask(shape, (2, 1, 4), (Nx - 3, Ny - 2, Nz - 5)), "vacuum"),
+              (block_mask(shape, (30, 3, 35), (80, 9, 60)), [3.0, 1.5, 40.0]),
+              (block_mask(shape, (90, 2, 33), (100, 11, 70)), "pec")]
+    osv, psv = make_pair(*shape, ["periodic", "pmc", "pml"], ["pec", "periodic", "pml"],
+                         dtype=np.float64, solids=solids, seed=47, kernel="fused", tile=tile)
+    for _ in range(8):
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    print(err)
+    assert max(err.values()) <= TOL64, err
+
+
 @pytest.mark.parametrize("tile", [0, 5])
 def test_fused_inner_chunks_k64_fp32(tile):
     """The lean path of the 64-plane tiles (320 x 3, 256 x 4) in FP32: 150

@@ -1009,15 +1009,15 @@ inline int pick_kc(int KC, int ncol, int nz, int slots) {
 
 // One tile with the code path for its J mode and z position: the lean path
 // when every plane the chunk touches (window and halo included) lies in
-// 1 .. nz-2, in FP32 (in FP64 its two packets and window buffers would halve
-// the waves a CU holds) and when its buffer offsets fit 32 bits: a field and
-// the metric tables span less than 2^32 bytes, or (PCR, set by the host only
-// when it holds) the planes of every chunk do, per field component.  NTS:
-// non-temporal field stores.
+// 1 .. nz-2 and its buffer offsets fit 32 bits: a field and the metric
+// tables span less than 2^32 bytes, or (PCR, set by the host only when it
+// holds) the planes of every chunk do, per field component.  (In FP64 the
+// lean tile holds 209 VGPRs against the general tile's 189: two waves per
+// SIMD either way.)  NTS: non-temporal field stores.
 template <typename T, int TX, int TY, int KC, bool NTS, bool PCR = false>
 __device__ __forceinline__ void fused_dispatch(const FusedArgs<T> &f, int tile,
                                                FusedSmem<T, TX, TY> &sm, T *smt) {
-  if constexpr (sizeof(T) == 4) {
+  {
     int k0, k1;
     chunk_span(f, tile / (f.ntx * f.nty), k0, k1);
     if ((PCR || f.a.buf32) && k0 >= 3 && k1 + 4 <= f.a.nz) {
@@ -1230,20 +1230,19 @@ inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
                                                          TX * TY, fused_mt_bytes<T>(a.nmat));
       slots = (ncu > 0 ? ncu : 256) * (occ > 0 ? occ : 1);
     }
-    // the face chunks and other chunk lengths serve the lean path (FP32,
-    // buffer-addressed fields); FP64 keeps uniform chunks of KC planes
-    if (sizeof(T) == 4) {
-      const int kc = pick_kc(KC, f.ntx * f.nty, a.nz, slots);
-      // fields of 2^32 bytes or more: buffer resources per field component
-      // based at each chunk, when the planes a chunk touches (at most kc+6
-      // from k0-2), the material map and the metric tables fit 32-bit offsets
-      constexpr int64_t LIM = 0xFFFFFFFFll;
-      pcr = (!a.buf32 || a.pcr) && a.geo_bytes < LIM && a.N < LIM &&
-            (int64_t)(kc + 8) * a.nx * a.ny * (int64_t)sizeof(T) < LIM;
-      chunk_layout(f, a.nz, kc, a.buf32 || pcr);
-    } else {
-      chunk_layout(f, a.nz, KC, false);
-    }
+    // the face chunks and other chunk lengths serve the FP32 lean path;
+    // FP64 keeps uniform chunks of KC planes (its general tile decodes the
+    // chunk at compile time)
+    const int kc = sizeof(T) == 4 ? pick_kc(KC, f.ntx * f.nty, a.nz, slots) : KC;
+    // FP32 fields of 2^32 bytes or more: buffer resources per field
+    // component based at each chunk, when the planes a chunk touches (at
+    // most kc+4 from k0-2), the material map and the metric tables fit
+    // 32-bit offsets.  (In FP64 that form was slower than the general path
+    // on 1024^3: 28.50 against 26.96 ms per step with 64 x 4 tiles.)
+    constexpr int64_t LIM = 0xFFFFFFFFll;
+    pcr = sizeof(T) == 4 && (!a.buf32 || a.pcr) && a.geo_bytes < LIM && a.N < LIM &&
+          (int64_t)(kc + 8) * a.nx * a.ny * (int64_t)sizeof(T) < LIM;
+    chunk_layout(f, a.nz, kc, sizeof(T) == 4 && (a.buf32 || pcr));
   }
   if constexpr (sizeof(T) == 4) {
     if (pcr) {
