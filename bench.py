@@ -686,7 +686,9 @@ def bench_large(args):
         # untimed warmup and tile search on the zero fields (they stay zero:
         # no source, J = sigma*E = 0), then the bunch and the probe from step 0
         tune_ms = None
-        if not args.no_tune and ctx.kernel_in_use() == _lib.KERNEL_FUSED:
+        if args.tile is not None and dtype == np.float32:
+            ctx.set_tile(args.tile)  # the FP32 run's tile (profiling runs)
+        elif not args.no_tune and ctx.kernel_in_use() == _lib.KERNEL_FUSED:
             tune_ms = s.tune(reps=2)
         ctx.step(4, cond)
         ctx.sync()
