@@ -8,7 +8,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _cavities(dtype, seed=5, n=3):
+def _cavities(dtype, seed=5, n=3, length=(0.02, 0.04)):
     import wakis_amd as wa
     from wakis_amd.workloads import build, rect_cavity
 
@@ -16,7 +16,7 @@ def _cavities(dtype, seed=5, n=3):
     out = []
     for i in range(n):
         wl = rect_cavity(side=rng.uniform(0.024, 0.036), half_pipe=rng.uniform(0.004, 0.007),
-                         length=rng.uniform(0.02, 0.04), dx=1.5e-3,
+                         length=rng.uniform(*length), dx=1.5e-3,
                          beta=[0.8, 0.9, 1.0][i % 3], n_pml=4, wakelength=0.08)
         out.append(build(wl, dtype=dtype))
     assert len({s.N for _, _, s in out}) == n  # uneven grids
@@ -52,6 +52,30 @@ def test_wakesolve_batch_equals_alone(dtype):
         assert np.array_equal(ea, eb) and np.array_equal(ha, hb)
     # members of different beta run different step counts: some left early
     assert len(set(nts)) > 1, nts
+
+
+def test_wakesolve_batch_equals_alone_fp64_inner_chunks():
+    """FP64 members of 155-167 planes, so chunks of 32 planes lie away from
+    both z ends and the batched launch runs its lean path there: the same
+    bits as each member solved alone."""
+    import wakis_amd as wa
+
+    print("[case] batch == alone float64 inner chunks", flush=True)
+    tile = 4
+    alone = _cavities(np.float64, seed=6, length=(0.07, 0.09))
+    assert min(s.Nz for _, _, s in alone) >= 68
+    for _, w, s in alone:
+        s.ctx.set_tile(tile)
+        s.wakesolve(compute_plane="longitudinal")
+    mixed = _cavities(np.float64, seed=6, length=(0.07, 0.09))
+    wa.wakesolve_batch([s for _, _, s in mixed], compute_plane="longitudinal", graph=True,
+                       group=4, tile=tile)
+    for (_, wa_, sa), (_, wb, sb) in zip(alone, mixed):
+        assert sa.Nt == sb.Nt
+        assert np.array_equal(np.asarray(wa_.WP), np.asarray(wb.WP))
+        ea, ha = _fields(sa)
+        eb, hb = _fields(sb)
+        assert np.array_equal(ea, eb) and np.array_equal(ha, hb)
 
 
 def test_batch_graph_replays_equal_eager():
