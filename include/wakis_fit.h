@@ -211,7 +211,8 @@ int fit_set_tile(fit_ctx *ctx, int32_t cfg);
 /* Addressing of the fused sweep's lean path (no reference counterpart: a
  * layout choice of this library).  0 (default): one buffer resource per field
  * while a field spans less than 2^32 bytes, else, in FP32, one per field
- * component based at each chunk (grids up to 2^32 cells); 1: per component
+ * component based at each chunk (any grid fit_create accepts: fewer than
+ * 2^31 cells); 1: per component
  * and chunk on every FP32 grid (the same results; lets small grids exercise
  * that path).  FP64 contexts ignore it. */
 int fit_set_chunk_rsrc(fit_ctx *ctx, int32_t on);

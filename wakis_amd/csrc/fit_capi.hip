@@ -253,7 +253,9 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
     if (ev) HIPCHK(hipEventRecord(ev[3 * s], c->stream));
     if (fused) {
       const int cfg = c->fused_cfg >= 0 ? c->fused_cfg : fit::sweep_default(c->nx, c->ny, c->nz, c->dtype == FIT_F64);
-      fit::launch_sweep<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg);
+      if (!fit::launch_sweep<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg))
+        return fail(FIT_ESTATE, "fused sweep tile configuration " + std::to_string(cfg) +
+                                    " spills to scratch memory in this build; not launched");
       std::swap(c->E, c->E_alt);
       std::swap(c->H, c->H_alt);
       if (ev) HIPCHK(hipEventRecord(ev[3 * s + 1], c->stream));

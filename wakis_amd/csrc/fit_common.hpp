@@ -15,7 +15,26 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <unordered_map>
+
 namespace fit {
+
+// A sweep variant is launched only if it keeps every value in registers:
+// variants the compiler spills to scratch memory are never launched (they
+// are slow, and their results have not been validated).  Cached per kernel,
+// behind a lock.
+inline bool kernel_spill_free(const void *fn) {
+  static std::mutex mu;
+  static std::unordered_map<const void *, bool> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(fn);
+  if (it != cache.end()) return it->second;
+  hipFuncAttributes at{};
+  const bool ok = hipFuncGetAttributes(&at, fn) == hipSuccess && at.localSizeBytes == 0;
+  cache[fn] = ok;
+  return ok;
+}
 
 // Face bits: xlo, xhi, ylo, yhi, zlo, zhi.
 constexpr uint32_t XLO = 1u, XHI = 2u, YLO = 4u, YHI = 8u, ZLO = 16u, ZHI = 32u;

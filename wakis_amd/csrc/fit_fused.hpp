@@ -1204,12 +1204,60 @@ inline int fused_pick(const int *nx, const int *ny, const int *nz, int n,
   return best;
 }
 
+// Resident blocks of kernel fn on device dev with `lds` bytes of dynamic
+// LDS per block (CUs x blocks per CU), cached per (device, kernel, LDS bytes)
+// behind a lock.
+inline int resident_slots(int dev, const void *fn, int nthreads, size_t lds) {
+  struct Key {
+    int dev;
+    const void *fn;
+    size_t lds;
+    bool operator==(const Key &o) const { return dev == o.dev && fn == o.fn && lds == o.lds; }
+  };
+  struct KeyHash {
+    size_t operator()(const Key &k) const {
+      return std::hash<const void *>()(k.fn) ^ (k.lds * 1000003u) ^ ((size_t)k.dev << 48);
+    }
+  };
+  static std::mutex mu;
+  static std::unordered_map<Key, int, KeyHash> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  const Key key{dev, fn, lds};
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int ncu = 0, occ = 0;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, nthreads, lds);
+  const int slots = (ncu > 0 ? ncu : 256) * (occ > 0 ? occ : 1);
+  cache[key] = slots;
+  return slots;
+}
+
+// Every single-grid instantiation of a tile configuration (stores ordinary
+// and non-temporal; FP32 also the per-component resources) is spill-free.
 template <typename T, int TX, int TY, int KC>
-inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
+inline bool fused_cfg_spill_free() {
+  if constexpr (!cfg_fits<T, TX, TY>()) {
+    return false;
+  } else {
+    bool ok = kernel_spill_free((const void *)fused_sweep<T, TX, TY, KC, false>) &&
+              kernel_spill_free((const void *)fused_sweep<T, TX, TY, KC, true>);
+    if constexpr (sizeof(T) == 4)
+      ok = ok && kernel_spill_free((const void *)fused_sweep<T, TX, TY, KC, false, true>) &&
+           kernel_spill_free((const void *)fused_sweep<T, TX, TY, KC, true, true>);
+    return ok;
+  }
+}
+
+// Launches one step of the fused sweep; false (nothing launched) when the
+// configuration does not fit the LDS or a variant of it spills.
+template <typename T, int TX, int TY, int KC>
+inline bool launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
   if constexpr (!cfg_fits<T, TX, TY>()) {
     (void)a; (void)En; (void)Hn; (void)s;
-    return;
+    return false;
   } else {
+  if (!fused_cfg_spill_free<T, TX, TY, KC>()) return false;
   FusedArgs<T> f;
   f.a = a;
   f.En = En;
@@ -1218,18 +1266,10 @@ inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
   f.nty = (a.ny + TY - 1) / TY;
   bool pcr = false;
   {
-    // resident blocks on the device (cached per device and configuration)
-    static int slots_cache[16] = {0};
     int dev = 0;
     (void)hipGetDevice(&dev);
-    int &slots = slots_cache[dev & 15];
-    if (slots == 0) {
-      int ncu = 0, occ = 0;
-      (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fused_sweep<T, TX, TY, KC, false>,
-                                                         TX * TY, fused_mt_bytes<T>(a.nmat));
-      slots = (ncu > 0 ? ncu : 256) * (occ > 0 ? occ : 1);
-    }
+    const int slots = resident_slots(dev, (const void *)fused_sweep<T, TX, TY, KC, false>,
+                                     TX * TY, fused_mt_bytes<T>(a.nmat));
     // the face chunks and other chunk lengths serve the FP32 lean path;
     // FP64 keeps uniform chunks of KC planes (its general tile decodes the
     // chunk at compile time)
@@ -1252,7 +1292,7 @@ inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
       else
         hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC, false, true>), dim3(f.nb), dim3(TX * TY),
                            fused_mt_bytes<T>(a.nmat), s, f);
-      return;
+      return true;
     }
   }
   if (a.nt)
@@ -1261,6 +1301,7 @@ inline void launch_cfg(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s) {
   else
     hipLaunchKernelGGL((fused_sweep<T, TX, TY, KC, false>), dim3(f.nb), dim3(TX * TY),
                        fused_mt_bytes<T>(a.nmat), s, f);
+  return true;
   }
 }
 
@@ -1291,15 +1332,28 @@ inline int fused_blocks_per_cu(int cfg) {
   }
 }
 
+// false: nothing launched (see launch_cfg)
 template <typename T>
-inline void launch_fused(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s, int cfg) {
+inline bool launch_fused(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s, int cfg) {
   switch (cfg) {
-    case FUSED_CFG_128x4: launch_cfg<T, 128, 4, 32>(a, En, Hn, s); break;
-    case FUSED_CFG_64x8: launch_cfg<T, 64, 8, 32>(a, En, Hn, s); break;
-    case FUSED_CFG_256x4: launch_cfg<T, 256, 4, 32>(a, En, Hn, s); break;
-    case FUSED_CFG_64x4: launch_cfg<T, 64, 4, 32>(a, En, Hn, s); break;
-    case FUSED_CFG_256x4_K64: launch_cfg<T, 256, 4, 64>(a, En, Hn, s); break;
-    default: launch_cfg<T, 320, 3, 64>(a, En, Hn, s); break;
+    case FUSED_CFG_128x4: return launch_cfg<T, 128, 4, 32>(a, En, Hn, s);
+    case FUSED_CFG_64x8: return launch_cfg<T, 64, 8, 32>(a, En, Hn, s);
+    case FUSED_CFG_256x4: return launch_cfg<T, 256, 4, 32>(a, En, Hn, s);
+    case FUSED_CFG_64x4: return launch_cfg<T, 64, 4, 32>(a, En, Hn, s);
+    case FUSED_CFG_256x4_K64: return launch_cfg<T, 256, 4, 64>(a, En, Hn, s);
+    default: return launch_cfg<T, 320, 3, 64>(a, En, Hn, s);
+  }
+}
+
+template <typename T>
+inline bool fused_spill_free(int cfg) {
+  switch (cfg) {
+    case FUSED_CFG_128x4: return fused_cfg_spill_free<T, 128, 4, 32>();
+    case FUSED_CFG_64x8: return fused_cfg_spill_free<T, 64, 8, 32>();
+    case FUSED_CFG_256x4: return fused_cfg_spill_free<T, 256, 4, 32>();
+    case FUSED_CFG_64x4: return fused_cfg_spill_free<T, 64, 4, 32>();
+    case FUSED_CFG_256x4_K64: return fused_cfg_spill_free<T, 256, 4, 64>();
+    default: return fused_cfg_spill_free<T, 320, 3, 64>();
   }
 }
 

@@ -628,21 +628,6 @@ inline bool row_offsets_fit(const StepArgs<T> &a, int KC) {
          a.ny < 16384;
 }
 
-// A row-sweep variant is used only if it keeps every value in registers:
-// variants the compiler spills to scratch memory are never launched (they
-// are slow, and their results have not been validated).
-inline bool kernel_spill_free(const void *fn) {
-  static std::mutex mu;
-  static std::unordered_map<const void *, bool> cache;
-  std::lock_guard<std::mutex> lk(mu);
-  auto it = cache.find(fn);
-  if (it != cache.end()) return it->second;
-  hipFuncAttributes at{};
-  const bool ok = hipFuncGetAttributes(&at, fn) == hipSuccess && at.localSizeBytes == 0;
-  cache[fn] = ok;
-  return ok;
-}
-
 // Launches the row sweep for these args (or only checks, when dry); false if
 // the variant it needs cannot run (LDS, 32-bit offsets, scratch spills).
 template <typename T, int TX, int TY, int KC>
@@ -710,17 +695,18 @@ inline int sweep_default(int nx, int ny, int nz, bool fp64) {
 // configuration that cannot run falls back to the fused_sweep default).
 template <typename T>
 inline bool sweep_cfg_runs(const StepArgs<T> &a, int cfg) {
-  if (cfg < FUSED_NCFG) return fused_cfg_ok<T>(cfg);
+  if (cfg < FUSED_NCFG) return fused_cfg_ok<T>(cfg) && fused_spill_free<T>(cfg);
   return launch_row_cfg<T>(a, nullptr, nullptr, nullptr, cfg - FUSED_NCFG, true);
 }
 
+// false: nothing launched (the fused_sweep configuration needed spills)
 template <typename T>
-inline void launch_sweep(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s, int cfg) {
+inline bool launch_sweep(const StepArgs<T> &a, T *En, T *Hn, hipStream_t s, int cfg) {
   if (cfg >= FUSED_NCFG) {
-    if (launch_row_cfg<T>(a, En, Hn, s, cfg - FUSED_NCFG, false)) return;
+    if (launch_row_cfg<T>(a, En, Hn, s, cfg - FUSED_NCFG, false)) return true;
     cfg = fused_default(a.nx, sizeof(T) == 8);
   }
-  launch_fused<T>(a, En, Hn, s, cfg);
+  return launch_fused<T>(a, En, Hn, s, cfg);
 }
 
 }  // namespace fit
