@@ -249,6 +249,21 @@ int fit_impedance(fit_ctx *ctx, const double *x, const double *lam,
                   double *z_re, double *z_im, double *x_re, double *x_im,
                   double *l_re, double *l_im);
 
+/* fit_impedance without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernel and results (bit for bit) as
+ * fit_impedance; the device staging lives in a fixed per-device scratch
+ * array of the library (4.375 MiB per device, held until the process ends),
+ * so min(len, N) must be <= FIT_NOALLOC_MAX_LEN and K <= FIT_NOALLOC_MAX_BINS
+ * (else FIT_EINVAL).  Every call on one device shares that array: calls that
+ * overlap on one device (two host threads, or two contexts' streams) race. */
+enum { FIT_NOALLOC_MAX_LEN = 262144, FIT_NOALLOC_MAX_BINS = 8192 };
+int fit_impedance_noalloc(fit_ctx *ctx, const double *x, const double *lam,
+                          int64_t len, int64_t N, int32_t K, double ds,
+                          double *z_re, double *z_im, double *x_re,
+                          double *x_im, double *l_re, double *l_im);
+
 /* ---------------- batches (no counterpart in the reference) ----------------
  * The reference steps one solver at a time (wakis/routines.py:231-485).  A
  * batch steps many independent contexts (same device and dtype, fused

@@ -85,6 +85,8 @@ SIGNATURES = {
     "fit_sync": [_P],
     "fit_wake_potential": [_P, _I32, _PI32, _I32, _I32, _I32, _I32, _PI32, _PD, _D, _PD],
     "fit_impedance": [_P, _PD, _PD, _I64, _I64, _I32, _D, _PD, _PD, _PD, _PD, _PD, _PD],
+    "fit_impedance_noalloc": [_P, _PD, _PD, _I64, _I64, _I32, _D, _PD, _PD, _PD, _PD, _PD,
+                              _PD],
     "fit_batch_create": [C.POINTER(_P), _I32, _PI32, C.POINTER(_P)],
     "fit_batch_set_members": [_P, C.POINTER(_P), _I32, _PI32],
     "fit_batch_destroy": [_P],
@@ -372,13 +374,18 @@ class Context:
                                           dptr(out)), "fit_wake_potential")
         return out.reshape(len(w), ns)
 
-    def impedance(self, x: np.ndarray, lam: np.ndarray, N: int, K: int, ds: float):
+    def impedance(self, x: np.ndarray, lam: np.ndarray, N: int, K: int, ds: float,
+                  noalloc: bool = False):
+        """Z, X*ds, Lam*ds of bins 0 .. K-1 (fit_impedance; noalloc=True uses
+        fit_impedance_noalloc, which stages in the library's fixed scratch)."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         lam = np.ascontiguousarray(lam, dtype=np.float64)
         assert x.size == lam.size
         bufs = [np.empty(K, dtype=np.float64) for _ in range(6)]
-        check(self.lib.fit_impedance(self.h, dptr(x), dptr(lam), x.size, int(N), int(K),
-                                     float(ds), *[dptr(b) for b in bufs]), "fit_impedance")
+        fn, name = ((self.lib.fit_impedance_noalloc, "fit_impedance_noalloc") if noalloc
+                    else (self.lib.fit_impedance, "fit_impedance"))
+        check(fn(self.h, dptr(x), dptr(lam), x.size, int(N), int(K), float(ds),
+                 *[dptr(b) for b in bufs]), name)
         z = bufs[0] + 1j * bufs[1]
         X = bufs[2] + 1j * bufs[3]
         L = bufs[4] + 1j * bufs[5]

@@ -199,7 +199,8 @@ __global__ void wake_potential(const T *__restrict__ hist, int64_t hist_nt, int 
 __global__ void __launch_bounds__(256) impedance_dft(const double *__restrict__ x,
                                                      const double *__restrict__ lam,
                                                      int64_t len, int64_t N, double ds,
-                                                     double *__restrict__ out) {
+                                                     double *__restrict__ out, int64_t oj,
+                                                     int64_t oq) {
   const int jb = blockIdx.x;
   const int64_t M = len < N ? len : N;
   double xr = 0, xi = 0, lr = 0, li = 0;
@@ -230,13 +231,15 @@ __global__ void __launch_bounds__(256) impedance_dft(const double *__restrict__ 
     const double den = Lr * Lr + Li * Li;
     const double zr = -(Xr * Lr + Xi * Li) / den;
     const double zi = -(Xi * Lr - Xr * Li) / den;
-    double *o = out + (int64_t)jb * 6;
-    o[0] = zr;
-    o[1] = zi;
-    o[2] = Xr;
-    o[3] = Xi;
-    o[4] = Lr;
-    o[5] = Li;
+    // output q of bin jb at out[jb * oj + q * oq]: interleaved (oj = 6,
+    // oq = 1) or one array per quantity (oj = 1, oq = K)
+    double *o = out + (int64_t)jb * oj;
+    o[0 * oq] = zr;
+    o[1 * oq] = zi;
+    o[2 * oq] = Xr;
+    o[3 * oq] = Xi;
+    o[4 * oq] = Lr;
+    o[5 * oq] = Li;
   }
 }
 

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -343,6 +344,43 @@ int tune_impl(fit_ctx *c, int reps, double *ms_out) {
 
 }  // namespace
 
+// ---------------- allocation-free impedance DFT ----------------
+//
+// fit_impedance_noalloc stages its inputs and outputs in this file-scope
+// device array instead of hipMalloc'd buffers: x (NOALLOC_MAX_LEN doubles),
+// lam (NOALLOC_MAX_LEN doubles), then the six output arrays of
+// NOALLOC_MAX_BINS doubles each.  Every device holds its own copy for the
+// life of the process.  Host code never passes `scratch` itself to a kernel
+// or a copy (in host code the name is only a stand-in): it uses the device
+// address that hipGetSymbolAddress returns, resolved once per device with
+// that device current.
+constexpr int64_t NOALLOC_MAX_LEN = FIT_NOALLOC_MAX_LEN;
+constexpr int64_t NOALLOC_MAX_BINS = FIT_NOALLOC_MAX_BINS;
+constexpr int64_t NOALLOC_WORDS = 2 * NOALLOC_MAX_LEN + 6 * NOALLOC_MAX_BINS;
+static __device__ __align__(256) double scratch[NOALLOC_WORDS];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on device `dev` (must be current); resolved on
+// the first call per device.
+int noalloc_scratch(int dev, double **out) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(FIT_EINVAL, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(FIT_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double *)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
 extern "C" {
 
 const char *fit_last_error(void) { return g_err.c_str(); }
@@ -368,6 +406,14 @@ int fit_create(const fit_desc *d, fit_ctx **out) {
   if (rc) {
     delete c;
     return rc;
+  }
+  {
+    // set-up of this device's allocation-free scratch (fit_impedance_noalloc)
+    double *sp = nullptr;
+    if ((rc = noalloc_scratch(c->device, &sp))) {
+      delete c;
+      return rc;
+    }
   }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
@@ -1074,7 +1120,8 @@ int fit_impedance(fit_ctx *c, const double *x, const double *lam, int64_t len, i
   hipError_t e = hipMemcpyAsync(dx, x, M * 8, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(dl, lam, M * 8, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(fit::impedance_dft, dim3(K), dim3(256), 0, c->stream, dx, dl, M, N, ds, dout);
+    hipLaunchKernelGGL(fit::impedance_dft, dim3(K), dim3(256), 0, c->stream, dx, dl, M, N, ds, dout,
+                       (int64_t)6, (int64_t)1);
     e = hipGetLastError();
   }
   if (e == hipSuccess)
@@ -1092,6 +1139,45 @@ int fit_impedance(fit_ctx *c, const double *x, const double *lam, int64_t len, i
     l_re[j] = out[j * 6 + 4];
     l_im[j] = out[j * 6 + 5];
   }
+  return 0;
+}
+
+int fit_impedance_noalloc(fit_ctx *c, const double *x, const double *lam, int64_t len,
+                          int64_t N, int32_t K, double ds, double *z_re, double *z_im,
+                          double *x_re, double *x_im, double *l_re, double *l_im) {
+  if (!c || !x || !lam || !z_re || !z_im || !x_re || !x_im || !l_re || !l_im)
+    return fail(FIT_EINVAL, "null argument");
+  if (len < 1 || N < 1 || K < 1 || K > N) return fail(FIT_EINVAL, "bad DFT sizes");
+  if ((int64_t)K * N > ((int64_t)1 << 62)) return fail(FIT_EINVAL, "DFT index overflow");
+  const int64_t M = std::min(len, N);
+  if (M > NOALLOC_MAX_LEN)
+    return fail(FIT_EINVAL, "fit_impedance_noalloc: min(len, N) = " + std::to_string(M) +
+                                " exceeds the scratch limit of " +
+                                std::to_string(NOALLOC_MAX_LEN) + " samples");
+  if (K > NOALLOC_MAX_BINS)
+    return fail(FIT_EINVAL, "fit_impedance_noalloc: K = " + std::to_string(K) +
+                                " exceeds the scratch limit of " +
+                                std::to_string(NOALLOC_MAX_BINS) + " bins");
+  if (int rc = set_device(c)) return rc;
+  double *sp = nullptr;
+  if (int rc = noalloc_scratch(c->device, &sp)) return rc;
+  double *dx = sp, *dl = sp + NOALLOC_MAX_LEN, *dout = sp + 2 * NOALLOC_MAX_LEN;
+  // every output word the copies below read is written by the kernel, and the
+  // inputs are written before it, so nothing needs clearing between calls
+  hipError_t e = hipMemcpyAsync(dx, x, M * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dl, lam, M * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(fit::impedance_dft, dim3(K), dim3(256), 0, c->stream, dx, dl, M, N, ds,
+                       dout, (int64_t)1, (int64_t)K);
+    e = hipGetLastError();
+  }
+  double *const outs[6] = {z_re, z_im, x_re, x_im, l_re, l_im};
+  for (int q = 0; q < 6 && e == hipSuccess; ++q)
+    e = hipMemcpyAsync(outs[q], dout + (int64_t)q * K, (size_t)K * 8, hipMemcpyDeviceToHost,
+                       c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess)
+    return fail(FIT_EHIP, std::string("impedance (no alloc): ") + hipGetErrorString(e));
   return 0;
 }
 
