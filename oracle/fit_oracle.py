@@ -54,6 +54,14 @@ calc_long_Z and calc_trans_Z against an independent direct DFT,
 calc_trans_WP against the Panofsky-Wenzel closed form of a field linear in
 x and y, and ``emsolve``'s ABC view aliasing against input/output vectors
 of the reference's own Field and BCsMixin (tests/golden/abc_aliasing.npz).
+``_apply_bc_to_C`` (periodic/ABC metric fix-ups, PEC column and PMC row
+masks with carried flags), ``_initialize_PML`` (grading and corner overwrite
+order) and ``OBeam`` (beta 0.8/0.9/1.0, default and given ti) against vectors
+of the reference's own boundaries.py/field.py/sources.py
+(tests/golden/bc_pml_beam.npz, tests/test_oracle_pins.py); the SIBC
+eps := sigma*eps_0 rule and the relaxation clamp of dt against closed forms
+(tau = eps_0 for any SIBC solid and for the outer PML cell; eps_r eps_0 /
+sigma without SIBC; the empty conductive set raising).
 """
 
 from __future__ import annotations

@@ -1,0 +1,249 @@
+"""Pins of the oracle functions that no closed form or reference test fixed:
+vectors written by the reference's own boundaries.py / field.py / sources.py
+(tests/golden/make_bc_golden.py), plus closed forms of the SIBC permittivity
+rule and the relaxation-time clamp of dt.  Where the product computes the same
+quantity on the host, it is checked against the same vectors."""
+
+import os
+from types import MethodType, SimpleNamespace
+
+import numpy as np
+import pytest
+from scipy.constants import c as c_light
+from scipy.constants import epsilon_0 as eps_0
+from scipy.sparse import coo_matrix
+
+from helpers import block_mask
+from oracle.fit_oracle import OBeam, OField, OGrid, OSolver
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+G = np.load(os.path.join(HERE, "golden", "bc_pml_beam.npz"), allow_pickle=False)
+CASES = [str(c) for c in G["cases"]]
+PML_CASES = [str(c) for c in G["pml/cases"]]
+
+
+def _golden_C(name, n3):
+    idx, val = G[name + "/C_idx"], G[name + "/C_val"]
+    return coo_matrix((val.astype(np.int64), (idx[0], idx[1])), shape=(n3, n3)).toarray()
+
+
+def _bcs(name):
+    # the cases of make_bc_golden.py, in its order
+    return {
+        "pec": (["pec"] * 3, ["pec"] * 3),
+        "periodic": (["periodic"] * 3, ["periodic"] * 3),
+        "mixed": (["periodic", "pec", "pec"], ["periodic", "pmc", "pml"]),
+        "pmc": (["pmc", "pec", "magnetic"], ["pec", "pmc", "pec"]),
+        "pmc_upper": (["PMC", "pec", "pec"], ["pec", "Magnetic", "pec"]),
+        "pmc_only": (["pmc"] * 3, ["pmc"] * 3),
+        "pec_pmc_edges": (["pec", "pmc", "pec"], ["pmc", "pec", "pmc"]),
+        "abc": (["pec", "abc", "pec"], ["abc", "pec", "abc"]),
+        "abc_high_all": (["pec"] * 3, ["abc"] * 3),
+        "abc_periodic": (["periodic", "abc", "pec"], ["periodic", "abc", "abc"]),
+        "pml_z": (["pec", "pec", "pml"], ["pec", "pec", "pml"]),
+        "pml_all": (["pml"] * 3, ["pml"] * 3),
+        "pml_pmc": (["pml", "pmc", "pec"], ["pmc", "pml", "pml"]),
+        "open_x": (["pec", "periodic", "pec"], ["periodic", "periodic", "pec"]),
+        "electric": (["electric", "periodic", "pec"], ["electric", "periodic", "pml"]),
+        "pml_low_x_high_y": (["pml", "pec", "pec"], ["pec", "pml", "pec"]),
+        "pml_mixed": (["pml", "pmc", "pec"], ["pmc", "pml", "pml"]),
+    }[name]
+
+
+def test_golden_case_lists_match():
+    assert len(CASES) == 15 and len(PML_CASES) == 4
+    for c in CASES + PML_CASES:
+        _bcs(c)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_apply_bc_matches_reference_vectors(name):
+    """OSolver._apply_bc_to_C on the reference's inputs (random metric
+    Fields, the unmasked curl) gives the reference's C, tL, itA, BC and
+    flags bit for bit."""
+    nx, ny, nz = (int(v) for v in G["dims"])
+    bl, bh = _bcs(name)
+    s = SimpleNamespace(bc_low=list(bl), bc_high=list(bh), Nx=nx, Ny=ny, Nz=nz, N=nx * ny * nz,
+                        dtype=np.float64, activate_abc=False, activate_pml=False,
+                        use_conductivity=False)
+    for k in ("L", "iA", "tL", "itA"):
+        f = OField(nx, ny, nz)
+        f.fromarray(G["in/" + k].copy())
+        setattr(s, k, f)
+    # the unmasked curl as the oracle assembles it (a periodic solver's C)
+    g = OGrid(Nx=nx, Ny=ny, Nz=nz, xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1)
+    s.C = OSolver(g, bc_low=["periodic"] * 3, bc_high=["periodic"] * 3).C
+    s._face_mask = MethodType(OSolver._face_mask, s)
+    OSolver._apply_bc_to_C(s)
+    n3 = 3 * s.N
+    assert np.array_equal(s.C.toarray().astype(np.int64), _golden_C(name, n3))
+    assert np.array_equal(s.tL.toarray(), G[name + "/tL"])
+    assert np.array_equal(s.itA.toarray(), G[name + "/itA"])
+    bc = G[name + "/BC"]
+    if bc.size:
+        assert np.array_equal(s.BC.toarray(), bc)
+    else:
+        assert not hasattr(s, "BC")
+    assert [s.activate_abc, s.activate_pml, s.use_conductivity] == list(G[name + "/flags"])
+
+
+def _mask_from_bits(bits, nx, ny, nz):
+    # the kernels' rule: faces in order xlo, xhi, ylo, yhi, zlo, zhi, the last
+    # face containing the cell decides (masked iff its bit is set)
+    m = np.ones((3, nx, ny, nz), dtype=np.int8)
+    for c in range(3):
+        if c != 0:
+            m[c, 0] = 0 if bits & 1 else 1
+            m[c, -1] = 0 if bits & 2 else 1
+        if c != 1:
+            m[c, :, 0] = 0 if bits & 4 else 1
+            m[c, :, -1] = 0 if bits & 8 else 1
+        if c != 2:
+            m[c, :, :, 0] = 0 if bits & 16 else 1
+            m[c, :, :, -1] = 0 if bits & 32 else 1
+    return np.concatenate([np.reshape(m[c], -1, order="F") for c in range(3)])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_product_face_bits_match_reference_vectors(name):
+    """The product's face masks (column mask of C for PEC/PML faces, row mask
+    for PMC faces, as the kernels apply them) rebuild the reference's C."""
+    from wakis_amd.solver import face_bits
+
+    nx, ny, nz = (int(v) for v in G["dims"])
+    bl, bh = _bcs(name)
+    pec, pmc = face_bits(bl, bh)
+    g = OGrid(Nx=nx, Ny=ny, Nz=nz, xmin=0, xmax=1, ymin=0, ymax=1, zmin=0, zmax=1)
+    C = OSolver(g, bc_low=["periodic"] * 3, bc_high=["periodic"] * 3).C.toarray().astype(np.int64)
+    if pec:
+        C = C * _mask_from_bits(pec, nx, ny, nz)[None, :]
+    if pmc:
+        C = _mask_from_bits(pmc, nx, ny, nz)[:, None] * C
+    assert np.array_equal(C, _golden_C(name, 3 * nx * ny * nz))
+
+
+def _pml_grid():
+    nx, ny, nz, npml = (int(v) for v in G["pml/dims"])
+    return nx, ny, nz, npml
+
+
+@pytest.mark.parametrize("name", PML_CASES)
+def test_oracle_pml_matches_reference_vectors(name):
+    """OSolver._initialize_PML: ieps and sigma equal the reference's, corners
+    included (faces overwrite in the order xlo, ylo, zlo, xhi, yhi, zhi)."""
+    nx, ny, nz, npml = _pml_grid()
+    bl, bh = _bcs(name)
+    s = SimpleNamespace(bc_low=list(bl), bc_high=list(bh), Nx=nx, Ny=ny, Nz=nz, n_pml=npml)
+    s.ieps = OField(nx, ny, nz, use_ones=True) * (1.0 / (2.0 * eps_0))
+    s.sigma = OField(nx, ny, nz, use_ones=True) * 0.25
+    OSolver._initialize_PML(s)
+    assert np.array_equal(s.ieps.toarray(), G["pml/" + name + "/ieps"])
+    assert np.array_equal(s.sigma.toarray(), G["pml/" + name + "/sigma"])
+
+
+@pytest.mark.parametrize("name", PML_CASES)
+def test_product_pml_matches_reference_vectors(name):
+    """The product's material table after PML grading (background eps_r 2,
+    sigma 0.25) gives the reference's ieps and sigma Fields bit for bit."""
+    import wakis_amd as wa
+
+    nx, ny, nz, npml = _pml_grid()
+    bl, bh = _bcs(name)
+    g = wa.GridFIT3D(0, 0.07, 0, 0.08, 0, 0.09, nx, ny, nz)
+    s = wa.SolverFIT3D(g, bc_low=bl, bc_high=bh, bg=[2.0, 1.0, 0.25], n_pml=npml, device=None)
+    assert np.array_equal(s.ieps.toarray(), G["pml/" + name + "/ieps"])
+    assert np.array_equal(s.sigma.toarray(), G["pml/" + name + "/sigma"])
+
+
+def _beam_solver():
+    bx, by, bz = G["beam/axes_x"], G["beam/axes_y"], G["beam/axes_z"]
+    x, y, z = bx[:-1] + np.diff(bx) / 2, by[:-1] + np.diff(by) / 2, bz[:-1] + np.diff(bz) / 2
+    return SimpleNamespace(x=x, y=y, z=z, dx=np.diff(bx), dy=np.diff(by), dz=np.diff(bz),
+                           J=OField(len(x), len(y), len(z)))
+
+
+@pytest.mark.parametrize("beta", [0.8, 0.9, 1.0])
+@pytest.mark.parametrize("ti", ["default", "given"])
+def test_beam_matches_reference_vectors(beta, ti):
+    """OBeam.update and the product's Beam.line write the reference's J_z line
+    bit for bit (injection time 8.548921333333334 sigma_z / v, no sqrt(beta))."""
+    from wakis_amd.sources import Beam
+
+    key = f"beam/b{beta}_ti{ti}"
+    tgiven = None if ti == "default" else 4.0e-11
+    ob = OBeam(xsource=0.004, ysource=-0.003, beta=beta, q=2e-9, sigmaz=0.012, ti=tgiven)
+    pb = Beam(xsource=0.004, ysource=-0.003, beta=beta, q=2e-9, sigmaz=0.012, ti=tgiven)
+    sol = _beam_solver()
+    psol = _beam_solver()
+    nx, ny = len(sol.x), len(sol.y)
+    for n, t in enumerate(G["beam/times"]):
+        ob.update(sol, t)
+        assert np.array_equal(sol.J.toarray(), G[key + "/J"][n])
+        line = pb.line(psol, t)
+        ref = np.reshape(G[key + "/J"][n][2 * nx * ny * len(sol.z):], (nx, ny, -1), order="F")
+        assert np.array_equal(line, ref[pb.ixs, pb.iys, :])
+    ixs, iys, zmin, tref, v = G[key + "/meta"]
+    assert (ob.ixs, ob.iys) == (int(ixs), int(iys)) == (pb.ixs, pb.iys)
+    assert ob.zmin == zmin == pb.zmin and ob.ti == tref == pb.ti and ob.v == v
+
+
+# ---- closed forms ---------------------------------------------------------
+
+def _coarse(solids, bl=("pec",) * 3, bh=("pec",) * 3, bg=(1.0, 1.0), n=6, ext=0.06):
+    g = OGrid(Nx=n, Ny=n, Nz=n, xmin=0, xmax=ext, ymin=0, ymax=ext, zmin=0, zmax=ext)
+    return g, OSolver(g, bc_low=list(bl), bc_high=list(bh), solids=solids, bg=list(bg),
+                      n_pml=2)
+
+
+def _cfl_dt(g, cfln=0.5):
+    return cfln / (c_light * np.sqrt(3.0) / g.dx.min())
+
+
+def test_sibc_permittivity_and_clamp_closed_form():
+    """use_sibc with a conductive solid: eps := sigma*eps_0, so the solid's
+    relaxation time eps/sigma is eps_0 whatever sigma is, and a CFL dt above
+    eps_0 is clamped to eps_0 (to rounding).  Without SIBC the solid keeps
+    eps_r: tau = eps_r*eps_0/sigma."""
+    import wakis_amd as wa
+
+    shape = (6, 6, 6)
+    blk = block_mask(shape, (1, 1, 1), (4, 4, 4))
+    for sigma in (3.0, 5.0e4):
+        g, o = _coarse([(blk, [7.0, 1.0, sigma])])
+        assert _cfl_dt(g) > 1.05 * eps_0  # 10 mm cells: the clamp is active
+        inside = o.ieps.toarray()[: np.prod(shape)].reshape(shape, order="F")[blk]
+        np.testing.assert_allclose(inside, 1.0 / (sigma * eps_0), rtol=2e-16)
+        assert abs(o.dt - eps_0) <= 4e-16 * eps_0
+        pg = wa.GridFIT3D(0, 0.06, 0, 0.06, 0, 0.06, 6, 6, 6, solids=[(blk, [7.0, 1.0, sigma])])
+        p = wa.SolverFIT3D(pg, bc_low=["pec"] * 3, bc_high=["pec"] * 3, use_stl=True,
+                           device=None)
+        assert p.dt == o.dt
+        assert np.array_equal(p.ieps.toarray(), o.ieps.toarray())
+    # no SIBC: tau = eps_r eps_0 / sigma = 0.7 eps_0 (< the CFL dt here)
+    g = OGrid(Nx=6, Ny=6, Nz=6, xmin=0, xmax=0.06, ymin=0, ymax=0.06, zmin=0, zmax=0.06)
+    o = OSolver(g, bc_low=["pec"] * 3, bc_high=["pec"] * 3, solids=[(blk, [7.0, 1.0, 10.0])],
+                use_sibc=False)
+    assert abs(o.dt - 0.7 * eps_0) <= 4e-16 * eps_0
+    pg = wa.GridFIT3D(0, 0.06, 0, 0.06, 0, 0.06, 6, 6, 6, solids=[(blk, [7.0, 1.0, 10.0])])
+    p = wa.SolverFIT3D(pg, bc_low=["pec"] * 3, bc_high=["pec"] * 3, use_stl=True,
+                       use_sibc=False, device=None)
+    assert p.dt == o.dt
+    # a fine grid: CFL dt below every tau, no clamp
+    g2, o2 = _coarse([(blk, [7.0, 1.0, 3.0])], ext=0.006)
+    assert o2.dt == np.float64(_cfl_dt(g2)) or abs(o2.dt / _cfl_dt(g2) - 1) < 4e-16
+
+
+def test_pml_clamp_and_empty_conductive_set():
+    """PML cells have ieps = 1/eps_0 and sigma up to 1 S/m, so their smallest
+    relaxation time is eps_0/1; a conductive background with sigma = 0 and
+    no PML has no conductive cell and the reference's tau.min() raises."""
+    import wakis_amd as wa
+
+    g, o = _coarse(None, bl=("pec", "pec", "pml"), bh=("pec", "pec", "pml"))
+    assert abs(o.dt - eps_0) <= 4e-16 * eps_0
+    with pytest.raises(ValueError):
+        OSolver(g, bc_low=["pec"] * 3, bc_high=["pec"] * 3, bg=[1.0, 1.0, 0.0])
+    pg = wa.GridFIT3D(0, 0.06, 0, 0.06, 0, 0.06, 6, 6, 6)
+    with pytest.raises(ValueError):
+        wa.SolverFIT3D(pg, bc_low=["pec"] * 3, bc_high=["pec"] * 3, bg=[1.0, 1.0, 0.0],
+                       device=None)
