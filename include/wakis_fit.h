@@ -43,8 +43,11 @@ enum {
 };
 /* beam line source of J_z at (ixs, iys, :) */
 enum { FIT_BEAM_OFF = 0, FIT_BEAM_VALUES = 1, FIT_BEAM_GAUSS = 2 };
-/* step kernels */
-enum { FIT_KERNEL_AUTO = 0, FIT_KERNEL_SPLIT = 1, FIT_KERNEL_FUSED = 2 };
+/* step kernels: AUTO = the one-step fused sweep when its second field
+ * buffers fit, else the split pair; FUSED = the one-step fused sweep;
+ * FUSED2 = two-step fused sweeps for pairs of steps where they apply (see
+ * fit_step), the one-step fused sweep for the other steps */
+enum { FIT_KERNEL_AUTO = 0, FIT_KERNEL_SPLIT = 1, FIT_KERNEL_FUSED = 2, FIT_KERNEL_FUSED2 = 3 };
 
 /* error codes */
 enum {
@@ -175,13 +178,25 @@ int fit_monitor_read(fit_ctx *ctx, int32_t f, int32_t comp, double *host);
 
 /* Advance nsteps leapfrog steps (wakis/solverFIT3D.py:504-525):
  *   H -= dt*M_H E;  E += dt*(M_E H - ieps*J);  J = sigma*E if conductive.
- * conductive != 0 selects J := sigma*E after each step.  Asynchronous. */
+ * conductive != 0 selects J := sigma*E after each step.  Asynchronous.
+ * With FIT_KERNEL_FUSED2, pairs of steps run as ONE
+ * two-step sweep (E and H read and written once per two steps) on FP32
+ * contexts whose nx is a multiple of 64 and ny of 8 (fields below 2^32
+ * bytes), when J is 0 or sigma*E
+ * (not FIT_J_BUFFER) and no field-monitor update is staged for the pair;
+ * every other step runs the one-step sweep.  Each step's beam value, J and
+ * E_z probe sample are those of the one-step sequence. */
 int fit_step(fit_ctx *ctx, int32_t nsteps, int32_t conductive);
+/* Launches of the step kernels since the context was created: counts[0]
+ * one-step launches (a fused sweep, or a split H/E pair), counts[1] two-step
+ * sweeps.  No counterpart in the reference. */
+int fit_launch_counts(fit_ctx *ctx, int64_t *counts);
 /* fit_step timed with HIP events on the context stream (synchronises).
  * per_kernel = 0: ms[0] = elapsed time of the whole launch sequence.
  * per_kernel = 1: events around every launch; ms[1] = summed time of the
- * first kernel of a step (H update, or the fused sweep), ms[2] = summed time
- * of the second (E update; 0 for the fused sweep); ms[0] = total.
+ * first kernel of a step (H update, or the fused sweep, one- or two-step),
+ * ms[2] = summed time of the second (E update; 0 for the fused sweeps);
+ * ms[0] = the whole launch sequence, as with per_kernel = 0.
  * No counterpart in the reference (it has no timing facility beyond
  * time.time() deltas, wakis/routines.py:437, 484). */
 int fit_step_timed(fit_ctx *ctx, int32_t nsteps, int32_t conductive, int32_t per_kernel,

@@ -1,0 +1,116 @@
+"""The two-step fused sweep (two leapfrog steps per launch, FP32; used by
+fit_step for pairs of steps on grids whose nx is a multiple of 64 and ny of
+8) against the FP32 oracle (relative L2 <= 1e-4, the FP32 bar of the other
+parity tests) and against the one-step fused sweep on the same context setup
+(<= 1e-5: the two kernels evaluate the same per-cell expressions).  Cases:
+every boundary kind, the helical x/y wrap across tile and grid edges, odd
+step counts (a single step after the pairs), odd plane counts, non-uniform
+spacing, conductive solids with PML (J = sigma*E in both steps), and the
+whole FP32 wakesolve (in-kernel beam and E_z probe of both steps)."""
+
+import numpy as np
+import pytest
+
+from helpers import block_mask, make_pair, rel
+
+pytestmark = pytest.mark.gpu
+
+TOL_ORACLE = 1e-4
+TOL_KERNEL = 1e-5
+
+CASES = [
+    ("pec", (64, 16, 24), ["pec"] * 3, ["pec"] * 3, False, False),
+    ("periodic", (64, 8, 20), ["periodic"] * 3, ["periodic"] * 3, False, False),
+    ("mixed_odd_nz", (128, 16, 17), ["periodic", "pec", "pec"], ["periodic", "pmc", "pml"],
+     False, False),
+    ("pmc", (64, 24, 13), ["pmc", "pec", "magnetic"], ["pec", "pmc", "pec"], False, False),
+    ("pml_conductive", (64, 16, 40), ["pec", "pec", "pml"], ["pec", "pec", "pml"], True, False),
+    ("nonuniform_abc_open", (64, 8, 30), ["pec", "periodic", "pec"],
+     ["periodic", "periodic", "pec"], False, True),
+    # tall grids: lean mid chunks between 4-plane face chunks
+    ("periodic_lean", (64, 16, 48), ["periodic"] * 3, ["periodic"] * 3, False, False),
+    ("pmc_lean_nonuniform", (128, 8, 45), ["pmc", "pec", "magnetic"], ["pec", "pmc", "pml"],
+     False, True),
+]
+
+
+def _solids(shape):
+    Nx, Ny, Nz = shape
+    return [
+        (block_mask(shape, (2, 2, 3), (Nx - 2, Ny - 2, Nz - 3)), "vacuum"),
+        (block_mask(shape, (10, 1, 4), (30, 6, 9)), [4.0, 1.5]),
+        (block_mask(shape, (20, 3, 5), (50, 7, 12)), [2.0, 1.0, 50.0]),
+        (block_mask(shape, (5, 4, 6), (9, 8, 10)), "pec"),
+    ]
+
+
+def _pair(shape, bl, bh, solids, nonuniform, kernel):
+    osv, psv = make_pair(*shape, bl, bh, dtype=np.float32, solids=_solids(shape) if solids else (),
+                         bg="pec" if solids else (1.0, 1.0), n_pml=3, nonuniform=nonuniform,
+                         init=False, kernel=kernel, seed=3)
+    rng = np.random.default_rng(17)
+    for name in ("E", "H"):  # J stays 0 (J mode zero / sigma*E)
+        a = rng.standard_normal(3 * osv.N).astype(np.float32)
+        getattr(osv, name).fromarray(a)
+        getattr(psv, name).fromarray(a)
+    return osv, psv
+
+
+@pytest.mark.parametrize("name,shape,bl,bh,solids,nonuniform", CASES, ids=[c[0] for c in CASES])
+def test_two_step_sweep_matches_oracle_and_one_step(name, shape, bl, bh, solids, nonuniform):
+    nsteps = 7  # three two-step launches and one single step
+    print(f"[case] two-step {name} {shape}", flush=True)
+    osv, p2 = _pair(shape, bl, bh, solids, nonuniform, "fused2")
+    _, p1 = _pair(shape, bl, bh, solids, nonuniform, "fused")
+    from wakis_amd import _lib
+
+    assert p2.ctx.kernel_in_use() == _lib.KERNEL_FUSED2
+    assert p1.ctx.kernel_in_use() == _lib.KERNEL_FUSED
+    for _ in range(nsteps):
+        osv.one_step()
+    p2._advance(nsteps)
+    p1._advance(nsteps)
+    n1, n2 = p2.ctx.launch_counts()
+    assert (n1, n2) == (1, 3), (n1, n2)
+    assert p1.ctx.launch_counts() == (nsteps, 0)
+    errs = {}
+    for f in "EHJ":
+        o = getattr(osv, f).toarray()
+        a2 = getattr(p2, f).toarray()
+        a1 = getattr(p1, f).toarray()
+        errs[f] = (rel(a2, o), rel(a2, a1))
+    print(errs, flush=True)
+    for f, (eo, ek) in errs.items():
+        assert eo <= TOL_ORACLE and ek <= TOL_KERNEL, (f, eo, ek)
+
+
+def test_two_step_wakesolve_matches_one_step():
+    """The FP32 wakesolve of a conductive cavity with z PML, beam on the axis
+    and the 3 x 3 E_z probe: fields, WP and Z of the two-step path against
+    the one-step path (and the fields against the FP32 oracle)."""
+    import wakis_amd as wa
+    from wakis_amd import _lib
+
+    shape = (64, 16, 48)
+    Nx, Ny, Nz = shape
+    wall = ~block_mask(shape, (4, 3, 0), (Nx - 4, Ny - 3, Nz))
+    kw = dict(wakelength=0.12, q=1e-9, sigmaz=0.01, beta=1.0)
+    res = {}
+    for kernel in ("fused2", "fused"):
+        osv, psv = make_pair(*shape, ["pec", "pec", "pml"], ["pec", "pec", "pml"],
+                             dtype=np.float32, solids=[(wall, [1.0, 1.0, 1e3])], n_pml=5,
+                             init=False, kernel=kernel, wake_kw=kw)
+        print(f"[case] two-step wakesolve kernel={kernel}", flush=True)
+        psv.wakesolve(compute_plane="longitudinal")
+        res[kernel] = (psv, osv)
+    p2, osv = res["fused2"]
+    p1, _ = res["fused"]
+    assert p2.ctx.launch_counts()[1] > p2.Nt // 4
+    osv.wakesolve(compute_plane="longitudinal")
+    errs = {"E": rel(p2.E.toarray(), p1.E.toarray()), "H": rel(p2.H.toarray(), p1.H.toarray()),
+            "WP": rel(p2.wake.WP, p1.wake.WP), "Z": rel(p2.wake.Z, p1.wake.Z),
+            "E_oracle": rel(p2.E.toarray(), osv.E.toarray()),
+            "WP_oracle": rel(p2.wake.WP, osv.wake.WP)}
+    print(errs, flush=True)
+    assert max(errs["E"], errs["H"], errs["WP"], errs["Z"]) <= TOL_KERNEL, errs
+    assert errs["E_oracle"] <= TOL_ORACLE and errs["WP_oracle"] <= TOL_ORACLE, errs

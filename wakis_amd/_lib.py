@@ -18,7 +18,7 @@ FIT_F64, FIT_F32 = 0, 1
 FIELD_E, FIELD_H, FIELD_J = 0, 1, 2
 J_ZERO, J_SIGMA_E, J_BUFFER = 0, 1, 2
 BEAM_OFF, BEAM_VALUES, BEAM_GAUSS = 0, 1, 2
-KERNEL_AUTO, KERNEL_SPLIT, KERNEL_FUSED = 0, 1, 2
+KERNEL_AUTO, KERNEL_SPLIT, KERNEL_FUSED, KERNEL_FUSED2 = 0, 1, 2, 3
 NTILE = 12
 
 
@@ -78,6 +78,7 @@ SIGNATURES = {
     "fit_get_step_index": [_P, C.POINTER(_I64)],
     "fit_set_kernel": [_P, _I32],
     "fit_kernel_in_use": [_P, _PI32],
+    "fit_launch_counts": [_P, C.POINTER(_I64)],
     "fit_set_tile": [_P, _I32],
     "fit_set_chunk_rsrc": [_P, _I32],
     "fit_get_tile": [_P, _PI32, _PI32, _PI32, _PI32],
@@ -307,6 +308,12 @@ class Context:
         v = C.c_int32()
         check(self.lib.fit_kernel_in_use(self.h, C.byref(v)), "fit_kernel_in_use")
         return v.value
+
+    def launch_counts(self):
+        """(one-step launches, two-step launches) since the context was created."""
+        v = (C.c_int64 * 2)()
+        check(self.lib.fit_launch_counts(self.h, v), "fit_launch_counts")
+        return int(v[0]), int(v[1])
 
     def set_tile(self, cfg: int):
         check(self.lib.fit_set_tile(self.h, int(cfg)), "fit_set_tile")

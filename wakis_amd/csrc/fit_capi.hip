@@ -15,6 +15,7 @@
 #include "../../include/wakis_fit.h"
 #include "fit_aux.hpp"
 #include "fit_fused.hpp"
+#include "fit_fused2.hpp"
 #include "fit_sweep.hpp"
 #include "fit_split.hpp"
 
@@ -78,6 +79,9 @@ struct fit_ctx {
   double *mon_ph = nullptr;
   int64_t mon_ph_cap = 0;  // rows
   int64_t mon_ph_steps = 0;
+  // launches of the step kernels so far: [0] one-step (fused or split pair),
+  // [1] two-step fused sweeps
+  int64_t launches[2] = {0, 0};
 };
 
 namespace {
@@ -245,8 +249,45 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
                        c->step, nsteps, (T *)c->btab);
     HIPCHK(hipGetLastError());
   }
-  for (int s = 0; s < nsteps; ++s) {
+  // two-step sweeps for pairs of steps where they apply: the fused kernel
+  // selected (or AUTO), FP32 full tiles, J zero or sigma*E, and no field
+  // monitor update staged between the two steps
+  const bool pairs = fused && c->kernel == FIT_KERNEL_FUSED2;
+  for (int s = 0; s < nsteps;) {
     fit::StepArgs<T> a = make_args<T>(c);
+    if (pairs && s + 1 < nsteps && !(c->mon && s < c->mon_ph_steps) &&
+        fit::fused2_supported<T>(a)) {
+      const T *jl = (const T *)c->jline;
+      const T *beam1 = nullptr, *beam2 = nullptr;
+      if (gauss) {
+        beam1 = (const T *)c->btab + (int64_t)s * c->nz;
+        beam2 = beam1 + c->nz;
+      } else if (c->beam_mode == FIT_BEAM_VALUES) {
+        beam1 = jl;
+        beam2 = conductive ? nullptr : jl;  // a conductive step consumes the line
+      }
+      const bool js1 = c->jmode == FIT_J_SIGMA_E;
+      const bool js2 = conductive ? true : js1;
+      if (ev) HIPCHK(hipEventRecord(ev[3 * s], c->stream));
+      if (!fit::launch_fused2<T>(a, (T *)c->E_alt, (T *)c->H_alt, beam1, beam2, js1, js2,
+                                 c->stream))
+        return fail(FIT_ESTATE, "two-step sweep not launched");
+      std::swap(c->E, c->E_alt);
+      std::swap(c->H, c->H_alt);
+      if (ev) {
+        HIPCHK(hipEventRecord(ev[3 * s + 1], c->stream));
+        for (int q = 3 * s + 2; q < 3 * s + 6; ++q) HIPCHK(hipEventRecord(ev[q], c->stream));
+      }
+      HIPCHK(hipGetLastError());
+      c->launches[1] += 1;
+      c->step += 2;
+      if (conductive) {
+        c->jmode = FIT_J_SIGMA_E;
+        if (c->beam_mode == FIT_BEAM_VALUES) c->beam_mode = FIT_BEAM_OFF;
+      }
+      s += 2;
+      continue;
+    }
     if (gauss) {
       a.beam_mode = FIT_BEAM_VALUES;
       a.jline = (const T *)c->btab + (int64_t)s * c->nz;
@@ -270,11 +311,13 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
     if (ev) HIPCHK(hipEventRecord(ev[3 * s + 2], c->stream));
     if (c->mon && s < c->mon_ph_steps) monitor_launch<T>(c, c->mon_ph + (int64_t)s * 2 * c->mon_nf);
     HIPCHK(hipGetLastError());
+    c->launches[0] += 1;
     c->step += 1;
     if (conductive) {
       c->jmode = FIT_J_SIGMA_E;
       if (c->beam_mode == FIT_BEAM_VALUES) c->beam_mode = FIT_BEAM_OFF;
     }
+    s += 1;
   }
   c->mon_ph_steps = 0;  // the staged monitor phases belong to this call
   return 0;
@@ -829,11 +872,16 @@ int fit_step_timed(fit_ctx *c, int32_t nsteps, int32_t conductive, int32_t per_k
     (void)hipEventDestroy(e1);
     return rc;
   }
-  std::vector<hipEvent_t> ev((size_t)3 * nsteps);
+  // one more event before the call: ms[0] spans the whole launch sequence
+  // (beam table included), the same window as per_kernel = 0
+  std::vector<hipEvent_t> ev((size_t)3 * nsteps + 1);
   for (auto &e : ev) HIPCHK(hipEventCreate(&e));
-  int rc = c->dtype == FIT_F64 ? step_impl<double>(c, nsteps, conductive, ev.data())
-                               : step_impl<float>(c, nsteps, conductive, ev.data());
-  if (!rc && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(FIT_EHIP, "event sync");
+  int rc = 0;
+  if (hipEventRecord(ev.back(), c->stream) != hipSuccess) rc = fail(FIT_EHIP, "event record");
+  if (!rc)
+    rc = c->dtype == FIT_F64 ? step_impl<double>(c, nsteps, conductive, ev.data())
+                             : step_impl<float>(c, nsteps, conductive, ev.data());
+  if (!rc && hipEventSynchronize(ev[3 * nsteps - 1]) != hipSuccess) rc = fail(FIT_EHIP, "event sync");
   for (int s = 0; !rc && s < nsteps; ++s) {
     float a = 0, b = 0;
     if (hipEventElapsedTime(&a, ev[3 * s], ev[3 * s + 1]) != hipSuccess ||
@@ -844,7 +892,10 @@ int fit_step_timed(fit_ctx *c, int32_t nsteps, int32_t conductive, int32_t per_k
     ms[1] += a;
     ms[2] += b;
   }
-  ms[0] = ms[1] + ms[2];
+  float tot = 0;
+  if (!rc && hipEventElapsedTime(&tot, ev.back(), ev[3 * nsteps - 1]) != hipSuccess)
+    rc = fail(FIT_EHIP, "elapsed");
+  ms[0] = tot;
   for (auto &e : ev) (void)hipEventDestroy(e);
   return rc;
 }
@@ -982,7 +1033,7 @@ int fit_get_step_index(fit_ctx *c, int64_t *step) {
 
 int fit_set_kernel(fit_ctx *c, int32_t kernel) {
   if (!c) return fail(FIT_EINVAL, "null context");
-  if (kernel < FIT_KERNEL_AUTO || kernel > FIT_KERNEL_FUSED) return fail(FIT_EINVAL, "bad kernel id");
+  if (kernel < FIT_KERNEL_AUTO || kernel > FIT_KERNEL_FUSED2) return fail(FIT_EINVAL, "bad kernel id");
   c->kernel = kernel;
   if (kernel == FIT_KERNEL_SPLIT) {
     // the second buffers are no longer needed
@@ -1036,6 +1087,18 @@ int fit_kernel_in_use(fit_ctx *c, int32_t *kernel) {
   bool fused = false;
   if (int rc = want_fused(c, &fused)) return rc;
   *kernel = fused ? FIT_KERNEL_FUSED : FIT_KERNEL_SPLIT;
+  if (fused && c->kernel == FIT_KERNEL_FUSED2 && c->dtype == FIT_F32 && c->geo_set &&
+      c->mat_set && c->dt_set) {
+    if (int rc = set_device(c)) return rc;
+    if (fit::fused2_supported<float>(make_args<float>(c))) *kernel = FIT_KERNEL_FUSED2;
+  }
+  return 0;
+}
+
+int fit_launch_counts(fit_ctx *c, int64_t *counts) {
+  if (!c || !counts) return fail(FIT_EINVAL, "null argument");
+  counts[0] = c->launches[0];
+  counts[1] = c->launches[1];
   return 0;
 }
 

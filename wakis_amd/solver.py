@@ -111,14 +111,21 @@ class SolverFIT3D(RoutinesMixin):
                  bc_high=("Periodic", "Periodic", "Periodic"),
                  use_stl=False, use_conductors=False, use_gpu=True, use_mpi=False,
                  use_sibc=True, fmax=1e9, dtype=np.float64, n_pml=10, bg=(1.0, 1.0),
-                 verbose=1, device=0, kernel="auto", tile=None, precision="reference"):
+                 verbose=1, device=0, kernel="auto", tile=None, precision="reference",
+                 pml_profile="geomspace", pml_sigma_max=1.0):
         """wakis/solverFIT3D.py:40-376.  precision (this build's addition):
         "reference" rounds dt and the material values to `dtype` as the
         reference does; "mixed" keeps dt, the material values and the beam
         time in FP64 (the device coefficient table dt/mu, dt/eps, sigma is
         computed in FP64 and stored in `dtype`), so an FP32 run shares the
         FP64 run's time axis and differs from it only by the field
-        arithmetic and storage."""
+        arithmetic and storage.
+
+        pml_profile (this build's addition): "geomspace" is the reference's
+        grading (wakis/boundaries.py:190-305); "cubic" grades sigma as
+        pml_sigma_max * (d / n_pml)**3 at depth d = 1 .. n_pml into the layer
+        (d = n_pml at the outer cell).  The default peak, 1 S/m, is the
+        reference's outer value, so the relaxation clamp of dt is unchanged."""
         t0 = time.time()
         if use_mpi:
             raise NotImplementedError(
@@ -192,6 +199,10 @@ class SolverFIT3D(RoutinesMixin):
                                   cdtype)
         if use_stl:
             self._apply_stl_materials()
+        if pml_profile not in ("geomspace", "cubic"):
+            raise ValueError("pml_profile: 'geomspace' or 'cubic'")
+        self.pml_profile = pml_profile
+        self.pml_sigma_max = float(pml_sigma_max)
         if self.activate_pml:
             self.n_pml = n_pml
             self._initialize_PML()
@@ -265,7 +276,7 @@ class SolverFIT3D(RoutinesMixin):
 
     def set_kernel(self, kernel="auto"):
         k = {"auto": _lib.KERNEL_AUTO, "split": _lib.KERNEL_SPLIT,
-             "fused": _lib.KERNEL_FUSED}[kernel]
+             "fused": _lib.KERNEL_FUSED, "fused2": _lib.KERNEL_FUSED2}[kernel]
         self.ctx.set_kernel(k)
         self.kernel = kernel
 
@@ -322,12 +333,21 @@ class SolverFIT3D(RoutinesMixin):
 
             self._mt.apply(region, fn)
 
-    def _initialize_PML(self):
-        """wakis/boundaries.py:190-305: ieps := 1/eps_0 and a geomspace sigma
-        (1 S/m at the outer cell down to 5e-3) on n_pml planes per PML face,
-        faces in the order xlo, ylo, zlo, xhi, yhi, zhi."""
+    def pml_sigma_profile(self, side):
+        """sigma of the n_pml planes of a PML face, plane 0 first (the outer
+        cell for a low face, the inner cell for a high face)."""
         n = self.n_pml
-        lo, hi = 5.0e-3, 1.0
+        if self.pml_profile == "geomspace":
+            lo, hi = 5.0e-3, 1.0
+            return np.geomspace(hi, lo, n) if side == "lo" else np.geomspace(lo, hi, n)
+        depth = np.arange(n, 0, -1) if side == "lo" else np.arange(1, n + 1)
+        return self.pml_sigma_max * (depth / n) ** 3
+
+    def _initialize_PML(self):
+        """wakis/boundaries.py:190-305: ieps := 1/eps_0 and a graded sigma on
+        n_pml planes per PML face (geomspace: 1 S/m at the outer cell down to
+        5e-3; or cubic), faces in the order xlo, ylo, zlo, xhi, yhi, zhi."""
+        n = self.n_pml
         ie = _cast(self.cdtype, 1 / eps_0)
         dims = (self.Nx, self.Ny, self.Nz)
         order = [(0, "lo"), (1, "lo"), (2, "lo"), (0, "hi"), (1, "hi"), (2, "hi")]
@@ -335,7 +355,7 @@ class SolverFIT3D(RoutinesMixin):
             bc = self.bc_low[ax] if side == "lo" else self.bc_high[ax]
             if bc.lower() != "pml":
                 continue
-            prof = np.geomspace(hi, lo, n) if side == "lo" else np.geomspace(lo, hi, n)
+            prof = self.pml_sigma_profile(side)
             for p in range(n):
                 plane = p if side == "lo" else dims[ax] - n + p
                 sig = _cast(self.cdtype, 0.0 + 1 * prof[p])
