@@ -324,22 +324,25 @@ def test_te101_resonator_device():
     assert f[half][np.argmax(F[half])] == pytest.approx(2.121e8, rel=0.05)
 
 
+@pytest.mark.parametrize("profile", ["geomspace", "cubic"])
 @pytest.mark.parametrize("eps_r,sigma", [(1.0, 0.0), (3.0, 1.0), (1.0, 1.0)],
                          ids=["vacuum", "resistive", "high-resistivity"])
-def test_pml_reflection_device(eps_r, sigma):
+def test_pml_reflection_device(eps_r, sigma, profile):
     """tests/test_005_pml_boundaries.py:19-336 on the device (FP32): a plane
     wave of amplitude 100 run into a 30-cell z PML through a background of
     (eps_r, sigma); after the round trip the field left on the axis is below
-    10 % of the amplitude, for H and for E / (mu_0 c)."""
+    10 % of the amplitude, for H and for E / (mu_0 c).  Both sigma gradings:
+    the reference's geomspace and the cubic profile (1 S/m peak)."""
     import wakis_amd as wa
     from scipy.constants import c, mu_0
 
     Nx, Ny, Nz = 20, 20, 200
-    print(f"[case] PML reflection eps_r={eps_r} sigma={sigma}", flush=True)
+    print(f"[case] PML reflection eps_r={eps_r} sigma={sigma} {profile}", flush=True)
     grid = wa.GridFIT3D(-1.0, 1.0, -1.0, 1.0, 0.0, 1.0, Nx, Ny, Nz)
     solver = wa.SolverFIT3D(grid, use_stl=False, bg=[eps_r, 1.0, sigma],
                             bc_low=["periodic", "periodic", "pec"],
-                            bc_high=["periodic", "periodic", "pml"], n_pml=30, dtype=np.float32)
+                            bc_high=["periodic", "periodic", "pml"], n_pml=30, dtype=np.float32,
+                            pml_profile=profile)
     amplitude = 100.0
     f = 15 / ((solver.z.max() - solver.z.min()) / c)
     pw = wa.PlaneWave(xs=slice(0, Nx), ys=slice(0, Ny), zs=0, beta=1.0, amplitude=amplitude,

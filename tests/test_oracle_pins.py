@@ -247,3 +247,40 @@ def test_pml_clamp_and_empty_conductive_set():
     with pytest.raises(ValueError):
         wa.SolverFIT3D(pg, bc_low=["pec"] * 3, bc_high=["pec"] * 3, bg=[1.0, 1.0, 0.0],
                        device=None)
+
+
+def test_cubic_pml_profile_closed_form():
+    """pml_profile="cubic" (an extension: the reference grades with
+    geomspace only): sigma = 1 S/m * (d / n)**3 at depth d = 1 .. n into each
+    PML face (d = n at the outer cell), ieps = 1/eps_0 there, the later face
+    overwriting corners as in the reference's face order; dt is clamped to
+    eps_0 / 1 S/m exactly as with geomspace."""
+    import wakis_amd as wa
+
+    nx, ny, nz, n = 7, 8, 12, 4
+    g = wa.GridFIT3D(0, 0.07, 0, 0.08, 0, 0.12, nx, ny, nz)
+    s = wa.SolverFIT3D(g, bc_low=["pml", "pec", "pml"], bc_high=["pec", "pec", "pml"],
+                       n_pml=n, device=None, pml_profile="cubic", bg=[2.0, 1.0])
+    N = nx * ny * nz
+    sig = s.sigma.toarray()[2 * N:].reshape((nx, ny, nz), order="F")
+    ie = s.ieps.toarray()[:N].reshape((nx, ny, nz), order="F")
+    want = np.zeros((nx, ny, nz))
+    depth = lambda d: (d / n) ** 3
+    for i in range(n):
+        want[i] = depth(n - i)              # xlo: plane 0 is the outer cell
+    for k in range(n):
+        want[:, :, k] = depth(n - k)        # zlo (after xlo)
+        want[:, :, nz - n + k] = depth(k + 1)  # zhi (last)
+    assert np.array_equal(sig, want)
+    pml = np.zeros((nx, ny, nz), bool)
+    pml[:n] = True
+    pml[:, :, :n] = True
+    pml[:, :, nz - n:] = True
+    assert np.array_equal(ie[pml], np.full(pml.sum(), 1 / eps_0))
+    assert np.array_equal(ie[~pml], np.full((~pml).sum(), 1 / (2.0 * eps_0)))
+    geo = wa.SolverFIT3D(g, bc_low=["pml", "pec", "pml"], bc_high=["pec", "pec", "pml"],
+                         n_pml=n, device=None, bg=[2.0, 1.0])
+    assert s.dt == geo.dt
+    with pytest.raises(ValueError):
+        wa.SolverFIT3D(g, bc_low=["pml"] * 3, bc_high=["pml"] * 3, n_pml=n, device=None,
+                       pml_profile="linear")

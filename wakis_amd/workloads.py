@@ -57,12 +57,13 @@ def pec_cubic_cavity(nx=64, ny=64, nz=192):
     )
 
 
-def lossy_pillbox(nx=256, ny=256, nz=768):
+def lossy_pillbox(nx=256, ny=256, nz=768, pml_profile="cubic"):
     """Pillbox cavity r = 60 mm, length 80 mm, beam pipe r = 20 mm, staircased
     in a 120 x 120 mm transverse box with dz = dx (z = +-180 mm at 768 cells).
     Walls are a conductor sigma = 1e5 S/m given as a solid (so, as in the
     reference's STL path with use_sibc, eps = sigma*eps_0 there); 10-cell PML
-    (the reference's geomspace grading) on both z ends, PEC on x and y;
+    on both z ends with a cubic sigma profile peaking at 1 S/m (or the
+    reference's geomspace grading, pml_profile="geomspace"), PEC on x and y;
     bunch sigma_z = 10 mm, beta = 1, wakelength 5 m.  CFL factor 0.5 (the
     reference's default): the conductivity term J = sigma*E is explicit
     (wakis/solverFIT3D.py:514-525), and at the CFL limit the leapfrog mode of
@@ -79,7 +80,7 @@ def lossy_pillbox(nx=256, ny=256, nz=768):
         name=f"lossy_pillbox_{nx}x{ny}x{nz}",
         grid=dict(x=x, y=y, z=z, solids=[(~vac, [1.0, 1.0, 1e5])]),
         solver=dict(bc_low=["pec", "pec", "pml"], bc_high=["pec", "pec", "pml"], bg="vacuum",
-                    use_stl=True, n_pml=10, cfln=0.5),
+                    use_stl=True, n_pml=10, cfln=0.5, pml_profile=pml_profile),
         wake=dict(q=1e-9, sigmaz=10e-3, beta=1.0, wakelength=5.0),
     )
 
