@@ -43,10 +43,10 @@ enum {
 };
 /* beam line source of J_z at (ixs, iys, :) */
 enum { FIT_BEAM_OFF = 0, FIT_BEAM_VALUES = 1, FIT_BEAM_GAUSS = 2 };
-/* step kernels: AUTO = the one-step fused sweep when its second field
- * buffers fit, else the split pair; FUSED = the one-step fused sweep;
- * FUSED2 = two-step fused sweeps for pairs of steps where they apply (see
- * fit_step), the one-step fused sweep for the other steps */
+/* step kernels: AUTO = the fused sweeps when their second field buffers fit
+ * (two-step sweeps for pairs of steps where they apply, see fit_step; the
+ * one-step sweep for the other steps), else the split pair; FUSED = the
+ * one-step fused sweep only; FUSED2 = as AUTO, but never the split pair */
 enum { FIT_KERNEL_AUTO = 0, FIT_KERNEL_SPLIT = 1, FIT_KERNEL_FUSED = 2, FIT_KERNEL_FUSED2 = 3 };
 
 /* error codes */
@@ -179,7 +179,7 @@ int fit_monitor_read(fit_ctx *ctx, int32_t f, int32_t comp, double *host);
 /* Advance nsteps leapfrog steps (wakis/solverFIT3D.py:504-525):
  *   H -= dt*M_H E;  E += dt*(M_E H - ieps*J);  J = sigma*E if conductive.
  * conductive != 0 selects J := sigma*E after each step.  Asynchronous.
- * With FIT_KERNEL_FUSED2, pairs of steps run as ONE
+ * With FIT_KERNEL_AUTO or FIT_KERNEL_FUSED2, pairs of steps run as ONE
  * two-step sweep (E and H read and written once per two steps) on FP32
  * contexts whose nx is a multiple of 64 and ny of 8 (fields below 2^32
  * bytes), when J is 0 or sigma*E

@@ -252,7 +252,7 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
   // two-step sweeps for pairs of steps where they apply: the fused kernel
   // selected (or AUTO), FP32 full tiles, J zero or sigma*E, and no field
   // monitor update staged between the two steps
-  const bool pairs = fused && c->kernel == FIT_KERNEL_FUSED2;
+  const bool pairs = fused && (c->kernel == FIT_KERNEL_AUTO || c->kernel == FIT_KERNEL_FUSED2);
   for (int s = 0; s < nsteps;) {
     fit::StepArgs<T> a = make_args<T>(c);
     if (pairs && s + 1 < nsteps && !(c->mon && s < c->mon_ph_steps) &&
@@ -1087,8 +1087,8 @@ int fit_kernel_in_use(fit_ctx *c, int32_t *kernel) {
   bool fused = false;
   if (int rc = want_fused(c, &fused)) return rc;
   *kernel = fused ? FIT_KERNEL_FUSED : FIT_KERNEL_SPLIT;
-  if (fused && c->kernel == FIT_KERNEL_FUSED2 && c->dtype == FIT_F32 && c->geo_set &&
-      c->mat_set && c->dt_set) {
+  if (fused && (c->kernel == FIT_KERNEL_AUTO || c->kernel == FIT_KERNEL_FUSED2) &&
+      c->dtype == FIT_F32 && c->geo_set && c->mat_set && c->dt_set) {
     if (int rc = set_device(c)) return rc;
     if (fit::fused2_supported<float>(make_args<float>(c))) *kernel = FIT_KERNEL_FUSED2;
   }

@@ -24,11 +24,11 @@
 // steps.  Every value a tile stores comes from window cells the tile
 // computed itself, so no workgroup waits for another.
 //
-// Each thread owns one tile cell (window (2 + t % TX, 2 + t / TX)); threads
-// t < NR also own one ring cell of the window's outer two rows/columns.
-// TX = 64: a tile row is one wave, so row-indexed metrics are scalar loads.
-// LDS: four 2-plane rings (u0, w1, u1, w2) of 3 x (TX+4)(TY+4) values; the
-// material table follows in dynamic LDS.
+// One thread per window cell ((TX+4)(TY+4) threads): every thread runs the
+// same code on its cell, the phases its cell takes part in predicated by its
+// position, so each thread holds the state of one cell only.  LDS: four
+// 2-plane rings (u0, w1, u1, w2) of 3 x (TX+4)(TY+4) values; the material
+// table follows in dynamic LDS.
 #pragma once
 
 #include <type_traits>
@@ -39,11 +39,9 @@ namespace fit {
 
 template <int TX, int TY>
 struct Win2 {
-  static constexpr int WW = TX + 4;           // window row length
-  static constexpr int WH = TY + 4;           // window rows
-  static constexpr int WC = WW * WH;          // window cells per plane
-  static constexpr int NT = TX * TY;          // threads (one tile cell each)
-  static constexpr int NR = 4 * WW + 4 * TY;  // ring cells (WC - NT)
+  static constexpr int WW = TX + 4;   // window row length
+  static constexpr int WH = TY + 4;   // window rows
+  static constexpr int WC = WW * WH;  // window cells per plane = threads
 };
 
 template <typename T, int TX, int TY>
@@ -64,23 +62,6 @@ struct Fused2Args {
   int ntx, nty, kc, nchunk, nb, nremap;
   int kh = 0, nmid = 0;  // z face chunks of kh planes (chunk2_span)
 };
-
-// Window coordinates of ring cell r (rows 0, 1, TY+2, TY+3, then columns
-// 0, 1, TX+2, TX+3 of rows 2 .. TY+1).
-template <int TX, int TY>
-__device__ __forceinline__ void ring_cell(int r, int &wi, int &wj) {
-  constexpr int WW = TX + 4;
-  if (r < 4 * WW) {
-    const int rr = r / WW;
-    wi = r - rr * WW;
-    wj = rr < 2 ? rr : TY + rr;
-  } else {
-    const int u = r - 4 * WW;
-    const int cc = u / TY;
-    wj = 2 + (u - cc * TY);
-    wi = cc < 2 ? cc : TX + cc;
-  }
-}
 
 // Planes [k0, k1) of z chunk c: with kh > 0 the first nmid chunks take kc
 // planes each from plane kh (the last ending at nz - kh), chunk nmid is
@@ -104,18 +85,15 @@ __host__ __device__ inline void chunk2_span(const Fused2Args<T> &f, int c, int &
   }
 }
 
-// One tile of a chunk.  LEAN: every plane the chunk touches (its window
-// cells' true planes included) lies inside the grid and away from the z
-// faces, so validity is constant and the boundary masks reduce to per-cell
-// 0/1 factors fixed over the march (the same expressions, with fewer
-// instructions per plane).
+// One tile of a chunk.  LEAN: every plane the chunk's outputs depend on (its
+// window cells' true planes) lies in 1 .. nz-2, so validity is constant and
+// the boundary masks reduce to per-cell 0/1 factors fixed over the march
+// (the same expressions, with fewer instructions per plane).
 template <typename T, int TX, int TY, bool NTS, bool LEAN>
 __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int tile,
                                             Smem2<T, TX, TY> &sm, T *__restrict__ smt) {
   using W = Win2<TX, TY>;
-  static_assert(TX == 64, "a wave covers one tile row");
-  constexpr int WW = W::WW, WC = W::WC, NT = W::NT, NR = W::NR;
-  static_assert(NR <= NT, "every ring cell needs a thread");
+  constexpr int WW = W::WW, WC = W::WC;
   const StepArgs<T> &a = f.a;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int plane = nx * ny;
@@ -128,117 +106,65 @@ __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int ti
   chunk2_span(f, ch, k0, k1);
 
   const Geo<T> &g = a.g;
-  const CT<T> *gLz = (const CT<T> *)g.Lz;
-  const CT<T> *gtLz = (const CT<T> *)g.tLz;
-  const CT<T> *giAx = (const CT<T> *)g.iAx;
-  const CT<T> *gitAx = (const CT<T> *)g.itAx;
   const CT<T> *mtab = (const CT<T> *)a.mtab;
   const uint32_t pec = a.pec, pmc = a.pmc;
-  // beam lines: the own cell's plane index is wave-uniform (scalar loads);
-  // the ring cell's value travels in its update packet.  A null line reads
-  // the L_z table instead (valid memory, value unused).
-  const bool b1on = f.beam1 != nullptr, b2on = f.beam2 != nullptr;
-  const CT<T> *cb1 = (const CT<T> *)(b1on ? f.beam1 : g.Lz);
-  const CT<T> *cb2 = (const CT<T> *)(b2on ? f.beam2 : g.Lz);
-  const T *gb1 = b1on ? f.beam1 : g.Lz;
-
-  for (int t = threadIdx.x; t < a.nmat * MAT_STRIDE; t += NT) smt[t] = mtab[t];
+  for (int t = threadIdx.x; t < a.nmat * MAT_STRIDE; t += WC) smt[t] = mtab[t];
 
   auto clampk = [&](int kk) __attribute__((always_inline)) { return min(max(kk, 0), nz - 1); };
+  auto fac = [](bool masked) __attribute__((always_inline)) { return masked ? T(0) : T(1); };
 
-  // ---- own cell (tile cell of this thread; never wraps: full tiles) ----
-  const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
-  const int oi = x0 + lx;
-  const int oj = __builtin_amdgcn_readfirstlane(y0 + ly);  // wave-uniform
-  const int ol = (ly + 2) * WW + (lx + 2);
-  const int ocol = oi + nx * oj;
-  bool opec[3], opmc[3];
+  // ---- this thread's window cell ----
+  const int cl = threadIdx.x;  // LDS index wj * WW + wi
+  const int wj = cl / WW, wi = cl - wj * WW;
+  const WinCell wc = win_cell(x0 - 2 + wi, y0 - 2 + wj, nx, ny);
+  const int ci = wc.i, cj = wc.j, cdk = wc.dk;
+  const int ccol = ci + nx * cj;
+  // roles: H1 and E1 of step 1, H2 of step 2, output (the tile)
+  const bool rh1 = wi <= TX + 2 && wj <= TY + 2;
+  const bool re1 = rh1 && wi >= 1 && wj >= 1;
+  const bool rh2 = wi >= 1 && wj >= 1 && wi <= TX + 1 && wj <= TY + 1;
+  const bool rout = wi >= 2 && wj >= 2 && wi < TX + 2 && wj < TY + 2;
+  bool mpe[3], mpm[3];  // (i, j) part of the pec / pmc masks
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    opec[d] = xy_mask(pec, d, oi, oj, nx, ny);
-    opmc[d] = xy_mask(pmc, d, oi, oj, nx, ny);
+    mpe[d] = xy_mask(pec, d, ci, cj, nx, ny);
+    mpm[d] = xy_mask(pmc, d, ci, cj, nx, ny);
   }
-  const T oLx = g.Lx[oi], oLy = g.Ly[oj], otLx = g.tLx[oi], otLy = g.tLy[oj];
-  const T oiAz = g.iAz[ocol], oitAz = g.itAz[ocol];
-  const bool obeam = a.beam_mode != BEAM_OFF && oi == a.ixs && oj == a.iys;
-  // wake probe window of this column (or -1)
+  const T cLx = g.Lx[ci], cLy = g.Ly[cj], ctLx = g.tLx[ci], ctLy = g.tLy[cj];
+  const T ciAz = g.iAz[ccol], citAz = g.itAz[ccol];
+  const bool cbeam = a.beam_mode != BEAM_OFF && ci == a.ixs && cj == a.iys;
+  // wake probe window of this column (tile cells only), or -1
   int pw = -1;
-  if (a.hist) {
-    const int di = oi - a.ixt, dj = oj - a.iyt;
+  if (a.hist && rout) {
+    const int di = ci - a.ixt, dj = cj - a.iyt;
     if (di >= -a.hw && di <= a.hw && dj >= -a.hw && dj <= a.hw)
       pw = (di + a.hw) * (2 * a.hw + 1) + (dj + a.hw);
   }
+  const bool probe1 = pw >= 0 && a.step < a.hist_nt;
+  const bool probe2 = pw >= 0 && a.step + 1 < a.hist_nt;
+  GT<T> *phist = pw >= 0 ? (GT<T> *)a.hist + (int64_t)pw * nz * a.hist_nt + a.step : nullptr;
 
-  // ---- ring cell of this thread (threads < NR; waves 0 .. NR/64) ----
-  const bool ring = (int)threadIdx.x < NR;
-  int rwi = 0, rwj = 0;
-  if (ring) ring_cell<TX, TY>(threadIdx.x, rwi, rwj);
-  const WinCell rc = win_cell(x0 - 2 + rwi, y0 - 2 + rwj, nx, ny);
-  const int rl = rwj * WW + rwi;
-  const int rcol = rc.i + nx * rc.j;
-  const int rdk = rc.dk;
-  const bool rh1 = ring && rwi <= TX + 2 && rwj <= TY + 2;
-  const bool re1 = rh1 && rwi >= 1 && rwj >= 1;
-  const bool rh2 = ring && rwi >= 1 && rwj >= 1 && rwi <= TX + 1 && rwj <= TY + 1;
-  bool rpec[3], rpmc[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    rpec[d] = xy_mask(pec, d, rc.i, rc.j, nx, ny);
-    rpmc[d] = xy_mask(pmc, d, rc.i, rc.j, nx, ny);
-  }
-  const T rLx = g.Lx[rc.i], rLy = g.Ly[rc.j], rtLx = g.tLx[rc.i], rtLy = g.tLy[rc.j];
-  const T riAz = g.iAz[rcol], ritAz = g.itAz[rcol];
-  const bool rbeam = a.beam_mode != BEAM_OFF && rc.i == a.ixs && rc.j == a.iys;
-
-  auto fac = [](bool masked) __attribute__((always_inline)) { return masked ? T(0) : T(1); };
-  // boundary-mask factors of the own (o) and ring (r) cell at true plane k,
-  // pec (column mask, E) and pmc (row mask, H); validity of a true plane
-  const T opeF[3] = {fac(opec[0]), fac(opec[1]), fac(opec[2])};
-  const T opmF[3] = {fac(opmc[0]), fac(opmc[1]), fac(opmc[2])};
-  const T rpeF[3] = {fac(rpec[0]), fac(rpec[1]), fac(rpec[2])};
-  const T rpmF[3] = {fac(rpmc[0]), fac(rpmc[1]), fac(rpmc[2])};
-  auto ope = [&](int c, int k) __attribute__((always_inline)) -> T {
-    if constexpr (LEAN) return opeF[c];
-    else return fac(full_mask(pec, c, opec[c], k, nz));
+  // boundary-mask factors (pec: E columns, pmc: H rows) at true plane kk
+  const T peF[3] = {fac(mpe[0]), fac(mpe[1]), fac(mpe[2])};
+  const T pmF[3] = {fac(mpm[0]), fac(mpm[1]), fac(mpm[2])};
+  auto pe = [&](int c, int kk) __attribute__((always_inline)) -> T {
+    if constexpr (LEAN) return peF[c];
+    else return fac(full_mask(pec, c, mpe[c], kk, nz));
   };
-  auto opm = [&](int c, int k) __attribute__((always_inline)) -> T {
-    if constexpr (LEAN) return opmF[c];
-    else return fac(full_mask(pmc, c, opmc[c], k, nz));
+  auto pm = [&](int c, int kk) __attribute__((always_inline)) -> T {
+    if constexpr (LEAN) return pmF[c];
+    else return fac(full_mask(pmc, c, mpm[c], kk, nz));
   };
-  auto rpe = [&](int c, int k) __attribute__((always_inline)) -> T {
-    if constexpr (LEAN) return rpeF[c];
-    else return fac(full_mask(pec, c, rpec[c], k, nz));
-  };
-  auto rpm = [&](int c, int k) __attribute__((always_inline)) -> T {
-    if constexpr (LEAN) return rpmF[c];
-    else return fac(full_mask(pmc, c, rpmc[c], k, nz));
-  };
-  auto valid = [&](int k) __attribute__((always_inline)) -> bool {
+  auto valid = [&](int kk) __attribute__((always_inline)) -> bool {
     if constexpr (LEAN) return true;
-    else return k >= 0 && k < nz;
+    else return kk >= 0 && kk < nz;
   };
 
-  // ---- packets ----
-  // staging data of one plane: E_old of the own and the ring cell, L_z of the
-  // ring cell (the own cell's is a scalar)
-  struct SPk {
-    T oe[3], re[3], rlz;
-  };
-  // update data of one plane: H_old, material, the per-lane areas
-  struct CPk {
-    T oh[3], oiay, oitay;
-    int om;
-    T rh[3], riax, riay, ritax, ritay, rtlz, rb1;
-    int rm;
-  };
-  // Field, material and metric loads are CDNA buffer instructions: one
-  // resource per array (the fields span < 2^32 bytes: fused2_supported), a
-  // per-lane 32-bit byte offset and the component offset c N sizeof(T) as the
+  // ---- loads: CDNA buffer instructions ----
+  // One resource per array (the fields span < 2^32 bytes: fused2_supported),
+  // a per-lane byte offset and the component offset c N sizeof(T) as the
   // uniform one.  Planes are clamped to the grid, so every access is in
-  // range; values of cells outside it are never used (their u and w are 0).
-  // (Packet fields are assigned once, after a branch: stores of different
-  // fields in the two arms would be merged through memory, which keeps the
-  // packet in scratch and makes every load wait.)
+  // range; values at clamped planes are never used (their u and w are 0).
   constexpr uint32_t SZ = sizeof(T);
   const uint32_t NB = (uint32_t)a.N * SZ, NB2 = 2 * NB;
   const uint32_t PB = (uint32_t)plane * SZ;
@@ -251,279 +177,180 @@ __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int ti
   auto gofs = [&](const T *t) __attribute__((always_inline)) -> uint32_t {
     return (uint32_t)((const char *)t - (const char *)g.Lx);
   };
-  const uint32_t oLzT = gofs(g.Lz), otLzT = gofs(g.tLz), oiAxT = gofs(g.iAx), oiAyT = gofs(g.iAy);
-  const uint32_t oitAxT = gofs(g.itAx), oitAyT = gofs(g.itAy);
-  const uint32_t ocb = (uint32_t)ocol * SZ;
-  const uint32_t rcb = (uint32_t)rcol * SZ;
-  const uint32_t orow = (uint32_t)nx * SZ, rrow = (uint32_t)ny * SZ;  // (i,k) / (j,k) tables
-  // ring beam line through the geometry resource when on (it is another
-  // allocation: read it with its own resource)
-  const Rsrc rB1 = make_rsrc(gb1, (int64_t)nz * SZ);
+  // beam lines (a null line reads the L_z table instead: valid memory, value
+  // unused)
+  const bool b1on = f.beam1 != nullptr, b2on = f.beam2 != nullptr;
+  const Rsrc rB1 = make_rsrc(b1on ? f.beam1 : g.Lz, (int64_t)nz * SZ);
+  const Rsrc rB2 = make_rsrc(b2on ? f.beam2 : g.Lz, (int64_t)nz * SZ);
+  const uint32_t cb = (uint32_t)ccol * SZ;
+  const uint32_t oLz = gofs(g.Lz), otLz = gofs(g.tLz);
+  const uint32_t oiAx = gofs(g.iAx) + (uint32_t)cj * SZ, oitAx = gofs(g.itAx) + (uint32_t)cj * SZ;
+  const uint32_t oiAy = gofs(g.iAy) + (uint32_t)ci * SZ, oitAy = gofs(g.itAy) + (uint32_t)ci * SZ;
+  const uint32_t rowx = (uint32_t)ny * SZ, rowy = (uint32_t)nx * SZ;  // (j,k) / (i,k) tables
 
+  // staging data of one plane: E_old and L_z
+  struct SPk {
+    T e[3], lz;
+  };
+  // update data of one plane: H_old, material, areas, tL_z, beam values
+  struct CPk {
+    T h[3], iax, iay, itax, itay, tlz, b1, b2;
+    int m;
+  };
   auto load_S = [&](int k, SPk &p) __attribute__((always_inline)) {
-    const uint32_t so = (uint32_t)clampk(k) * PB;  // wave-uniform
-    p.oe[0] = bload<T>(rE, ocb, so);
-    p.oe[1] = bload<T>(rE, ocb, so + NB);
-    p.oe[2] = bload<T>(rE, ocb, so + NB2);
-    T e0 = T(0), e1 = T(0), e2 = T(0), lz = T(0);
-    if (ring) {
-      const int kr = clampk(k + rdk);
-      const uint32_t v = rcb + (uint32_t)kr * PB;
-      e0 = bload<T>(rE, v, 0);
-      e1 = bload<T>(rE, v, NB);
-      e2 = bload<T>(rE, v, NB2);
-      lz = bload<T>(rG, oLzT + (uint32_t)kr * SZ, 0);
-    }
-    p.re[0] = e0;
-    p.re[1] = e1;
-    p.re[2] = e2;
-    p.rlz = lz;
+    const int kr = clampk(k + cdk);
+    const uint32_t v = cb + (uint32_t)kr * PB;
+    p.e[0] = bload<T>(rE, v, 0);
+    p.e[1] = bload<T>(rE, v, NB);
+    p.e[2] = bload<T>(rE, v, NB2);
+    p.lz = bload<T>(rG, oLz + (uint32_t)kr * SZ, 0);
   };
   auto load_C = [&](int k, CPk &p) __attribute__((always_inline)) {
-    const int ko = clampk(k);
-    const uint32_t so = (uint32_t)ko * PB;
-    p.oh[0] = bload<T>(rH, ocb, so);
-    p.oh[1] = bload<T>(rH, ocb, so + NB);
-    p.oh[2] = bload<T>(rH, ocb, so + NB2);
-    p.om = hasmat ? bload_u8(rM, (uint32_t)ocol, (uint32_t)ko * (uint32_t)plane) : 0;
-    p.oiay = bload<T>(rG, oiAyT + (uint32_t)oi * SZ, (uint32_t)ko * orow);
-    p.oitay = bload<T>(rG, oitAyT + (uint32_t)oi * SZ, (uint32_t)ko * orow);
-    T h0 = T(0), h1 = T(0), h2 = T(0), iax = T(0), iay = T(0), itax = T(0), itay = T(0),
-      tlz = T(0), b1 = T(0);
-    int m = 0;
-    if (ring) {
-      const int kr = clampk(k + rdk);
-      const uint32_t v = rcb + (uint32_t)kr * PB;
-      h0 = bload<T>(rH, v, 0);
-      h1 = bload<T>(rH, v, NB);
-      h2 = bload<T>(rH, v, NB2);
-      m = hasmat ? bload_u8(rM, (uint32_t)rcol + (uint32_t)kr * (uint32_t)plane, 0) : 0;
-      const uint32_t jk = (uint32_t)(rc.j + ny * kr) * SZ, ik = (uint32_t)(rc.i + nx * kr) * SZ;
-      iax = bload<T>(rG, oiAxT + jk, 0);
-      iay = bload<T>(rG, oiAyT + ik, 0);
-      itax = bload<T>(rG, oitAxT + jk, 0);
-      itay = bload<T>(rG, oitAyT + ik, 0);
-      tlz = bload<T>(rG, otLzT + (uint32_t)kr * SZ, 0);
-      b1 = bload<T>(rB1, (uint32_t)kr * SZ, 0);
-    }
-    p.rb1 = b1;
-    p.rh[0] = h0;
-    p.rh[1] = h1;
-    p.rh[2] = h2;
-    p.rm = m;
-    p.riax = iax;
-    p.riay = iay;
-    p.ritax = itax;
-    p.ritay = itay;
-    p.rtlz = tlz;
+    const int kr = clampk(k + cdk);
+    const uint32_t v = cb + (uint32_t)kr * PB;
+    p.h[0] = bload<T>(rH, v, 0);
+    p.h[1] = bload<T>(rH, v, NB);
+    p.h[2] = bload<T>(rH, v, NB2);
+    p.m = hasmat ? bload_u8(rM, (uint32_t)ccol + (uint32_t)kr * (uint32_t)plane, 0) : 0;
+    p.iax = bload<T>(rG, oiAx + (uint32_t)kr * rowx, 0);
+    p.iay = bload<T>(rG, oiAy + (uint32_t)kr * rowy, 0);
+    p.itax = bload<T>(rG, oitAx + (uint32_t)kr * rowx, 0);
+    p.itay = bload<T>(rG, oitAy + (uint32_t)kr * rowy, 0);
+    p.tlz = bload<T>(rG, otLz + (uint32_t)kr * SZ, 0);
+    p.b1 = bload<T>(rB1, (uint32_t)kr * SZ, 0);
+    p.b2 = bload<T>(rB2, (uint32_t)kr * SZ, 0);
   };
 
-  auto curl_u = [&](const T (*u)[WC], const T (*un)[WC], int d, int l)
+  auto curl_u = [&](const T (*u)[WC], const T (*un)[WC], int d)
       __attribute__((always_inline)) -> T {
+    const int l = cl;
     if (d == 0) return ((u[1][l] - un[1][l]) - u[2][l]) + u[2][l + WW];
     if (d == 1) return ((u[2][l] - u[2][l + 1]) - u[0][l]) + un[0][l];
     return ((u[0][l] - u[0][l + WW]) - u[1][l]) + u[1][l + 1];
   };
-  auto curl_w = [&](const T (*w)[WC], const T (*wm)[WC], int d, int l)
+  auto curl_w = [&](const T (*w)[WC], const T (*wm)[WC], int d)
       __attribute__((always_inline)) -> T {
+    const int l = cl;
     if (d == 0) return ((w[2][l] - w[2][l - WW]) - w[1][l]) + wm[1][l];
     if (d == 1) return ((w[0][l] - wm[0][l]) - w[2][l]) + w[2][l - 1];
     return ((w[1][l] - w[1][l - 1]) - w[0][l]) + w[0][l - WW];
   };
-
-  // stage u0 of window plane k (own and ring cell) from the packet
-  auto stage_u0 = [&](int k, int slot, const SPk &p) __attribute__((always_inline)) {
-    T(*u)[WC] = sm.u0[slot];
-    {
-      const bool v = valid(k);  // wave-uniform
-      const T lz = gLz[clampk(k)];
-      u[0][ol] = v ? (oLx * ope(0, k)) * p.oe[0] : T(0);
-      u[1][ol] = v ? (oLy * ope(1, k)) * p.oe[1] : T(0);
-      u[2][ol] = v ? ope(2, k) * (lz * p.oe[2]) : T(0);
-    }
-    if (ring) {
-      const int kr = k + rdk;
-      const bool v = valid(kr);
-      u[0][rl] = v ? (rLx * rpe(0, kr)) * p.re[0] : T(0);
-      u[1][rl] = v ? (rLy * rpe(1, kr)) * p.re[1] : T(0);
-      u[2][rl] = v ? rpe(2, kr) * (p.rlz * p.re[2]) : T(0);
-    }
+  // u = L pec E of this cell at window plane k into ring slot u
+  auto put_u = [&](T (*u)[WC], int k, const T (&e)[3], T lz) __attribute__((always_inline)) {
+    const int kr = k + cdk;
+    const bool v = valid(kr);
+    u[0][cl] = v ? (cLx * pe(0, kr)) * e[0] : T(0);
+    u[1][cl] = v ? (cLy * pe(1, kr)) * e[1] : T(0);
+    u[2][cl] = v ? pe(2, kr) * (lz * e[2]) : T(0);
   };
 
   // ---- pipeline state ----
-  SPk S;          // E_old of the plane staged next (in flight)
-  T oE0c[3], oE0n[3], rE0c[3], rE0n[3];  // raw E_old of planes k / k+1
-  T rLzc = 0, rLzn = 0;                  // ring L_z of planes k / k+1
-  T oH1p[3], rH1p[3];                    // raw H1 of plane k-1
-  T oE1p[3];                             // raw E1 of plane k-1 (own)
-  int omp = 0, rmp = 0;                  // materials of plane k-1
-  T oiayp = 0, oitayp = 0, riaxp = 0, riayp = 0, rtlzp = 0;  // areas of plane k-1
+  SPk S;                   // E_old of the plane staged next (in flight)
+  T E0c[3], E0n[3];        // raw E_old of planes k / k+1
+  T lzc = 0, lzn = 0;      // L_z of planes k / k+1
+  T H1p[3], E1p[3];        // raw H1 and E1 of plane k-1
+  int mp = 0;              // material of plane k-1
+  T iaxp = 0, iayp = 0, itaxp = 0, itayp = 0, tlzp = 0, b2p = 0;  // plane k-1
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    oE0c[d] = oE0n[d] = rE0c[d] = rE0n[d] = T(0);
-    oH1p[d] = rH1p[d] = oE1p[d] = T(0);
-  }
+  for (int d = 0; d < 3; ++d) E0c[d] = E0n[d] = H1p[d] = E1p[d] = T(0);
 
   // prologue: u0 of plane k0-2 staged; S = E_old of plane k0-1; C of k0-2
   const int kb = k0 - 2;
   load_S(kb, S);
-  stage_u0(kb, 0, S);
+  put_u(sm.u0[0], kb, S.e, S.lz);
   load_S(kb + 1, S);
   CPk CA, CB;
   load_C(kb, CA);
 
-  const bool probe1 = pw >= 0 && a.step < a.hist_nt;
-  const bool probe2 = pw >= 0 && a.step + 1 < a.hist_nt;
-  GT<T> *phist = pw >= 0 ? (GT<T> *)a.hist + (int64_t)pw * nz * a.hist_nt + a.step : nullptr;
-
   // One plane: stage u0(k+1); H1(k); E1(k); H2(k-1); E2(k-1).  Cc: update
   // data of plane k (loaded one iteration earlier); Cn receives plane k+1's.
-  // PH: phases run (compile time): 1 = E1, 2 = H2, 4 = E2 (staging and H1
-  // always); the three pipeline-fill iterations before the loop run fewer
-  // PA: LDS ring slot of plane k, (k - kb) & 1, a compile-time constant in
-  // every unrolled copy (so LDS addresses are immediate offsets)
+  // PH: phases run besides staging and H1 (1 = E1, 2 = H2, 4 = E2); the three
+  // pipeline-fill iterations before the loop run fewer.  PA: LDS ring slot of
+  // plane k, (k - kb) & 1, a compile-time constant in every unrolled copy.
   auto iter = [&](const int k, const CPk &Cc, CPk &Cn, auto ph, auto pa)
       __attribute__((always_inline)) {
     constexpr int PH = decltype(ph)::value;
     constexpr int PA = decltype(pa)::value, PB1 = PA ^ 1;
-    // A. stage u0(k+1); keep its raw E for E1(k+1)
-    stage_u0(k + 1, PB1, S);
+    // A. stage u0(k+1); keep its raw E and L_z for E1(k+1)
+    put_u(sm.u0[PB1], k + 1, S.e, S.lz);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      oE0n[d] = S.oe[d];
-      rE0n[d] = S.re[d];
-    }
-    rLzn = S.rlz;
+    for (int d = 0; d < 3; ++d) E0n[d] = S.e[d];
+    lzn = S.lz;
     // B. loads of the next iteration (unconditional, at clamped planes: a
     // conditional load would make the compiler merge and copy registers of
-    // loads in flight at the loop's back edge)
+    // loads in flight)
     load_S(k + 2, S);
     load_C(k + 1, Cn);
     __syncthreads();
 
     // C. H1(k) -> w1 slot PA
-    T oH1n[3], rH1n[3];
-    {
+    T H1n[3] = {T(0), T(0), T(0)};
+    if (rh1) {
+      const int kr = k + cdk;
+      const T *mt = smt + Cc.m * MAT_STRIDE;
+      const T fm0 = pm(0, kr), fm1 = pm(1, kr), fm2 = pm(2, kr);
       const T(*u)[WC] = sm.u0[PA];
       const T(*un)[WC] = sm.u0[PB1];
+      H1n[0] = Cc.h[0] - (mt[0] * (fm0 * Cc.iax)) * curl_u(u, un, 0);
+      H1n[1] = Cc.h[1] - (mt[1] * (fm1 * Cc.iay)) * curl_u(u, un, 1);
+      H1n[2] = Cc.h[2] - (mt[2] * (fm2 * ciAz)) * curl_u(u, un, 2);
+      const T vf = valid(kr) ? T(1) : T(0);
       T(*w)[WC] = sm.w1[PA];
-      {
-        const bool v = valid(k);
-        const int kc = clampk(k);
-        const T *mt = smt + Cc.om * MAT_STRIDE;
-        const T fm0 = opm(0, k), fm1 = opm(1, k), fm2 = opm(2, k);
-        const T iax = giAx[oj + ny * kc];
-        oH1n[0] = Cc.oh[0] - (mt[0] * (fm0 * iax)) * curl_u(u, un, 0, ol);
-        oH1n[1] = Cc.oh[1] - (mt[1] * (fm1 * Cc.oiay)) * curl_u(u, un, 1, ol);
-        oH1n[2] = Cc.oh[2] - (mt[2] * (fm2 * oiAz)) * curl_u(u, un, 2, ol);
-        const T vf = v ? T(1) : T(0);
-        w[0][ol] = (vf * (fm0 * otLx)) * oH1n[0];
-        w[1][ol] = (vf * (fm1 * otLy)) * oH1n[1];
-        w[2][ol] = (vf * (fm2 * gtLz[kc])) * oH1n[2];
-      }
-      if (rh1) {
-        const int kr = k + rdk;
-        const bool v = valid(kr);
-        const T *mt = smt + Cc.rm * MAT_STRIDE;
-        const T fm0 = rpm(0, kr), fm1 = rpm(1, kr), fm2 = rpm(2, kr);
-        rH1n[0] = Cc.rh[0] - (mt[0] * (fm0 * Cc.riax)) * curl_u(u, un, 0, rl);
-        rH1n[1] = Cc.rh[1] - (mt[1] * (fm1 * Cc.riay)) * curl_u(u, un, 1, rl);
-        rH1n[2] = Cc.rh[2] - (mt[2] * (fm2 * riAz)) * curl_u(u, un, 2, rl);
-        const T vf = v ? T(1) : T(0);
-        w[0][rl] = (vf * (fm0 * rtLx)) * rH1n[0];
-        w[1][rl] = (vf * (fm1 * rtLy)) * rH1n[1];
-        w[2][rl] = (vf * (fm2 * Cc.rtlz)) * rH1n[2];
-      } else {
-        rH1n[0] = rH1n[1] = rH1n[2] = T(0);
-      }
+      w[0][cl] = (vf * (fm0 * ctLx)) * H1n[0];
+      w[1][cl] = (vf * (fm1 * ctLy)) * H1n[1];
+      w[2][cl] = (vf * (fm2 * Cc.tlz)) * H1n[2];
     }
     __syncthreads();
 
-    // D. E1(k) -> u1 slot PA; probe both steps
-    T oE1n[3] = {T(0), T(0), T(0)};
+    // D. E1(k) -> u1 slot PA; probe both steps (tile cells)
+    T E1n[3] = {T(0), T(0), T(0)};
     if constexpr ((PH & 1) != 0) {
-      const T(*w)[WC] = sm.w1[PA];
-      const T(*wm)[WC] = sm.w1[PB1];
-      T(*u)[WC] = sm.u1[PA];
-      {
-        const bool v = valid(k);
-        const int kc = clampk(k);
-        const T *mt = smt + Cc.om * MAT_STRIDE;
-        const T ax = ope(0, k) * gitAx[oj + ny * kc];
-        const T ay = ope(1, k) * Cc.oitay;
-        const T az = ope(2, k) * oitAz;
-        const T jx = f.js1 * (mt[6] * oE0c[0]);
-        const T jy = f.js1 * (mt[7] * oE0c[1]);
-        const T jz = (obeam && b1on) ? cb1[kc] : f.js1 * (mt[8] * oE0c[2]);
-        oE1n[0] = oE0c[0] + mt[3] * (ax * curl_w(w, wm, 0, ol) - jx);
-        oE1n[1] = oE0c[1] + mt[4] * (ay * curl_w(w, wm, 1, ol) - jy);
-        oE1n[2] = oE0c[2] + mt[5] * (az * curl_w(w, wm, 2, ol) - jz);
-        u[0][ol] = v ? (oLx * ope(0, k)) * oE1n[0] : T(0);
-        u[1][ol] = v ? (oLy * ope(1, k)) * oE1n[1] : T(0);
-        u[2][ol] = v ? ope(2, k) * (gLz[kc] * oE1n[2]) : T(0);
-        if (k >= k0 && k < k1) {
-          if (probe1) phist[(int64_t)k * a.hist_nt] = oE0c[2];
-          if (probe2) phist[(int64_t)k * a.hist_nt + 1] = oE1n[2];
-        }
-      }
       if (re1) {
-        const int kr = k + rdk;
-        const bool v = valid(kr);
-        const T *mt = smt + Cc.rm * MAT_STRIDE;
-        const T ax = rpe(0, kr) * Cc.ritax;
-        const T ay = rpe(1, kr) * Cc.ritay;
-        const T az = rpe(2, kr) * ritAz;
-        const T jx = f.js1 * (mt[6] * rE0c[0]);
-        const T jy = f.js1 * (mt[7] * rE0c[1]);
-        const T jz = (rbeam && b1on) ? Cc.rb1 : f.js1 * (mt[8] * rE0c[2]);
-        const T e0 = rE0c[0] + mt[3] * (ax * curl_w(w, wm, 0, rl) - jx);
-        const T e1 = rE0c[1] + mt[4] * (ay * curl_w(w, wm, 1, rl) - jy);
-        const T e2 = rE0c[2] + mt[5] * (az * curl_w(w, wm, 2, rl) - jz);
-        u[0][rl] = v ? (rLx * rpe(0, kr)) * e0 : T(0);
-        u[1][rl] = v ? (rLy * rpe(1, kr)) * e1 : T(0);
-        u[2][rl] = v ? rpe(2, kr) * (rLzc * e2) : T(0);
+        const int kr = k + cdk;
+        const T *mt = smt + Cc.m * MAT_STRIDE;
+        const T(*w)[WC] = sm.w1[PA];
+        const T(*wm)[WC] = sm.w1[PB1];
+        const T ax = pe(0, kr) * Cc.itax;
+        const T ay = pe(1, kr) * Cc.itay;
+        const T az = pe(2, kr) * citAz;
+        const T jx = f.js1 * (mt[6] * E0c[0]);
+        const T jy = f.js1 * (mt[7] * E0c[1]);
+        const T jz = (cbeam && b1on) ? Cc.b1 : f.js1 * (mt[8] * E0c[2]);
+        E1n[0] = E0c[0] + mt[3] * (ax * curl_w(w, wm, 0) - jx);
+        E1n[1] = E0c[1] + mt[4] * (ay * curl_w(w, wm, 1) - jy);
+        E1n[2] = E0c[2] + mt[5] * (az * curl_w(w, wm, 2) - jz);
+        put_u(sm.u1[PA], k, E1n, lzc);
+        if (k >= k0 && k < k1) {
+          if (probe1) phist[(int64_t)k * a.hist_nt] = E0c[2];
+          if (probe2) phist[(int64_t)k * a.hist_nt + 1] = E1n[2];
+        }
       }
     }
     __syncthreads();
 
-    // E. H2(k-1) -> w2 slot PB1; store H2(k-1)
+    // E. H2(k-1) -> w2 slot PB1; store H2(k-1) (tile cells)
     if constexpr ((PH & 2) != 0) {
-      const int km = k - 1;
-      const T(*u)[WC] = sm.u1[PB1];
-      const T(*un)[WC] = sm.u1[PA];
-      T(*w)[WC] = sm.w2[PB1];
-      {
-        const bool v = valid(km);
-        const int kc = clampk(km);
-        const T *mt = smt + omp * MAT_STRIDE;
-        const T fm0 = opm(0, km), fm1 = opm(1, km), fm2 = opm(2, km);
-        const T iax = giAx[oj + ny * kc];
-        const T h0 = oH1p[0] - (mt[0] * (fm0 * iax)) * curl_u(u, un, 0, ol);
-        const T h1 = oH1p[1] - (mt[1] * (fm1 * oiayp)) * curl_u(u, un, 1, ol);
-        const T h2 = oH1p[2] - (mt[2] * (fm2 * oiAz)) * curl_u(u, un, 2, ol);
-        const T vf = v ? T(1) : T(0);
-        w[0][ol] = (vf * (fm0 * otLx)) * h0;
-        w[1][ol] = (vf * (fm1 * otLy)) * h1;
-        w[2][ol] = (vf * (fm2 * gtLz[kc])) * h2;
-        if (km >= k0 && km < k1) {
-          const uint32_t so = (uint32_t)km * PB;
-          bstore(rHn, ocb, so, h0, NTS);
-          bstore(rHn, ocb, so + NB, h1, NTS);
-          bstore(rHn, ocb, so + NB2, h2, NTS);
-        }
-      }
       if (rh2) {
-        const int kr = km + rdk;
-        const bool v = valid(kr);
-        const T *mt = smt + rmp * MAT_STRIDE;
-        const T fm0 = rpm(0, kr), fm1 = rpm(1, kr), fm2 = rpm(2, kr);
-        const T h0 = rH1p[0] - (mt[0] * (fm0 * riaxp)) * curl_u(u, un, 0, rl);
-        const T h1 = rH1p[1] - (mt[1] * (fm1 * riayp)) * curl_u(u, un, 1, rl);
-        const T h2 = rH1p[2] - (mt[2] * (fm2 * riAz)) * curl_u(u, un, 2, rl);
-        const T vf = v ? T(1) : T(0);
-        w[0][rl] = (vf * (fm0 * rtLx)) * h0;
-        w[1][rl] = (vf * (fm1 * rtLy)) * h1;
-        w[2][rl] = (vf * (fm2 * rtlzp)) * h2;
+        const int km = k - 1;
+        const int kr = km + cdk;
+        const T *mt = smt + mp * MAT_STRIDE;
+        const T fm0 = pm(0, kr), fm1 = pm(1, kr), fm2 = pm(2, kr);
+        const T(*u)[WC] = sm.u1[PB1];
+        const T(*un)[WC] = sm.u1[PA];
+        const T h0 = H1p[0] - (mt[0] * (fm0 * iaxp)) * curl_u(u, un, 0);
+        const T h1 = H1p[1] - (mt[1] * (fm1 * iayp)) * curl_u(u, un, 1);
+        const T h2 = H1p[2] - (mt[2] * (fm2 * ciAz)) * curl_u(u, un, 2);
+        const T vf = valid(kr) ? T(1) : T(0);
+        T(*w)[WC] = sm.w2[PB1];
+        w[0][cl] = (vf * (fm0 * ctLx)) * h0;
+        w[1][cl] = (vf * (fm1 * ctLy)) * h1;
+        w[2][cl] = (vf * (fm2 * tlzp)) * h2;
+        if (rout && km >= k0 && km < k1) {
+          const uint32_t v = cb + (uint32_t)km * PB;
+          bstore(rHn, v, 0, h0, NTS);
+          bstore(rHn, v, NB, h1, NTS);
+          bstore(rHn, v, NB2, h2, NTS);
+        }
       }
     }
     __syncthreads();
@@ -531,41 +358,38 @@ __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int ti
     // F. E2(k-1) of the tile; store it
     if constexpr ((PH & 4) != 0) {
       const int km = k - 1;
-      const T(*w)[WC] = sm.w2[PB1];
-      const T(*wm)[WC] = sm.w2[PA];  // plane k-2
-      const int kc = clampk(km);
-      const T *mt = smt + omp * MAT_STRIDE;
-      const T ax = ope(0, km) * gitAx[oj + ny * kc];
-      const T ay = ope(1, km) * oitayp;
-      const T az = ope(2, km) * oitAz;
-      const T jx = f.js2 * (mt[6] * oE1p[0]);
-      const T jy = f.js2 * (mt[7] * oE1p[1]);
-      const T jz = (obeam && b2on) ? cb2[kc] : f.js2 * (mt[8] * oE1p[2]);
-      if (km < k1) {
-        const uint32_t so = (uint32_t)km * PB;
-        bstore(rEn, ocb, so, oE1p[0] + mt[3] * (ax * curl_w(w, wm, 0, ol) - jx), NTS);
-        bstore(rEn, ocb, so + NB, oE1p[1] + mt[4] * (ay * curl_w(w, wm, 1, ol) - jy), NTS);
-        bstore(rEn, ocb, so + NB2, oE1p[2] + mt[5] * (az * curl_w(w, wm, 2, ol) - jz), NTS);
+      if (rout && km < k1) {
+        const T *mt = smt + mp * MAT_STRIDE;
+        const T(*w)[WC] = sm.w2[PB1];
+        const T(*wm)[WC] = sm.w2[PA];  // plane k-2
+        const T ax = pe(0, km) * itaxp;
+        const T ay = pe(1, km) * itayp;
+        const T az = pe(2, km) * citAz;
+        const T jx = f.js2 * (mt[6] * E1p[0]);
+        const T jy = f.js2 * (mt[7] * E1p[1]);
+        const T jz = (cbeam && b2on) ? b2p : f.js2 * (mt[8] * E1p[2]);
+        const uint32_t v = cb + (uint32_t)km * PB;
+        bstore(rEn, v, 0, E1p[0] + mt[3] * (ax * curl_w(w, wm, 0) - jx), NTS);
+        bstore(rEn, v, NB, E1p[1] + mt[4] * (ay * curl_w(w, wm, 1) - jy), NTS);
+        bstore(rEn, v, NB2, E1p[2] + mt[5] * (az * curl_w(w, wm, 2) - jz), NTS);
       }
     }
 
-    // rotate plane k into the "previous plane" state
+    // plane k becomes the previous plane
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      oH1p[d] = oH1n[d];
-      rH1p[d] = rH1n[d];
-      oE1p[d] = oE1n[d];
-      oE0c[d] = oE0n[d];
-      rE0c[d] = rE0n[d];
+      H1p[d] = H1n[d];
+      E1p[d] = E1n[d];
+      E0c[d] = E0n[d];
     }
-    rLzc = rLzn;
-    omp = Cc.om;
-    rmp = Cc.rm;
-    oiayp = Cc.oiay;
-    oitayp = Cc.oitay;
-    riaxp = Cc.riax;
-    riayp = Cc.riay;
-    rtlzp = Cc.rtlz;
+    lzc = lzn;
+    mp = Cc.m;
+    iaxp = Cc.iax;
+    iayp = Cc.iay;
+    itaxp = Cc.itax;
+    itayp = Cc.itay;
+    tlzp = Cc.tlz;
+    b2p = Cc.b2;
   };
 
   __syncthreads();  // u0(k0-2) staged, material table loaded
@@ -591,7 +415,7 @@ __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int ti
 extern __shared__ double fit_dyn_lds[];
 
 template <typename T, int TX, int TY, bool NTS>
-__global__ void __launch_bounds__(TX *TY) fused2_sweep(Fused2Args<T> f) {
+__global__ void __launch_bounds__((TX + 4) * (TY + 4)) fused2_sweep(Fused2Args<T> f) {
   __shared__ Smem2<T, TX, TY> sm;
   T *smt = (T *)fit_dyn_lds;
   const int b = blockIdx.x;
@@ -605,9 +429,9 @@ __global__ void __launch_bounds__(TX *TY) fused2_sweep(Fused2Args<T> f) {
   else fused2_tile<T, TX, TY, NTS, false>(f, tile, sm, smt);
 }
 
-// Tile of the two-step sweep: 64 x 8 columns, 512 threads (a 64 x 16 tile's
-// 1024 threads leave 128 registers per lane, which this body exceeds).
-// FP32 only: an FP64 window does not fit the LDS.
+// Tile of the two-step sweep: 64 x 8 columns, a (68 x 12)-cell window, one
+// thread per window cell (816 threads).  FP32 only: an FP64 window does not
+// fit the LDS.
 constexpr int F2_TX = 64, F2_TY = 8;
 
 template <typename T, int TX, int TY>
@@ -662,7 +486,7 @@ inline bool launch_fused2(const StepArgs<T> &a, T *En, T *Hn, const T *beam1, co
     return false;
   } else {
     if (!fused2_supported<T>(a)) return false;
-    constexpr int TX = F2_TX, TY = F2_TY;
+    constexpr int TX = F2_TX, TY = F2_TY, NT = (TX + 4) * (TY + 4);
     Fused2Args<T> f;
     f.a = a;
     f.En = En;
@@ -676,8 +500,7 @@ inline bool launch_fused2(const StepArgs<T> &a, T *En, T *Hn, const T *beam1, co
     int dev = 0;
     (void)hipGetDevice(&dev);
     const size_t lds = fused_mt_bytes<T>(a.nmat);
-    const int slots = resident_slots(dev, (const void *)fused2_sweep<T, TX, TY, false>, TX * TY,
-                                     lds);
+    const int slots = resident_slots(dev, (const void *)fused2_sweep<T, TX, TY, false>, NT, lds);
     const int ncol = f.ntx * f.nty;
     f.kc = pick_kc2(ncol, a.nz, slots);
     // 4-plane chunks at the z faces take the general path, the chunks
@@ -696,9 +519,9 @@ inline bool launch_fused2(const StepArgs<T> &a, T *En, T *Hn, const T *beam1, co
     }
     f.nb = ncol * f.nchunk;
     if (a.nt)
-      hipLaunchKernelGGL((fused2_sweep<T, TX, TY, true>), dim3(f.nb), dim3(TX * TY), lds, s, f);
+      hipLaunchKernelGGL((fused2_sweep<T, TX, TY, true>), dim3(f.nb), dim3(NT), lds, s, f);
     else
-      hipLaunchKernelGGL((fused2_sweep<T, TX, TY, false>), dim3(f.nb), dim3(TX * TY), lds, s, f);
+      hipLaunchKernelGGL((fused2_sweep<T, TX, TY, false>), dim3(f.nb), dim3(NT), lds, s, f);
     return true;
   }
 }
