@@ -181,8 +181,8 @@ int fit_monitor_read(fit_ctx *ctx, int32_t f, int32_t comp, double *host);
  * conductive != 0 selects J := sigma*E after each step.  Asynchronous.
  * With FIT_KERNEL_AUTO or FIT_KERNEL_FUSED2, pairs of steps run as ONE
  * two-step sweep (E and H read and written once per two steps) on FP32
- * contexts whose nx is a multiple of 64 and ny of 8 (fields below 2^32
- * bytes), when J is 0 or sigma*E
+ * contexts with nx >= 64 and ny >= 16 (fields below 2^32 bytes), when J is
+ * 0 or sigma*E
  * (not FIT_J_BUFFER) and no field-monitor update is staged for the pair;
  * every other step runs the one-step sweep.  Each step's beam value, J and
  * E_z probe sample are those of the one-step sequence. */

@@ -123,7 +123,10 @@ __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int ti
   const bool rh1 = wi <= TX + 2 && wj <= TY + 2;
   const bool re1 = rh1 && wi >= 1 && wj >= 1;
   const bool rh2 = wi >= 1 && wj >= 1 && wi <= TX + 1 && wj <= TY + 1;
-  const bool rout = wi >= 2 && wj >= 2 && wi < TX + 2 && wj < TY + 2;
+  // (a tile may overhang the grid's high x/y edges: its cells there are the
+  // flat continuation, computed like any other, but stored by their own tile)
+  const bool rout = wi >= 2 && wj >= 2 && wi < TX + 2 && wj < TY + 2 && x0 - 2 + wi < nx &&
+                    y0 - 2 + wj < ny;
   bool mpe[3], mpm[3];  // (i, j) part of the pec / pmc masks
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -412,26 +415,327 @@ __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int ti
   }
 }
 
+// The same tile step with two barriers per plane instead of four, for the
+// lean chunks: all H updates of a plane, then all E updates,
+//   P(k): H1(k)   (reads u0 of k, k+1)    and H2(k-2) (reads u1 of k-2, k-1)
+//   Q(k): E1(k)   (reads w1 of k, k-1)    and E2(k-2) (reads w2 of k-2, k-3),
+//         staging of u0(k+2)
+// so the second step lags two planes; every ring keeps 2 planes (a plane's
+// slot is its parity) and each thread carries its cell's H1, E1 and update
+// data of the two previous planes.  Same per-cell expressions as
+// fused2_tile.
+template <typename T, int TX, int TY, bool NTS>
+__device__ __forceinline__ void fused2_tile_lean2b(const Fused2Args<T> &f, const int tile,
+                                                   Smem2<T, TX, TY> &sm, T *__restrict__ smt) {
+  using W = Win2<TX, TY>;
+  constexpr int WW = W::WW, WC = W::WC;
+  const StepArgs<T> &a = f.a;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int plane = nx * ny;
+  const int tx = tile % f.ntx;
+  const int tr = tile / f.ntx;
+  const int ty = tr % f.nty;
+  const int ch = tr / f.nty;
+  const int x0 = tx * TX, y0 = ty * TY;
+  int k0, k1;
+  chunk2_span(f, ch, k0, k1);
+
+  const Geo<T> &g = a.g;
+  const CT<T> *mtab = (const CT<T> *)a.mtab;
+  const uint32_t pec = a.pec, pmc = a.pmc;
+  for (int t = threadIdx.x; t < a.nmat * MAT_STRIDE; t += WC) smt[t] = mtab[t];
+  auto fac = [](bool masked) __attribute__((always_inline)) { return masked ? T(0) : T(1); };
+
+  // ---- this thread's window cell ----
+  const int cl = threadIdx.x;
+  const int wj = cl / WW, wi = cl - wj * WW;
+  const WinCell wc = win_cell(x0 - 2 + wi, y0 - 2 + wj, nx, ny);
+  const int ci = wc.i, cj = wc.j, cdk = wc.dk;
+  const int ccol = ci + nx * cj;
+  const bool rh1 = wi <= TX + 2 && wj <= TY + 2;
+  const bool re1 = rh1 && wi >= 1 && wj >= 1;
+  const bool rh2 = wi >= 1 && wj >= 1 && wi <= TX + 1 && wj <= TY + 1;
+  const bool rout = wi >= 2 && wj >= 2 && wi < TX + 2 && wj < TY + 2 && x0 - 2 + wi < nx &&
+                    y0 - 2 + wj < ny;
+  // lean: interior planes only, so the masks are their (i, j) parts
+  const T pe0 = fac(xy_mask(pec, 0, ci, cj, nx, ny)), pe1 = fac(xy_mask(pec, 1, ci, cj, nx, ny));
+  const T pe2 = fac(xy_mask(pec, 2, ci, cj, nx, ny));
+  const T pm0 = fac(xy_mask(pmc, 0, ci, cj, nx, ny)), pm1 = fac(xy_mask(pmc, 1, ci, cj, nx, ny));
+  const T pm2 = fac(xy_mask(pmc, 2, ci, cj, nx, ny));
+  const T uxF = g.Lx[ci] * pe0, uyF = g.Ly[cj] * pe1;
+  const T wxF = pm0 * g.tLx[ci], wyF = pm1 * g.tLy[cj];
+  const T iAz2 = pm2 * g.iAz[ccol];
+  const T itAz2 = pe2 * g.itAz[ccol];
+  const bool cbeam = a.beam_mode != BEAM_OFF && ci == a.ixs && cj == a.iys;
+  int pw = -1;
+  if (a.hist && rout) {
+    const int di = ci - a.ixt, dj = cj - a.iyt;
+    if (di >= -a.hw && di <= a.hw && dj >= -a.hw && dj <= a.hw)
+      pw = (di + a.hw) * (2 * a.hw + 1) + (dj + a.hw);
+  }
+  const bool probe1 = pw >= 0 && a.step < a.hist_nt;
+  const bool probe2 = pw >= 0 && a.step + 1 < a.hist_nt;
+  GT<T> *phist = pw >= 0 ? (GT<T> *)a.hist + (int64_t)pw * nz * a.hist_nt + a.step : nullptr;
+
+  // ---- loads (as fused2_tile; every plane read lies inside the grid) ----
+  constexpr uint32_t SZ = sizeof(T);
+  const uint32_t NB = (uint32_t)a.N * SZ, NB2 = 2 * NB;
+  const uint32_t PB = (uint32_t)plane * SZ;
+  const int64_t FB = 3 * a.N * (int64_t)SZ;
+  const Rsrc rE = make_rsrc(a.E, FB), rH = make_rsrc(a.H, FB);
+  const Rsrc rEn = make_rsrc(f.En, FB), rHn = make_rsrc(f.Hn, FB);
+  const bool hasmat = a.mat != nullptr;
+  const Rsrc rM = make_rsrc(hasmat ? (const void *)a.mat : (const void *)g.Lx, hasmat ? a.N : 1);
+  const Rsrc rG = make_rsrc(g.Lx, a.geo_bytes);
+  auto gofs = [&](const T *t) __attribute__((always_inline)) -> uint32_t {
+    return (uint32_t)((const char *)t - (const char *)g.Lx);
+  };
+  const bool b1on = f.beam1 != nullptr, b2on = f.beam2 != nullptr;
+  const Rsrc rB1 = make_rsrc(b1on ? f.beam1 : g.Lz, (int64_t)nz * SZ);
+  const Rsrc rB2 = make_rsrc(b2on ? f.beam2 : g.Lz, (int64_t)nz * SZ);
+  const uint32_t cb = (uint32_t)ccol * SZ;
+  const uint32_t cbk = cb + (uint32_t)cdk * PB;  // plane offset of the carry
+  const uint32_t cmk = (uint32_t)ccol + (uint32_t)cdk * (uint32_t)plane;
+  const uint32_t oLz = gofs(g.Lz) + (uint32_t)cdk * SZ, otLz = gofs(g.tLz) + (uint32_t)cdk * SZ;
+  const uint32_t rowx = (uint32_t)ny * SZ, rowy = (uint32_t)nx * SZ;
+  const uint32_t oiAx = gofs(g.iAx) + (uint32_t)cj * SZ + (uint32_t)cdk * rowx;
+  const uint32_t oitAx = gofs(g.itAx) + (uint32_t)cj * SZ + (uint32_t)cdk * rowx;
+  const uint32_t oiAy = gofs(g.iAy) + (uint32_t)ci * SZ + (uint32_t)cdk * rowy;
+  const uint32_t oitAy = gofs(g.itAy) + (uint32_t)ci * SZ + (uint32_t)cdk * rowy;
+  const uint32_t ob = (uint32_t)cdk * SZ;
+
+  struct SPk {
+    T e[3], lz;
+  };
+  struct CPk {
+    T h[3], iax, iay, itax, itay, tlz, b1, b2;
+    int m;
+  };
+  // window plane k (true plane k + cdk, inside the grid for a lean chunk)
+  auto load_S = [&](int k, SPk &p) __attribute__((always_inline)) {
+    const uint32_t v = cbk + (uint32_t)k * PB;
+    p.e[0] = bload<T>(rE, v, 0);
+    p.e[1] = bload<T>(rE, v, NB);
+    p.e[2] = bload<T>(rE, v, NB2);
+    p.lz = bload<T>(rG, oLz + (uint32_t)k * SZ, 0);
+  };
+  auto load_C = [&](int k, CPk &p) __attribute__((always_inline)) {
+    const uint32_t v = cbk + (uint32_t)k * PB;
+    p.h[0] = bload<T>(rH, v, 0);
+    p.h[1] = bload<T>(rH, v, NB);
+    p.h[2] = bload<T>(rH, v, NB2);
+    p.m = hasmat ? bload_u8(rM, cmk + (uint32_t)k * (uint32_t)plane, 0) : 0;
+    p.iax = bload<T>(rG, oiAx + (uint32_t)k * rowx, 0);
+    p.iay = bload<T>(rG, oiAy + (uint32_t)k * rowy, 0);
+    p.itax = bload<T>(rG, oitAx + (uint32_t)k * rowx, 0);
+    p.itay = bload<T>(rG, oitAy + (uint32_t)k * rowy, 0);
+    p.tlz = bload<T>(rG, otLz + (uint32_t)k * SZ, 0);
+    p.b1 = bload<T>(rB1, ob + (uint32_t)k * SZ, 0);
+    p.b2 = bload<T>(rB2, ob + (uint32_t)k * SZ, 0);
+  };
+  auto curl_u = [&](const T (*u)[WC], const T (*un)[WC], int d)
+      __attribute__((always_inline)) -> T {
+    const int l = cl;
+    if (d == 0) return ((u[1][l] - un[1][l]) - u[2][l]) + u[2][l + WW];
+    if (d == 1) return ((u[2][l] - u[2][l + 1]) - u[0][l]) + un[0][l];
+    return ((u[0][l] - u[0][l + WW]) - u[1][l]) + u[1][l + 1];
+  };
+  auto curl_w = [&](const T (*w)[WC], const T (*wm)[WC], int d)
+      __attribute__((always_inline)) -> T {
+    const int l = cl;
+    if (d == 0) return ((w[2][l] - w[2][l - WW]) - w[1][l]) + wm[1][l];
+    if (d == 1) return ((w[0][l] - wm[0][l]) - w[2][l]) + w[2][l - 1];
+    return ((w[1][l] - w[1][l - 1]) - w[0][l]) + w[0][l - WW];
+  };
+  auto put_u = [&](T (*u)[WC], const T (&e)[3], T lz) __attribute__((always_inline)) {
+    u[0][cl] = uxF * e[0];
+    u[1][cl] = uyF * e[1];
+    u[2][cl] = pe2 * (lz * e[2]);
+  };
+
+  // ---- pipeline state: plane k-1 ("p") and k-2 ("q") of this cell ----
+  struct Prev {
+    T H1[3], E1[3];
+    T iax, iay, itax, itay, tlz, b2;
+    int m;
+  };
+  SPk S;                         // E_old of the plane staged next (in flight)
+  T E0a[3], E0b[3];              // raw E_old of planes k and k+1
+  T lza = 0, lzb = 0;            // L_z of planes k and k+1
+  Prev Pp{}, Pq{};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) E0a[d] = E0b[d] = T(0);
+
+  // prologue: u0 of planes k0-2 and k0-1 staged; S = E_old of plane k0;
+  // C of plane k0-2 loaded
+  const int kb = k0 - 2;
+  load_S(kb, S);
+  put_u(sm.u0[0], S.e, S.lz);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) E0a[d] = S.e[d];
+  lza = S.lz;
+  load_S(kb + 1, S);
+  put_u(sm.u0[1], S.e, S.lz);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) E0b[d] = S.e[d];
+  lzb = S.lz;
+  load_S(kb + 2, S);
+  CPk CA, CB;
+  load_C(kb, CA);
+
+  // One plane k.  Cc: update data of plane k (loaded one iteration earlier),
+  // Cn receives plane k+1's.  PH: 1 = E1(k), 2 = H2 and E2 of plane k-2 (the
+  // pipeline-fill iterations before the loop run fewer).  PA: ring slot of
+  // plane k, (k - kb) & 1, a compile-time constant in every unrolled copy.
+  auto iter = [&](const int k, const CPk &Cc, CPk &Cn, auto ph, auto pa)
+      __attribute__((always_inline)) {
+    constexpr int PH = decltype(ph)::value;
+    constexpr int PA = decltype(pa)::value, PB1 = PA ^ 1;
+    load_C(k + 1, Cn);
+    // P. H1(k) -> w1[PA]; H2(k-2) -> w2[PA]
+    T H1n[3] = {T(0), T(0), T(0)};
+    if (rh1) {
+      const T *mt = smt + Cc.m * MAT_STRIDE;
+      const T(*u)[WC] = sm.u0[PA];
+      const T(*un)[WC] = sm.u0[PB1];
+      H1n[0] = Cc.h[0] - (mt[0] * (pm0 * Cc.iax)) * curl_u(u, un, 0);
+      H1n[1] = Cc.h[1] - (mt[1] * (pm1 * Cc.iay)) * curl_u(u, un, 1);
+      H1n[2] = Cc.h[2] - (mt[2] * iAz2) * curl_u(u, un, 2);
+      T(*w)[WC] = sm.w1[PA];
+      w[0][cl] = wxF * H1n[0];
+      w[1][cl] = wyF * H1n[1];
+      w[2][cl] = (pm2 * Cc.tlz) * H1n[2];
+    }
+    if constexpr ((PH & 2) != 0) {
+      if (rh2) {
+        const int km = k - 2;
+        const T *mt = smt + Pq.m * MAT_STRIDE;
+        const T(*u)[WC] = sm.u1[PA];    // plane k-2
+        const T(*un)[WC] = sm.u1[PB1];  // plane k-1
+        const T h0 = Pq.H1[0] - (mt[0] * (pm0 * Pq.iax)) * curl_u(u, un, 0);
+        const T h1 = Pq.H1[1] - (mt[1] * (pm1 * Pq.iay)) * curl_u(u, un, 1);
+        const T h2 = Pq.H1[2] - (mt[2] * iAz2) * curl_u(u, un, 2);
+        T(*w)[WC] = sm.w2[PA];
+        w[0][cl] = wxF * h0;
+        w[1][cl] = wyF * h1;
+        w[2][cl] = (pm2 * Pq.tlz) * h2;
+        if (rout && km >= k0 && km < k1) {
+          const uint32_t v = cb + (uint32_t)km * PB;
+          bstore(rHn, v, 0, h0, NTS);
+          bstore(rHn, v, NB, h1, NTS);
+          bstore(rHn, v, NB2, h2, NTS);
+        }
+      }
+    }
+    __syncthreads();
+
+    // Q. E1(k) -> u1[PA]; E2(k-2); stage u0(k+2) -> u0[PA]
+    T E1n[3] = {T(0), T(0), T(0)};
+    if constexpr ((PH & 1) != 0) {
+      if (re1) {
+        const T *mt = smt + Cc.m * MAT_STRIDE;
+        const T(*w)[WC] = sm.w1[PA];
+        const T(*wm)[WC] = sm.w1[PB1];
+        const T ax = pe0 * Cc.itax;
+        const T ay = pe1 * Cc.itay;
+        const T jx = f.js1 * (mt[6] * E0a[0]);
+        const T jy = f.js1 * (mt[7] * E0a[1]);
+        const T jz = (cbeam && b1on) ? Cc.b1 : f.js1 * (mt[8] * E0a[2]);
+        E1n[0] = E0a[0] + mt[3] * (ax * curl_w(w, wm, 0) - jx);
+        E1n[1] = E0a[1] + mt[4] * (ay * curl_w(w, wm, 1) - jy);
+        E1n[2] = E0a[2] + mt[5] * (itAz2 * curl_w(w, wm, 2) - jz);
+        put_u(sm.u1[PA], E1n, lza);
+        if (k >= k0 && k < k1) {
+          if (probe1) phist[(int64_t)k * a.hist_nt] = E0a[2];
+          if (probe2) phist[(int64_t)k * a.hist_nt + 1] = E1n[2];
+        }
+      }
+    }
+    if constexpr ((PH & 2) != 0) {
+      const int km = k - 2;
+      if (rout && km >= k0 && km < k1) {
+        const T *mt = smt + Pq.m * MAT_STRIDE;
+        const T(*w)[WC] = sm.w2[PA];    // plane k-2
+        const T(*wm)[WC] = sm.w2[PB1];  // plane k-3
+        const T ax = pe0 * Pq.itax;
+        const T ay = pe1 * Pq.itay;
+        const T jx = f.js2 * (mt[6] * Pq.E1[0]);
+        const T jy = f.js2 * (mt[7] * Pq.E1[1]);
+        const T jz = (cbeam && b2on) ? Pq.b2 : f.js2 * (mt[8] * Pq.E1[2]);
+        const uint32_t v = cb + (uint32_t)km * PB;
+        bstore(rEn, v, 0, Pq.E1[0] + mt[3] * (ax * curl_w(w, wm, 0) - jx), NTS);
+        bstore(rEn, v, NB, Pq.E1[1] + mt[4] * (ay * curl_w(w, wm, 1) - jy), NTS);
+        bstore(rEn, v, NB2, Pq.E1[2] + mt[5] * (itAz2 * curl_w(w, wm, 2) - jz), NTS);
+      }
+    }
+    put_u(sm.u0[PA], S.e, S.lz);  // plane k+2 (u0 of plane k is no longer read)
+    // rotate: planes k+1, k+2 become "k, k+1"; k, k-1 become "k-1, k-2"
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      E0a[d] = E0b[d];
+      E0b[d] = S.e[d];
+    }
+    lza = lzb;
+    lzb = S.lz;
+    load_S(k + 3, S);
+    Pq = Pp;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      Pp.H1[d] = H1n[d];
+      Pp.E1[d] = E1n[d];
+    }
+    Pp.m = Cc.m;
+    Pp.iax = Cc.iax;
+    Pp.iay = Cc.iay;
+    Pp.itax = Cc.itax;
+    Pp.itay = Cc.itay;
+    Pp.tlz = Cc.tlz;
+    Pp.b2 = Cc.b2;
+    __syncthreads();
+  };
+
+  __syncthreads();  // u0 of k0-2, k0-1 staged, material table loaded
+  using P0 = std::integral_constant<int, 0>;
+  using P1 = std::integral_constant<int, 1>;
+  using P3 = std::integral_constant<int, 3>;
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  iter(kb, CA, CB, P0{}, S0{});      // H1(k0-2)
+  iter(kb + 1, CB, CA, P1{}, S1{});  // H1, E1 of k0-1
+  // iterations k0 .. k1+1 (H2/E2 of plane k-2 from k0-2 on; only planes in
+  // [k0, k1) are stored), plus one more when their number is odd, so that
+  // the loop body, written out twice, has no conditional part
+  const int kend = ((k1 - k0) & 1) ? k1 + 3 : k1 + 2;
+  for (int k = k0; k < kend; k += 2) {
+    iter(k, CA, CB, P3{}, S0{});
+    iter(k + 1, CB, CA, P3{}, S1{});
+  }
+}
+
 extern __shared__ double fit_dyn_lds[];
 
+// The lean chunks (between the 4-plane z-face chunks) and the others are two
+// kernels, so that each is compiled for its own register need: block b takes
+// chunk b / (ntx nty) of the lean chunks (nremap blocks, under the XCD
+// remap), or of the others, which follow them in the chunk order.
 template <typename T, int TX, int TY, bool NTS>
 __global__ void __launch_bounds__((TX + 4) * (TY + 4)) fused2_sweep(Fused2Args<T> f) {
   __shared__ Smem2<T, TX, TY> sm;
   T *smt = (T *)fit_dyn_lds;
-  const int b = blockIdx.x;
-  const int tile = b < f.nremap ? xcd_tile(b, f.nremap) : b;
-  int k0, k1;
-  chunk2_span(f, tile / (f.ntx * f.nty), k0, k1);
-  // the true planes that feed the chunk's outputs span k0-3 .. k1+2 (window
-  // planes k0-2 .. k1+1, plane carry -1 .. 1); lean when all lie in
-  // 1 .. nz-2 (no z face, no plane outside the grid)
-  if (k0 >= 4 && k1 + 4 <= f.a.nz) fused2_tile<T, TX, TY, NTS, true>(f, tile, sm, smt);
-  else fused2_tile<T, TX, TY, NTS, false>(f, tile, sm, smt);
+  fused2_tile_lean2b<T, TX, TY, NTS>(f, xcd_tile(blockIdx.x, f.nremap), sm, smt);
+}
+
+template <typename T, int TX, int TY, bool NTS>
+__global__ void __launch_bounds__((TX + 4) * (TY + 4)) fused2_sweep_faces(Fused2Args<T> f) {
+  __shared__ Smem2<T, TX, TY> sm;
+  T *smt = (T *)fit_dyn_lds;
+  fused2_tile<T, TX, TY, NTS, false>(f, f.nremap + blockIdx.x, sm, smt);
 }
 
 // Tile of the two-step sweep: 64 x 8 columns, a (68 x 12)-cell window, one
-// thread per window cell (816 threads).  FP32 only: an FP64 window does not
-// fit the LDS.
+// thread per window cell (816 threads).  (64 x 7 balances the waves over the
+// SIMDs, 12 against 13, and ran 10 % faster per plane, but 8 % slower per
+// cell.)  FP32 only: an FP64 window does not fit the LDS.
 constexpr int F2_TX = 64, F2_TY = 8;
 
 template <typename T, int TX, int TY>
@@ -462,18 +766,21 @@ inline int pick_kc2(int ncol, int nz, int slots) {
   return best;
 }
 
-// Whether the two-step sweep serves these step arguments: FP32, full tiles,
-// fields of less than 2^32 bytes, J zero or sigma*E (no J buffer), the
-// kernels spill-free.
+// Whether the two-step sweep serves these step arguments: FP32, nx >= 64 and
+// ny >= 16 (a window cell's flat carry then stays within one row and one
+// plane), fields of less than 2^32 bytes, J zero or sigma*E (no J buffer),
+// the kernels spill-free.
 template <typename T>
 inline bool fused2_supported(const StepArgs<T> &a) {
   if constexpr (!cfg2_fits<T, F2_TX, F2_TY>()) {
     (void)a;
     return false;
   } else {
-    return a.jmode != J_BUFFER && a.buf32 && a.nx % F2_TX == 0 && a.ny % F2_TY == 0 &&
-           a.nz >= 2 && kernel_spill_free((const void *)fused2_sweep<T, F2_TX, F2_TY, false>) &&
-           kernel_spill_free((const void *)fused2_sweep<T, F2_TX, F2_TY, true>);
+    return a.jmode != J_BUFFER && a.buf32 && a.nx >= F2_TX && a.ny >= 16 && a.nz >= 2 &&
+           kernel_spill_free((const void *)fused2_sweep<T, F2_TX, F2_TY, false>) &&
+           kernel_spill_free((const void *)fused2_sweep<T, F2_TX, F2_TY, true>) &&
+           kernel_spill_free((const void *)fused2_sweep_faces<T, F2_TX, F2_TY, false>) &&
+           kernel_spill_free((const void *)fused2_sweep_faces<T, F2_TX, F2_TY, true>);
   }
 }
 
@@ -495,16 +802,18 @@ inline bool launch_fused2(const StepArgs<T> &a, T *En, T *Hn, const T *beam1, co
     f.beam2 = beam2;
     f.js1 = js1 ? T(1) : T(0);
     f.js2 = js2 ? T(1) : T(0);
-    f.ntx = a.nx / TX;
-    f.nty = a.ny / TY;
+    f.ntx = (a.nx + TX - 1) / TX;
+    f.nty = (a.ny + TY - 1) / TY;
     int dev = 0;
     (void)hipGetDevice(&dev);
     const size_t lds = fused_mt_bytes<T>(a.nmat);
     const int slots = resident_slots(dev, (const void *)fused2_sweep<T, TX, TY, false>, NT, lds);
     const int ncol = f.ntx * f.nty;
     f.kc = pick_kc2(ncol, a.nz, slots);
-    // 4-plane chunks at the z faces take the general path, the chunks
-    // between (all lean) come first, under the XCD remap
+    // 4-plane chunks at the z faces take the general path (fused2_sweep_faces),
+    // the chunks between (all lean: every plane their outputs depend on lies
+    // in 1 .. nz-2) the two-barrier lean path (fused2_sweep); a grid too short
+    // for a lean chunk takes the general path throughout
     constexpr int KH = 4;
     if (a.nz - 2 * KH >= f.kc) {
       f.kh = KH;
@@ -515,13 +824,21 @@ inline bool launch_fused2(const StepArgs<T> &a, T *En, T *Hn, const T *beam1, co
       f.kh = 0;
       f.nmid = 0;
       f.nchunk = (a.nz + f.kc - 1) / f.kc;
-      f.nremap = ncol * f.nchunk;
+      f.nremap = 0;
     }
     f.nb = ncol * f.nchunk;
+    const int nfaces = f.nb - f.nremap;
+    if (f.nremap > 0) {
+      if (a.nt)
+        hipLaunchKernelGGL((fused2_sweep<T, TX, TY, true>), dim3(f.nremap), dim3(NT), lds, s, f);
+      else
+        hipLaunchKernelGGL((fused2_sweep<T, TX, TY, false>), dim3(f.nremap), dim3(NT), lds, s, f);
+    }
     if (a.nt)
-      hipLaunchKernelGGL((fused2_sweep<T, TX, TY, true>), dim3(f.nb), dim3(NT), lds, s, f);
+      hipLaunchKernelGGL((fused2_sweep_faces<T, TX, TY, true>), dim3(nfaces), dim3(NT), lds, s, f);
     else
-      hipLaunchKernelGGL((fused2_sweep<T, TX, TY, false>), dim3(f.nb), dim3(NT), lds, s, f);
+      hipLaunchKernelGGL((fused2_sweep_faces<T, TX, TY, false>), dim3(nfaces), dim3(NT), lds, s,
+                         f);
     return true;
   }
 }
