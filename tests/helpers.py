@@ -115,3 +115,37 @@ def stratified_pair(n=12, nlayers=4, n_pml=2, dtype=np.float64, device=0, seed=5
     psv = wa.SolverFIT3D(wa.GridFIT3D(x=g["x"], y=g["y"], z=g["z"], solids=g["solids"]),
                          dtype=dtype, device=device, kernel=kernel, **sv)
     return osv, psv, wl
+
+
+def oracle_from_workload(w, dtype=np.float64):
+    """The oracle solver (with its wake) of a wakis_amd.workloads dict whose
+    solids are (mask, material) pairs; the reference's geomspace PML grading
+    (a workload with pml_profile "cubic" gets the cubic sigma through the
+    sigma Field and update_tensors, the reference's route to per-cell
+    conductivities)."""
+    g, sv = w["grid"], w["solver"]
+    og = OGrid(x=g["x"], y=g["y"], z=g["z"])
+    osv = OSolver(og, OWake(**w["wake"]), cfln=sv.get("cfln", 0.5), bc_low=sv["bc_low"],
+                  bc_high=sv["bc_high"], solids=list(g["solids"]), bg=sv["bg"], dtype=dtype,
+                  n_pml=sv.get("n_pml", 10), use_sibc=sv.get("use_sibc", True))
+    if sv.get("pml_profile", "geomspace") == "cubic":
+        n, nz = sv["n_pml"], osv.Nz
+        assert [b.lower() for b in sv["bc_low"]][:2] != ["pml", "pml"]  # z faces only
+        for p in range(n):
+            for d in "xyz":
+                osv.sigma[:, :, p, d] = (float(n - p) / n) ** 3
+                osv.sigma[:, :, nz - n + p, d] = (float(p + 1) / n) ** 3
+        osv.update_tensors("sigma")
+        osv.step_0 = True
+    return osv
+
+
+def z_errors(z, z_ref, rel_floor=1e-2):
+    """(max |Z - Z_ref| / max |Z_ref|, max over bins with |Z_ref| >= rel_floor
+    max |Z_ref| of |Z - Z_ref| / |Z_ref|) over the kept bins (all below the
+    bunch cutoff fmax)."""
+    z, z_ref = np.asarray(z), np.asarray(z_ref)
+    zmax = np.max(np.abs(z_ref))
+    d = np.abs(z - z_ref)
+    big = np.abs(z_ref) >= rel_floor * zmax
+    return float(np.max(d) / zmax), float(np.max(d[big] / np.abs(z_ref[big])))
