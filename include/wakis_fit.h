@@ -191,6 +191,13 @@ int fit_step(fit_ctx *ctx, int32_t nsteps, int32_t conductive);
  * one-step launches (a fused sweep, or a split H/E pair), counts[1] two-step
  * sweeps.  No counterpart in the reference. */
 int fit_launch_counts(fit_ctx *ctx, int64_t *counts);
+/* Device memory of the context: *now the bytes of the buffers it holds
+ * (fields, second field buffers, material ids and table, metric tables,
+ * beam tables, probe history, staging, monitor), *peak the most it has held
+ * at once since fit_create, the scratch that fit_wake_potential and
+ * fit_impedance hold during the call included.  A device admission budget
+ * is checked against these.  No counterpart in the reference. */
+int fit_device_bytes(fit_ctx *ctx, int64_t *now, int64_t *peak);
 /* fit_step timed with HIP events on the context stream (synchronises).
  * per_kernel = 0: ms[0] = elapsed time of the whole launch sequence.
  * per_kernel = 1: events around every launch; ms[1] = summed time of the
@@ -297,6 +304,11 @@ int fit_batch_set_members(fit_batch *batch, fit_ctx *const *members, int32_t n,
 int fit_batch_destroy(fit_batch *batch);
 /* Step every member nsteps; ms (optional) receives the elapsed time. */
 int fit_batch_step(fit_batch *batch, int64_t nsteps, double *ms);
+/* Device bytes of the batch's own tables (member table, tile offsets, step
+ * counter, beam tables: group x nz values per member with a Gaussian beam);
+ * they grow to the largest member set seen and are freed by
+ * fit_batch_destroy. */
+int fit_batch_device_bytes(fit_batch *batch, int64_t *bytes);
 /* Tile configuration of the batched launch (0 .. FIT_NTILE-1 of the fused
  * sweep family; -1: default for the widest member). */
 int fit_batch_set_tile(fit_batch *batch, int32_t cfg);

@@ -1,0 +1,241 @@
+"""Serving admission without a device: a fake context that counts the bytes
+the library would allocate (the sizes of fit_create, fit_set_geometry,
+fit_set_materials, fit_wake_probe, the sweep's second buffers and the finish
+scratch) shows that a burst of requests never holds more device memory than
+the budget, that nothing is allocated before admission, that a request too
+large for the budget is rejected, and that a virtual clock is advanced over
+idle gaps."""
+
+import threading
+
+import numpy as np
+import pytest
+
+from wakis_amd import _lib, serve
+
+
+class Ledger:
+    def __init__(self):
+        self.lock = threading.Lock()
+        self.live = 0
+        self.peak = 0
+        self.created = []  # (request grid shape, live bytes after creation)
+
+    def add(self, n):
+        with self.lock:
+            self.live += n
+            self.peak = max(self.peak, self.live)
+
+    def transient(self, n):
+        with self.lock:
+            self.peak = max(self.peak, self.live + n)
+
+
+class FakeContext:
+    ledger: Ledger = None
+
+    def __init__(self, nx, ny, nz, dtype=np.float64, device=0):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.N = nx * ny * nz
+        self.esz = np.dtype(dtype).itemsize
+        self.np_dtype = np.dtype(dtype)
+        self.held = {}
+        self.peak = 0
+        self.closed = False
+        self._hold("fields", 9 * self.N * self.esz + self.N + nz * (self.esz + 8))
+        self.ledger.created.append((nx, ny, nz))
+
+    def _hold(self, key, n):
+        self.ledger.add(n - self.held.get(key, 0))
+        self.held[key] = n
+        self.peak = max(self.peak, sum(self.held.values()))
+
+    def _transient(self, n):
+        self.ledger.transient(n)
+        self.peak = max(self.peak, sum(self.held.values()) + n)
+
+    def set_geometry(self, packed):
+        self._hold("geo", np.asarray(packed).size * self.esz)
+
+    def set_materials(self, ids, table):
+        self._hold("mtab", np.asarray(table).shape[0] * 9 * self.esz)
+
+    def wake_probe(self, ixt, iyt, h, nt):
+        self.probe = (ixt, iyt, h, int(nt))
+        self._hold("hist", (2 * h + 1) ** 2 * self.nz * int(nt) * self.esz)
+
+    def second_buffers(self):
+        self._hold("alt", 6 * self.N * self.esz)
+
+    def wake_potential(self, window_ids, carry, cols, k0, dz, qnorm):
+        nzw, ns = np.asarray(cols).shape
+        nwin = len(window_ids)
+        self._transient(4 * nwin + 4 * max(nzw * ns, 1) + 8 * max(nzw, 1) + 8 * nwin * ns)
+        return np.zeros((nwin, ns))
+
+    def impedance(self, x, lam, N, K, ds, noalloc=False):
+        self._transient(16 * min(len(x), N) + 48 * K)
+        z = np.zeros(K, dtype=complex)
+        return z, z.copy(), z.copy()
+
+    def device_bytes(self):
+        return sum(self.held.values()), self.peak
+
+    def close(self):
+        if not self.closed:
+            self.ledger.add(-sum(self.held.values()))
+            self.held = {}
+            self.closed = True
+
+    def set_bc(self, *a): pass
+    def set_dt(self, *a): pass
+    def set_kernel(self, *a): pass
+    def set_tile(self, *a): pass
+    def set_step_index(self, *a): pass
+    def set_beam_gauss(self, *a): pass
+    def set_beam_values(self, *a): pass
+    def beam_off(self): pass
+    def zero_ghosts(self): pass
+    def sync(self): pass
+
+
+class FakeBatch:
+    """Steps nothing; members get their second field buffers on joining, as
+    the batched sweep's member check allocates them."""
+
+    def __init__(self, solvers, graph=True, group=16, tile=None):
+        self.group = group
+        self.set_members(solvers)
+
+    def set_members(self, solvers):
+        for s in solvers:
+            assert not s.ctx.closed
+            s.ctx.second_buffers()
+        self.members = list(solvers)
+
+    def step(self, n):
+        return 0.1 * n
+
+    def device_bytes(self):
+        return sum(self.group * s.Nz * s.ctx.esz + 4096 for s in self.members)
+
+    def close(self):
+        pass
+
+
+class VClock:
+    """Virtual seconds: 1 ms per step the server takes, plus idle jumps."""
+
+    def __init__(self):
+        self.server = None
+        self.offset = 0.0
+
+    def __call__(self):
+        steps = 0 if self.server is None else sum(e[2] for e in self.server.log)
+        return steps * 1e-3 + self.offset
+
+    def wait_until(self, t):
+        self.offset += max(0.0, t - self())
+
+
+@pytest.fixture
+def fake_device(monkeypatch):
+    led = Ledger()
+    FakeContext.ledger = led
+    monkeypatch.setattr(_lib, "Context", FakeContext)
+    monkeypatch.setattr(serve, "BatchStepper", FakeBatch)
+    return led
+
+
+def _burst(n, arrivals=None):
+    reqs = serve.make_requests(n, seed=7, median_cells=2e4, sigma_ln=0.4, clip=(1e4, 4e4),
+                               slab_prob=0.5, wakelength=0.05)
+    for i, r in enumerate(reqs):
+        r.workload["solver"]["n_pml"] = 4
+        r.arrival = 0.0 if arrivals is None else arrivals[i]
+    return reqs
+
+
+def _model(r, esz=4):
+    from wakis_amd.workloads import build
+
+    _, _, s = build(r.workload, dtype=np.float32, device=None, precision="mixed")
+    run = s._wake_plan(None, None, "longitudinal")
+    return serve.device_bytes(s.Nx, s.Ny, s.Nz, run["Nt"], esz, nmat=len(s._mt.table()),
+                              half=run["h"], group=8)
+
+
+@pytest.mark.timeout(300)
+def test_burst_stays_within_budget(fake_device):
+    reqs = _burst(10)
+    models = {r.rid: _model(r) for r in reqs}
+    budget = 2.5 * max(models.values())
+    clock = VClock()
+    srv = serve.Server(dtype=np.float32, budget=budget, chunk_steps=50, group=8, clock=clock)
+    clock.server = srv
+    done = srv.run(reqs)
+    assert all(r.result is not None and "error" not in r.result for r in done)
+    # the device never held more than the budget, the finish scratch included
+    assert fake_device.peak <= budget, (fake_device.peak, budget)
+    assert fake_device.live == 0
+    assert len(fake_device.created) == len(reqs)
+    # admission reserved at most the budget, and the burst had to queue
+    assert srv.peak_admitted <= budget
+    assert max(e[1] for e in srv.log) >= 2
+    assert any(r.admitted > 0 for r in done)
+    for r in done:
+        # the model bounds what each request held at once
+        assert r.result["device_peak"] <= r.result["device_model"] == models[r.rid]
+        assert r.admitted >= r.arrival and r.finished >= r.admitted
+    for e in srv.log:
+        assert e[6] <= e[3] <= budget  # stepped members' bytes <= admitted bytes
+
+
+@pytest.mark.timeout(300)
+def test_oversized_request_is_rejected(fake_device):
+    reqs = _burst(4)
+    models = {r.rid: _model(r) for r in reqs}
+    big = max(models, key=models.get)
+    budget = sorted(models.values())[-2] * 1.01  # every request but the largest fits
+    clock = VClock()
+    srv = serve.Server(dtype=np.float32, budget=budget, chunk_steps=50, group=8, clock=clock)
+    clock.server = srv
+    done = srv.run(reqs)
+    for r in done:
+        if r.rid == big:
+            assert "error" in r.result and "budget" in r.result["error"]
+        else:
+            assert "error" not in r.result
+    assert fake_device.peak <= budget
+    assert len(fake_device.created) == len(reqs) - 1
+
+
+@pytest.mark.timeout(300)
+def test_virtual_clock_crosses_idle_gaps(fake_device):
+    # the second request arrives long after the first is done: the server
+    # must move the virtual clock to it instead of waiting for ever
+    reqs = _burst(2, arrivals=[0.0, 50.0])
+    clock = VClock()
+    srv = serve.Server(dtype=np.float32, budget=288e9, chunk_steps=50, group=8, clock=clock)
+    clock.server = srv
+    done = srv.run(reqs)
+    assert all("error" not in r.result for r in done)
+    assert done[1].admitted >= 50.0
+    assert done[0].finished < 50.0
+
+
+def test_model_terms():
+    # the model is the library's allocation list, term by term
+    nx, ny, nz, nt, esz = 20, 22, 60, 300, 4
+    n = nx * ny * nz
+    b = serve.device_bytes(nx, ny, nz, nt, esz, nmat=3, half=0, group=8, samples=11)
+    fields = 15 * n * esz + n
+    metric = (2 * (nx + ny + nz) + 2 * (ny * nz + nx * nz + nx * ny)) * esz
+    tables = 27 * esz + nz * (esz + 8)
+    hist = nz * nt * esz
+    ns = nt + 2
+    finish = 4 + 4 * nz * ns + 8 * nz + 8 * ns + 16 * ns + 48 * 11
+    assert b == fields + metric + tables + hist + 8 * nz * esz + 4096 + finish
+    # a 3 x 3 probe window holds nine histories
+    b9 = serve.device_bytes(nx, ny, nz, nt, esz, nmat=3, half=1, group=8, samples=11)
+    assert b9 - b == 8 * nz * nt * esz + 32 + 64 * ns

@@ -79,6 +79,7 @@ SIGNATURES = {
     "fit_set_kernel": [_P, _I32],
     "fit_kernel_in_use": [_P, _PI32],
     "fit_launch_counts": [_P, C.POINTER(_I64)],
+    "fit_device_bytes": [_P, C.POINTER(_I64), C.POINTER(_I64)],
     "fit_set_tile": [_P, _I32],
     "fit_set_chunk_rsrc": [_P, _I32],
     "fit_get_tile": [_P, _PI32, _PI32, _PI32, _PI32],
@@ -92,6 +93,7 @@ SIGNATURES = {
     "fit_batch_set_members": [_P, C.POINTER(_P), _I32, _PI32],
     "fit_batch_destroy": [_P],
     "fit_batch_step": [_P, _I64, _PD],
+    "fit_batch_device_bytes": [_P, C.POINTER(_I64)],
     "fit_batch_set_tile": [_P, _I32],
     "fit_batch_get_tile": [_P, _PI32, _PI32, _PI32],
     "fit_batch_tune": [_P, _I32, _PD],
@@ -315,6 +317,12 @@ class Context:
         check(self.lib.fit_launch_counts(self.h, v), "fit_launch_counts")
         return int(v[0]), int(v[1])
 
+    def device_bytes(self):
+        """(bytes held now, most bytes held at once) on the device."""
+        now, peak = C.c_int64(0), C.c_int64(0)
+        check(self.lib.fit_device_bytes(self.h, C.byref(now), C.byref(peak)), "fit_device_bytes")
+        return int(now.value), int(peak.value)
+
     def set_tile(self, cfg: int):
         check(self.lib.fit_set_tile(self.h, int(cfg)), "fit_set_tile")
 
@@ -435,6 +443,12 @@ class Batch:
         out = np.zeros(1, dtype=np.float64)
         check(self.lib.fit_batch_step(self.h, int(nsteps), dptr(out)), "fit_batch_step")
         return float(out[0])
+
+    def device_bytes(self) -> int:
+        """Device bytes of the batch's own tables."""
+        v = C.c_int64(0)
+        check(self.lib.fit_batch_device_bytes(self.h, C.byref(v)), "fit_batch_device_bytes")
+        return int(v.value)
 
     def set_tile(self, cfg: int):
         check(self.lib.fit_batch_set_tile(self.h, int(cfg)), "fit_batch_set_tile")

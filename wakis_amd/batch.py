@@ -58,6 +58,10 @@ class BatchStepper:
     def tile(self):
         return self._b.get_tile()
 
+    def device_bytes(self) -> int:
+        """Device bytes of the batch's own tables (the members' are theirs)."""
+        return self._b.device_bytes()
+
     def close(self):
         self._b.close()
 

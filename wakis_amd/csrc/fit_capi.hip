@@ -82,6 +82,11 @@ struct fit_ctx {
   // launches of the step kernels so far: [0] one-step (fused or split pair),
   // [1] two-step fused sweeps
   int64_t launches[2] = {0, 0};
+  // device bytes (fit_device_bytes): sizes of the buffers above that are not
+  // fixed by the grid, and the high-water mark of everything the context has
+  // held at once, the scratch of fit_wake_potential / fit_impedance included
+  size_t mtab_bytes = 0, hist_bytes = 0, mon_bytes = 0, mon_ph_bytes = 0;
+  int64_t peak_bytes = 0;
 };
 
 namespace {
@@ -89,6 +94,27 @@ namespace {
 int set_device(fit_ctx *c) {
   HIPCHK(hipSetDevice(c->device));
   return 0;
+}
+
+// Device bytes the context holds now.
+int64_t resident_bytes(const fit_ctx *c) {
+  const int64_t fb = 3 * c->N * (int64_t)c->esz;
+  int64_t b = 0;
+  for (const void *p : {c->E, c->H, c->J, c->E_alt, c->H_alt})
+    if (p) b += fb;
+  if (c->mat) b += c->N;
+  if (c->geo) b += c->geo_count * (int64_t)c->esz;
+  if (c->jline) b += (int64_t)c->nz * c->esz;
+  if (c->zc) b += (int64_t)c->nz * 8;
+  if (c->btab) b += c->btab_steps * c->nz * (int64_t)c->esz;
+  b += (int64_t)(c->mtab_bytes + c->hist_bytes + c->staging_bytes + c->mon_bytes + c->mon_ph_bytes);
+  return b;
+}
+
+// Raise the high-water mark to the resident bytes plus `extra` held for the
+// duration of a call.
+void note_peak(fit_ctx *c, int64_t extra = 0) {
+  c->peak_bytes = std::max(c->peak_bytes, resident_bytes(c) + extra);
 }
 
 int ensure_staging(fit_ctx *c, size_t bytes) {
@@ -99,6 +125,7 @@ int ensure_staging(fit_ctx *c, size_t bytes) {
   if (hipMalloc(&c->staging, bytes) != hipSuccess)
     return fail(FIT_ENOMEM, "staging allocation failed");
   c->staging_bytes = bytes;
+  note_peak(c);
   return 0;
 }
 
@@ -117,8 +144,11 @@ int upload_mtab(fit_ctx *c) {
   }
   if (c->mtab) HIPCHK(hipFree(c->mtab));
   c->mtab = nullptr;
+  c->mtab_bytes = 0;
   if (hipMalloc(&c->mtab, t.size() * c->esz) != hipSuccess)
     return fail(FIT_ENOMEM, "material table allocation failed");
+  c->mtab_bytes = t.size() * c->esz;
+  note_peak(c);
   if (c->dtype == FIT_F64) {
     HIPCHK(hipMemcpy(c->mtab, t.data(), t.size() * 8, hipMemcpyHostToDevice));
   } else {
@@ -212,8 +242,10 @@ int want_fused(fit_ctx *c, bool *fused) {
       (void)hipGetLastError();
       if (c->kernel == FIT_KERNEL_FUSED)
         return fail(FIT_ENOMEM, "fused sweep: second field buffers do not fit");
+      note_peak(c);
       return 0;
     }
+    note_peak(c);
   }
   *fused = true;
   return 0;
@@ -242,6 +274,7 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
       if (hipMalloc(&c->btab, (size_t)nsteps * c->nz * c->esz) != hipSuccess)
         return fail(FIT_ENOMEM, "beam table allocation failed");
       c->btab_steps = nsteps;
+      note_peak(c);
     }
     fit::StepArgs<T> a = make_args<T>(c);
     const int64_t cnt = (int64_t)nsteps * c->nz;
@@ -478,6 +511,7 @@ int fit_create(const fit_desc *d, fit_ctx **out) {
     fit_destroy(c);
     return fail(FIT_EHIP, "field initialisation failed");
   }
+  note_peak(c);
   *out = c;
   return 0;
 }
@@ -505,16 +539,18 @@ int fit_set_geometry(fit_ctx *c, const double *packed, int64_t count) {
   if (int rc = set_device(c)) return rc;
   if (c->geo) HIPCHK(hipFree(c->geo));
   c->geo = nullptr;
+  c->geo_count = 0;
   if (hipMalloc(&c->geo, (size_t)count * c->esz) != hipSuccess)
     return fail(FIT_ENOMEM, "geometry allocation failed");
+  c->geo_count = count;
   if (c->dtype == FIT_F64) {
     HIPCHK(hipMemcpy(c->geo, packed, (size_t)count * 8, hipMemcpyHostToDevice));
   } else {
     std::vector<float> f(packed, packed + count);
     HIPCHK(hipMemcpy(c->geo, f.data(), (size_t)count * 4, hipMemcpyHostToDevice));
   }
-  c->geo_count = count;
   c->geo_set = true;
+  note_peak(c);
   return 0;
 }
 
@@ -735,6 +771,7 @@ int fit_wake_probe(fit_ctx *c, int32_t ixt, int32_t iyt, int32_t h, int64_t nt_c
   if (c->hist) HIPCHK(hipFree(c->hist));
   c->hist = nullptr;
   c->hist_nt = 0;
+  c->hist_bytes = 0;
   if (nt_cap <= 0) return 0;
   if (h < 0 || h > 1) return fail(FIT_EINVAL, "window half-width must be 0 or 1");
   if (ixt - h < 0 || ixt + h >= c->nx || iyt - h < 0 || iyt + h >= c->ny)
@@ -743,6 +780,8 @@ int fit_wake_probe(fit_ctx *c, int32_t ixt, int32_t iyt, int32_t h, int64_t nt_c
   const size_t bytes = (size_t)nw * c->nz * nt_cap * c->esz;
   if (hipMalloc(&c->hist, bytes) != hipSuccess)
     return fail(FIT_ENOMEM, "probe history allocation failed");
+  c->hist_bytes = bytes;
+  note_peak(c);
   HIPCHK(hipMemsetAsync(c->hist, 0, bytes, c->stream));
   c->hist_nt = nt_cap;
   c->ixt = ixt;
@@ -770,6 +809,7 @@ int fit_monitor_set(fit_ctx *c, int32_t nf) {
   if (c->mon) HIPCHK(hipFree(c->mon));
   c->mon = nullptr;
   c->mon_nf = 0;
+  c->mon_bytes = 0;
   c->mon_ph_steps = 0;
   if (nf == 0) return 0;
   const size_t bytes = (size_t)3 * nf * c->N * 16;
@@ -778,6 +818,8 @@ int fit_monitor_set(fit_ctx *c, int32_t nf) {
     (void)hipGetLastError();
     return fail(FIT_ENOMEM, "field monitor accumulators do not fit");
   }
+  c->mon_bytes = bytes;
+  note_peak(c);
   HIPCHK(hipMemsetAsync(c->mon, 0, bytes, c->stream));
   c->mon_nf = nf;
   return 0;
@@ -790,12 +832,15 @@ int monitor_stage(fit_ctx *c, const double *rows, int64_t nrows) {
     if (c->mon_ph) HIPCHK(hipFree(c->mon_ph));
     c->mon_ph = nullptr;
     c->mon_ph_cap = 0;
+    c->mon_ph_bytes = 0;
     if (hipMalloc((void **)&c->mon_ph, (size_t)nrows * 2 * c->mon_nf * 8) != hipSuccess) {
       c->mon_ph = nullptr;
       (void)hipGetLastError();
       return fail(FIT_ENOMEM, "field monitor phase table allocation failed");
     }
     c->mon_ph_cap = nrows;
+    c->mon_ph_bytes = (size_t)nrows * 2 * c->mon_nf * 8;
+    note_peak(c);
   }
   // the previous use of the table (launches of an earlier call) must be done
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1095,6 +1140,13 @@ int fit_kernel_in_use(fit_ctx *c, int32_t *kernel) {
   return 0;
 }
 
+int fit_device_bytes(fit_ctx *c, int64_t *now, int64_t *peak) {
+  if (!c || !now || !peak) return fail(FIT_EINVAL, "null argument");
+  *now = resident_bytes(c);
+  *peak = c->peak_bytes;
+  return 0;
+}
+
 int fit_launch_counts(fit_ctx *c, int64_t *counts) {
   if (!c || !counts) return fail(FIT_EINVAL, "null argument");
   counts[0] = c->launches[0];
@@ -1134,6 +1186,8 @@ int fit_wake_potential(fit_ctx *c, int32_t nwin, const int32_t *wids, int32_t ca
     (void)hipFree(dwp);
     return fail(FIT_ENOMEM, "wake potential scratch allocation failed");
   }
+  note_peak(c, (int64_t)nwin * 4 + (int64_t)cb + (int64_t)std::max(nzw, 1) * 8 +
+                   (int64_t)nwin * ns * 8);
   int rc = 0;
   hipError_t e = hipMemcpyAsync(dw, wids, nwin * 4, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess && nzw > 0)
@@ -1179,6 +1233,7 @@ int fit_impedance(fit_ctx *c, const double *x, const double *lam, int64_t len, i
     (void)hipFree(dout);
     return fail(FIT_ENOMEM, "impedance scratch allocation failed");
   }
+  note_peak(c, 16 * M + (int64_t)K * 48);
   std::vector<double> out((size_t)K * 6);
   hipError_t e = hipMemcpyAsync(dx, x, M * 8, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(dl, lam, M * 8, hipMemcpyHostToDevice, c->stream);

@@ -106,6 +106,13 @@ class RoutinesMixin:
     def _wake_setup(self, wakelength, wake, compute_plane, add_space=None):
         """Test point, Nt, beam and E_z probe of a wake run
         (wakis/routines.py:366-432)."""
+        run = self._wake_plan(wakelength, wake, compute_plane, add_space)
+        self._wake_attach(run)
+        return run
+
+    def _wake_plan(self, wakelength, wake, compute_plane, add_space=None):
+        """The host part of _wake_setup: test point, Nt, beam and time axis
+        (no device calls)."""
         if wake is not None:
             self.wake = wake
         if self.wake is None:
@@ -128,9 +135,12 @@ class RoutinesMixin:
         beam = Beam(q=self.wake.q, sigmaz=self.wake.sigmaz, beta=self.wake.beta,
                     xsource=self.wake.xsource, ysource=self.wake.ysource)
         t_axis = np.arange(0, Nt * self.dt, self.dt)
-        self.ctx.wake_probe(self.ixt, self.iyt, h, max(Nt, 1))
-        self.ctx.set_step_index(0)
         return {"Nt": Nt, "beam": beam, "h": h, "t": t_axis, "t0": time.time()}
+
+    def _wake_attach(self, run):
+        """The device part of _wake_setup: the E_z probe history."""
+        self.ctx.wake_probe(self.ixt, self.iyt, run["h"], max(run["Nt"], 1))
+        self.ctx.set_step_index(0)
 
     def _wake_device_beam(self, run):
         """The Gaussian bunch evaluated on the device at every step."""

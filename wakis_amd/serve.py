@@ -12,9 +12,13 @@ potential and impedance are then computed on the device and its memory is
 released.  Each member's arithmetic is the same as a solve alone, so the
 results do not depend on the mix.
 
-The host side of a new request (geometry, materials, metric tables, upload)
-is prepared by a worker thread while the device steps the current batch
-(ctypes releases the GIL during library calls).
+The host side of a new request (geometry, materials, metric tables) is
+prepared by worker threads without touching the device.  Admission reserves
+the request's device bytes (device_bytes, an upper bound of what the library
+allocates for it) against the budget; only then does a device worker
+allocate and upload it, and, after its last step, compute its wake and
+impedance and free it, while the main thread steps the batch (ctypes
+releases the GIL during library calls).
 
 There is no counterpart in the reference (it runs one solve per process,
 wakis/routines.py:231-485).
@@ -132,64 +136,126 @@ def make_requests(n, seed=4, rate=2.0, median_cells=8e6, sigma_ln=1.0,
     return out
 
 
-def device_bytes(n_cells: int, nz: int, nt: int, esz: int) -> int:
-    """Device memory of one resident solve: E, H, their second buffers and J
-    (5 x 3N), material ids, metric tables, the E_z probe history (one
-    window) and slack for the beam table."""
-    return int(15 * n_cells * esz + n_cells + 64 * (n_cells ** (2 / 3)) * esz + nz * nt * esz + (1 << 20))
+def device_bytes(nx, ny, nz, nt, esz, nmat=256, half=0, group=16, samples=1001):
+    """Most device bytes one served solve holds at once, term by term as the
+    library allocates them (fit_device_bytes reports the real figure):
+
+    fields      E, H, J and the sweep's second E, H buffers: 15 N esz
+    ids         the uint8 material id of every cell: N
+    metric      the packed metric tables: (2 (nx+ny+nz) + 2 (ny nz + nx nz + nx ny)) esz
+    materials   nmat rows of dt/mu, dt/eps, sigma: 9 nmat esz
+    lines       the beam line and cell centres: nz (esz + 8)
+    history     the E_z probe: (2 half + 1)^2 nz nt esz
+    batch       this member's rows of the batch's beam table (group nz esz) and
+                its entries in the member tables (4 KiB)
+    finish      the wake potential's scratch (window ids, nz x ns time columns,
+                dz, the WP rows) and the impedance DFT's (2 ns + 6 K doubles),
+                with ns <= nt + 2 wake samples and K <= samples DFT bins; both
+                are counted although they are held one after the other."""
+    n = nx * ny * nz
+    nwin = (2 * half + 1) ** 2
+    ns = nt + 2
+    fields = 15 * n * esz + n
+    metric = (2 * (nx + ny + nz) + 2 * (ny * nz + nx * nz + nx * ny)) * esz
+    tables = 9 * nmat * esz + nz * (esz + 8)
+    history = nwin * nz * max(nt, 1) * esz
+    batch = group * nz * esz + 4096
+    finish = (4 * nwin + 4 * nz * ns + 8 * nz + 8 * nwin * ns) + (16 * ns + 48 * samples)
+    return int(fields + metric + tables + history + batch + finish)
 
 
 class Server:
     """Continuous-batching server on one device.
 
-    budget: device bytes the resident solves may use (default 288 GB).
-    chunk_steps: steps per batch call (a request admitted while a chunk runs
-    joins at the next call).  clock: seconds since start (wall time by
-    default; tests pass a virtual clock)."""
+    budget: device bytes the resident solves may use (default 288 GB); a
+    request is admitted only when its device_bytes fit next to those already
+    admitted, and nothing of it is allocated on the device before that, so
+    the device memory of the served solves (fit_device_bytes, plus the
+    batch's own tables) stays within the budget under any burst.  A request
+    whose device_bytes alone exceed the budget is rejected: its result is
+    {"error": ...}.  chunk_steps: steps per batch call (a request attached
+    while a chunk runs joins at the next call).  clock: seconds since start
+    (wall time by default); a caller's clock that advances only with the
+    server's own work (a virtual clock) should have wait_until(t), which the
+    server calls when it is idle until the next arrival at t.
+    device: the device the solves run on."""
 
     def __init__(self, dtype=np.float32, budget=288e9, chunk_steps=256, graph=True, group=16,
                  tile=None, compute_plane="longitudinal", clock: Callable[[], float] | None = None,
-                 prep_workers=2, precision="mixed"):
+                 prep_workers=2, precision="mixed", device=0):
         self.dtype = np.dtype(dtype)
         self.precision = precision
         self.budget = float(budget)
         self.chunk_steps = int(chunk_steps)
         self.graph, self.group, self.tile = graph, group, tile
         self.compute_plane = compute_plane
+        self.device = device
         self._t0 = None
         self._clock = clock
         self.prep_workers = prep_workers
+        # per chunk: (time, members, steps, admitted bytes, ms, cells,
+        #             device bytes of the stepped members and the batch)
         self.log = []
+        self.peak_admitted = 0.0
 
     def now(self) -> float:
         if self._clock is not None:
             return self._clock()
         return time.monotonic() - self._t0
 
-    # -- host preparation of a request (worker thread) ------------------------
+    # -- host preparation of a request (worker thread, no device calls) -------
     def _prepare(self, req: Request):
         t0 = time.monotonic()
-        grid, wake, solver = build(req.workload, dtype=self.dtype, precision=self.precision)
-        if self.tile is not None:
-            solver.ctx.set_tile(int(self.tile))
-        run = solver._wake_setup(None, None, self.compute_plane)
-        solver._wake_device_beam(run)
-        solver.ctx.sync()
-        esz = self.dtype.itemsize
-        mem = device_bytes(solver.N, solver.Nz, run["Nt"], esz)
+        grid, wake, solver = build(req.workload, dtype=self.dtype, device=None,
+                                   precision=self.precision)
+        run = solver._wake_plan(None, None, self.compute_plane)
+        mem = device_bytes(solver.Nx, solver.Ny, solver.Nz, run["Nt"], self.dtype.itemsize,
+                           nmat=len(solver._mt.table()), half=run["h"], group=self.group)
         req.prep_s = time.monotonic() - t0
         return {"req": req, "solver": solver, "wake": wake, "run": run, "done": 0, "mem": mem}
+
+    # -- device work of a request (the device worker thread) ------------------
+    def _attach(self, it):
+        """Allocate and upload an admitted request and arm its beam and probe."""
+        t0 = time.monotonic()
+        s, run = it["solver"], it["run"]
+        s.attach_device(self.device, tile=self.tile)
+        s._wake_attach(run)
+        s._wake_device_beam(run)
+        s.ctx.sync()
+        it["attach_s"] = time.monotonic() - t0
+        return it
+
+    def _finish(self, it):
+        """Wake potential and impedance of a request after its last step;
+        frees its device memory."""
+        tf = time.monotonic()
+        s = it["solver"]
+        try:
+            s._wake_device_end(it["run"])
+            s._wake_finish(it["run"], self.compute_plane, save_logs=False)
+            w = it["wake"]
+            it["req"].result = {"WP": np.asarray(w.WP).copy(), "Z": np.asarray(w.Z).copy(),
+                                "f": np.asarray(w.f).copy(), "s": np.asarray(w.s).copy(),
+                                "Nt": it["run"]["Nt"], "N": s.N,
+                                "device_peak": s.ctx.device_bytes()[1], "device_model": it["mem"]}
+        finally:
+            s.ctx.close()
+        it["req"].finish_s = time.monotonic() - tf
+        return it
 
     def run(self, requests):
         """Serve the requests (sorted by arrival); returns them with latency,
         steps and results filled in."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         self._t0 = time.monotonic()
-        ready: queue.Queue = queue.Queue()
-        todo: queue.Queue = queue.Queue()
+        ready: queue.Queue = queue.Queue()    # host-prepared, not admitted
+        todo: queue.Queue = queue.Queue()     # arrived, not prepared
+        dev_in: queue.Queue = queue.Queue()   # ("attach" | "finish", item)
+        dev_out: queue.Queue = queue.Queue()  # (kind, item or error)
         stop = threading.Event()
 
-        def worker():
+        def prep_worker():
             while not stop.is_set():
                 try:
                     r = todo.get(timeout=0.05)
@@ -200,13 +266,28 @@ class Server:
                 except BaseException as e:  # surfaced by the main loop
                     ready.put({"req": r, "error": e})
 
-        threads = [threading.Thread(target=worker, daemon=True) for _ in range(self.prep_workers)]
+        def device_worker():
+            while not stop.is_set():
+                try:
+                    kind, it = dev_in.get(timeout=0.05)
+                except queue.Empty:
+                    continue
+                try:
+                    dev_out.put((kind, self._attach(it) if kind == "attach" else self._finish(it)))
+                except BaseException as e:
+                    dev_out.put(("error", e))
+
+        threads = [threading.Thread(target=prep_worker, daemon=True)
+                   for _ in range(self.prep_workers)]
+        threads.append(threading.Thread(target=device_worker, daemon=True))
         for th in threads:
             th.start()
         nxt = 0          # next request not yet handed to the workers
-        waiting = []     # prepared, not admitted (memory)
+        received = 0     # requests back from the preparation workers
+        waiting = []     # prepared, not admitted
         active = []      # resident and stepping
-        used = 0.0
+        in_flight = 0    # admitted items at the device worker (attach or finish)
+        used = 0.0       # device_bytes of every admitted, unfinished request
         stepper = None
         finished = 0
         try:
@@ -222,21 +303,47 @@ class Server:
                         break
                     if "error" in item:
                         raise item["error"]
+                    received += 1
                     waiting.append(item)
-                # admission against the budget, in arrival order
+                while True:
+                    try:
+                        kind, it = dev_out.get_nowait()
+                    except queue.Empty:
+                        break
+                    if kind == "error":
+                        raise it
+                    in_flight -= 1
+                    if kind == "attach":
+                        active.append(it)
+                    else:
+                        it["req"].finished = self.now()
+                        used -= it["mem"]
+                        finished += 1
+                # admission against the budget, in arrival order; a request
+                # that does not fit waits, later smaller ones may pass it
                 waiting.sort(key=lambda it: it["req"].arrival)
                 keep = []
                 for it in waiting:
-                    if used + it["mem"] <= self.budget or not active:
+                    if it["mem"] > self.budget:
+                        it["req"].result = {"error": f"needs {it['mem']} device bytes, more "
+                                                     f"than the {self.budget:.0f} byte budget"}
+                        it["req"].finished = self.now()
+                        finished += 1
+                    elif used + it["mem"] <= self.budget:
                         it["req"].admitted = self.now()
                         used += it["mem"]
-                        active.append(it)
+                        self.peak_admitted = max(self.peak_admitted, used)
+                        dev_in.put(("attach", it))
+                        in_flight += 1
                     else:
                         keep.append(it)
                 waiting = keep
                 if not active:
-                    # idle: wait for the next arrival or preparation
-                    if nxt < len(reqs) and self._clock is None:
+                    # idle: wait for the next arrival, preparation or attach
+                    if (nxt < len(reqs) and received == nxt and not waiting and in_flight == 0
+                            and hasattr(self._clock, "wait_until")):
+                        self._clock.wait_until(reqs[nxt].arrival)
+                    elif nxt < len(reqs) and self._clock is None:
                         time.sleep(max(0.0005, min(0.01, reqs[nxt].arrival - self.now())))
                     else:
                         time.sleep(0.002)
@@ -252,24 +359,15 @@ class Server:
                 for it in active:
                     it["done"] += n
                     it["req"].steps = it["done"]
+                real = stepper.device_bytes() + sum(it["solver"].ctx.device_bytes()[0]
+                                                    for it in active)
                 self.log.append((self.now(), len(active), n, used, ms,
-                                 sum(it["solver"].N for it in active)))
+                                 sum(it["solver"].N for it in active), real))
                 still = []
                 for it in active:
                     if it["done"] >= it["run"]["Nt"]:
-                        s = it["solver"]
-                        tf = time.monotonic()
-                        s._wake_device_end(it["run"])
-                        s._wake_finish(it["run"], self.compute_plane, save_logs=False)
-                        w = it["wake"]
-                        it["req"].result = {"WP": np.asarray(w.WP).copy(), "Z": np.asarray(w.Z).copy(),
-                                            "f": np.asarray(w.f).copy(), "s": np.asarray(w.s).copy(),
-                                            "Nt": it["run"]["Nt"], "N": s.N}
-                        it["req"].finished = self.now()
-                        it["req"].finish_s = time.monotonic() - tf
-                        s.ctx.close()
-                        used -= it["mem"]
-                        finished += 1
+                        dev_in.put(("finish", it))
+                        in_flight += 1
                     else:
                         still.append(it)
                 active = still

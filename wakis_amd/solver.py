@@ -224,10 +224,24 @@ class SolverFIT3D(RoutinesMixin):
 
         self.geometry = metric_tables(grid.x, grid.y, grid.z, self.bc_low, self.bc_high)
 
-        # device context (device=None: host-side setup only, no stepping)
+        # device context (device=None: host-side setup only; attach_device
+        # allocates it later)
         self.ctx = None
         if device is not None:
-            self.ctx = _lib.Context(self.Nx, self.Ny, self.Nz, dtype=dtype, device=device)
+            self.attach_device(device, kernel=kernel, tile=tile)
+        self.solverInitializationTime = time.time() - t0
+        self.update_logger(["solverInitializationTime"])
+
+    def attach_device(self, device=0, kernel="auto", tile=None):
+        """Create the device context of a solver built with device=None:
+        allocate E, H, J and the material ids, upload the metric and
+        material tables, boundaries and dt.  Everything before this point
+        (grid, materials, PML, dt) is host work, so a caller can build the
+        solver first and decide later whether its device memory fits."""
+        if self.ctx is not None:
+            raise RuntimeError("the solver already has a device context")
+        self.ctx = _lib.Context(self.Nx, self.Ny, self.Nz, dtype=self.dtype, device=device)
+        try:
             self.ctx.set_geometry(self.geometry)
             self._upload_materials()
             self.ctx.set_bc(self.pec_bits, self.pmc_bits)
@@ -235,11 +249,13 @@ class SolverFIT3D(RoutinesMixin):
             self.set_kernel(kernel)
             if tile is not None:
                 self.ctx.set_tile(int(tile))
-            self.E = DeviceField(self.ctx, _lib.FIELD_E, self)
-            self.H = DeviceField(self.ctx, _lib.FIELD_H, self)
-            self.J = DeviceField(self.ctx, _lib.FIELD_J, self)
-        self.solverInitializationTime = time.time() - t0
-        self.update_logger(["solverInitializationTime"])
+        except BaseException:
+            self.ctx.close()
+            self.ctx = None
+            raise
+        self.E = DeviceField(self.ctx, _lib.FIELD_E, self)
+        self.H = DeviceField(self.ctx, _lib.FIELD_H, self)
+        self.J = DeviceField(self.ctx, _lib.FIELD_J, self)
 
     def update_logger(self, attrs):
         """Copy solver attributes into ``self.logger.solver``; "grid" links
