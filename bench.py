@@ -497,7 +497,12 @@ def bench_serve(args):
     from wakis_amd.serve import Server, fifo_latencies, make_requests, solve_alone, z_rel_error
 
     reqs = make_requests(args.nreq, seed=4, rate=2.0)
-    srv = Server(dtype=np.float32, budget=288e9, chunk_steps=args.chunk, graph=True, group=16)
+    def done_line(r):
+        print(f"[serve] req {r.rid:3d} N {r.cells:9d} Nt {r.steps:6d} done at {r.finished:7.2f}s "
+              f"(arrived {r.arrival:6.2f}s)", file=sys.stderr, flush=True)
+
+    srv = Server(dtype=np.float32, budget=288e9, chunk_steps=args.chunk, graph=True, group=16,
+                 on_done=done_line)
     t0 = time.perf_counter()
     done = srv.run(reqs)
     wall = time.perf_counter() - t0
@@ -507,7 +512,7 @@ def bench_serve(args):
     print(f"[bench] served {len(done)} requests in {wall:.1f}s: p50 {np.percentile(lat, 50):.2f}s "
           f"p99 {np.percentile(lat, 99):.2f}s", file=sys.stderr, flush=True)
     busy_ms = sum(e[4] for e in srv.log)
-    cell_steps = sum(e[5] * e[2] for e in srv.log)
+    cell_steps = sum(e[7] for e in srv.log)
     print(f"[bench] device stepping {busy_ms / 1e3:.1f}s of {wall:.1f}s wall, "
           f"{cell_steps / (busy_ms * 1e-3) / 1e9:.1f} Gcell-steps/s while stepping, "
           f"{len(srv.log)} chunks", file=sys.stderr, flush=True)

@@ -66,3 +66,40 @@ def test_serve_results_equal_alone(budget):
         assert r.result["Nt"] == alone["Nt"]
         assert np.array_equal(r.result["WP"], alone["WP"]), r.rid
         assert np.array_equal(r.result["Z"], alone["Z"]), r.rid
+
+
+def test_serve_two_step_members_equal_alone():
+    """Requests wide enough for the two-step sweep (nx >= 64) are stepped by
+    their own fit_step calls beside the batch; each is bit for bit the same
+    as solved alone, and the context bytes stay within the model."""
+    from wakis_amd import _lib
+    from wakis_amd.serve import Request, Server, serve_cavity, solve_alone
+
+    print("[case] serve two-step members", flush=True)
+    specs = [("rect", 0.050, 0.060, 0.015, 8e5, None, 0.0),
+             ("cyl", 0.050, 0.070, 0.020, 9e5, (3.0, 1e-3), 0.05)]
+    reqs = []
+    for i, (shape, r, L, p, cells, slab, t) in enumerate(specs):
+        wl = serve_cavity(shape, r, L, p, cells, slab=slab, n_pml=4, wakelength=0.05,
+                          name=f"w{i}")
+        n = (len(wl["grid"]["x"]) - 1) * (len(wl["grid"]["y"]) - 1) * (len(wl["grid"]["z"]) - 1)
+        reqs.append(Request(rid=i, arrival=t, workload=wl, cells=n))
+    for r in _requests()[:2]:
+        r.rid += 10
+        r.arrival += 0.02
+        reqs.append(r)
+    tile = 2
+    clock = VClock()
+    srv = Server(dtype=np.float32, budget=288e9, chunk_steps=40, graph=True, group=8, tile=tile,
+                 clock=clock)
+    clock.server = srv
+    done = srv.run(reqs)
+    assert max(e[8] for e in srv.log) >= 1 and max(e[1] for e in srv.log) >= 2
+    for r in done:
+        assert 0 < r.result["device_peak"] <= r.result["device_model"], r.rid
+        t, alone = solve_alone(r, dtype=np.float32, tile=tile)
+        assert r.result["Nt"] == alone["Nt"]
+        assert np.array_equal(r.result["WP"], alone["WP"]), r.rid
+        assert np.array_equal(r.result["Z"], alone["Z"]), r.rid
+        print(r.rid, r.cells, r.steps, "peak", r.result["device_peak"], "model",
+              r.result["device_model"], flush=True)

@@ -67,6 +67,19 @@ class FakeContext:
     def second_buffers(self):
         self._hold("alt", 6 * self.N * self.esz)
 
+    def kernel_in_use(self):
+        # the two-step sweep applies from nx >= 64, ny >= 16 in FP32
+        two = self.esz == 4 and self.nx >= 64 and self.ny >= 16
+        return _lib.KERNEL_FUSED2 if two else _lib.KERNEL_FUSED
+
+    def step(self, n, conductive=False):
+        assert not self.closed
+        self.second_buffers()
+        self._hold("btab", max(int(n), self.held.get("btab", 0) // (self.nz * self.esz))
+                   * self.nz * self.esz)
+        self.steps = getattr(self, "steps", 0) + int(n)
+        self.calls = getattr(self, "calls", []) + [int(n)]
+
     def wake_potential(self, window_ids, carry, cols, k0, dz, qnorm):
         nzw, ns = np.asarray(cols).shape
         nwin = len(window_ids)
@@ -162,7 +175,7 @@ def _model(r, esz=4):
     _, _, s = build(r.workload, dtype=np.float32, device=None, precision="mixed")
     run = s._wake_plan(None, None, "longitudinal")
     return serve.device_bytes(s.Nx, s.Ny, s.Nz, run["Nt"], esz, nmat=len(s._mt.table()),
-                              half=run["h"], group=8)
+                              half=run["h"], beam_rows=50)
 
 
 @pytest.mark.timeout(300)
@@ -228,7 +241,7 @@ def test_model_terms():
     # the model is the library's allocation list, term by term
     nx, ny, nz, nt, esz = 20, 22, 60, 300, 4
     n = nx * ny * nz
-    b = serve.device_bytes(nx, ny, nz, nt, esz, nmat=3, half=0, group=8, samples=11)
+    b = serve.device_bytes(nx, ny, nz, nt, esz, nmat=3, half=0, beam_rows=8, samples=11)
     fields = 15 * n * esz + n
     metric = (2 * (nx + ny + nz) + 2 * (ny * nz + nx * nz + nx * ny)) * esz
     tables = 27 * esz + nz * (esz + 8)
@@ -237,5 +250,47 @@ def test_model_terms():
     finish = 4 + 4 * nz * ns + 8 * nz + 8 * ns + 16 * ns + 48 * 11
     assert b == fields + metric + tables + hist + 8 * nz * esz + 4096 + finish
     # a 3 x 3 probe window holds nine histories
-    b9 = serve.device_bytes(nx, ny, nz, nt, esz, nmat=3, half=1, group=8, samples=11)
+    b9 = serve.device_bytes(nx, ny, nz, nt, esz, nmat=3, half=1, beam_rows=8, samples=11)
     assert b9 - b == 8 * nz * nt * esz + 32 + 64 * ns
+
+
+@pytest.mark.timeout(300)
+def test_two_step_members_step_alone_in_even_calls(fake_device):
+    # requests wide enough for the two-step sweep are stepped by their own
+    # fit_step calls of min(chunk, steps left): every call but the last is
+    # even, so pairs of steps start at even indices as in a solve alone
+    reqs = serve.make_requests(3, seed=7, median_cells=8e5, sigma_ln=0.0, clip=(8e5, 8e5),
+                               slab_prob=0.0, wakelength=0.05)
+    small = _burst(2)
+    for r in small:
+        r.rid += 10
+    reqs += small
+    for r in reqs:
+        r.workload["solver"]["n_pml"] = 4
+        r.arrival = 0.0
+    clock = VClock()
+    srv = serve.Server(dtype=np.float32, budget=288e9, chunk_steps=50, group=8, clock=clock)
+    clock.server = srv
+    made = []
+    orig = FakeContext.__init__
+
+    def track(self, *a, **k):
+        orig(self, *a, **k)
+        made.append(self)
+
+    FakeContext.__init__ = track
+    try:
+        done = srv.run(reqs)
+    finally:
+        FakeContext.__init__ = orig
+    assert all("error" not in r.result for r in done)
+    solo = [c for c in made if c.kernel_in_use() == _lib.KERNEL_FUSED2]
+    assert len(solo) == 3 and len(made) == 5
+    for c in solo:
+        assert all(n == 50 for n in c.calls[:-1]) and 0 < c.calls[-1] <= 50
+    for c in made:
+        if c not in solo:
+            assert not hasattr(c, "calls")  # batched members are stepped by the batch
+    assert max(e[8] for e in srv.log) >= 1
+    assert sum(e[7] for e in srv.log) == sum(r.cells * r.steps for r in done)
+    assert fake_device.peak <= srv.peak_admitted

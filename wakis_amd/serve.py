@@ -34,6 +34,7 @@ from typing import Callable
 
 import numpy as np
 
+from . import _lib
 from .batch import BatchStepper
 from .workloads import build, cells_from_nodes
 
@@ -136,7 +137,7 @@ def make_requests(n, seed=4, rate=2.0, median_cells=8e6, sigma_ln=1.0,
     return out
 
 
-def device_bytes(nx, ny, nz, nt, esz, nmat=256, half=0, group=16, samples=1001):
+def device_bytes(nx, ny, nz, nt, esz, nmat=256, half=0, beam_rows=256, samples=1001):
     """Most device bytes one served solve holds at once, term by term as the
     library allocates them (fit_device_bytes reports the real figure):
 
@@ -146,8 +147,10 @@ def device_bytes(nx, ny, nz, nt, esz, nmat=256, half=0, group=16, samples=1001):
     materials   nmat rows of dt/mu, dt/eps, sigma: 9 nmat esz
     lines       the beam line and cell centres: nz (esz + 8)
     history     the E_z probe: (2 half + 1)^2 nz nt esz
-    batch       this member's rows of the batch's beam table (group nz esz) and
-                its entries in the member tables (4 KiB)
+    beam        the beam table: beam_rows x nz values (a solve stepped alone holds
+                one row per step of a fit_step call, a batched one its rows of
+                the batch's table, group per member), and the member's entries
+                in the batch's tables (4 KiB)
     finish      the wake potential's scratch (window ids, nz x ns time columns,
                 dz, the WP rows) and the impedance DFT's (2 ns + 6 K doubles),
                 with ns <= nt + 2 wake samples and K <= samples DFT bins; both
@@ -159,9 +162,9 @@ def device_bytes(nx, ny, nz, nt, esz, nmat=256, half=0, group=16, samples=1001):
     metric = (2 * (nx + ny + nz) + 2 * (ny * nz + nx * nz + nx * ny)) * esz
     tables = 9 * nmat * esz + nz * (esz + 8)
     history = nwin * nz * max(nt, 1) * esz
-    batch = group * nz * esz + 4096
+    beam = beam_rows * nz * esz + 4096
     finish = (4 * nwin + 4 * nz * ns + 8 * nz + 8 * nwin * ns) + (16 * ns + 48 * samples)
-    return int(fields + metric + tables + history + batch + finish)
+    return int(fields + metric + tables + history + beam + finish)
 
 
 class Server:
@@ -178,11 +181,24 @@ class Server:
     (wall time by default); a caller's clock that advances only with the
     server's own work (a virtual clock) should have wait_until(t), which the
     server calls when it is idle until the next arrival at t.
-    device: the device the solves run on."""
+    device: the device the solves run on.
+
+    two_step: a solve on which fit_step runs the two-step sweep (FP32, nx >=
+    64, ny >= 16) is stepped by its own fit_step calls on its own stream,
+    min(chunk_steps, steps left) per round, so its pairs of steps fall as
+    they do when it is solved alone (chunk_steps must be even) and its
+    launches overlap on the device with the other solves' and with the
+    batched launch of the remaining solves.  on_done(request) is called in the
+    main loop as each request completes (or is rejected)."""
 
     def __init__(self, dtype=np.float32, budget=288e9, chunk_steps=256, graph=True, group=16,
                  tile=None, compute_plane="longitudinal", clock: Callable[[], float] | None = None,
-                 prep_workers=2, precision="mixed", device=0):
+                 prep_workers=2, precision="mixed", device=0, two_step=True,
+                 on_done: Callable[[Request], None] | None = None):
+        if int(chunk_steps) < 2 or int(chunk_steps) % 2:
+            raise ValueError("chunk_steps must be even (pairs of steps start at even indices)")
+        self.two_step = bool(two_step)
+        self.on_done = on_done
         self.dtype = np.dtype(dtype)
         self.precision = precision
         self.budget = float(budget)
@@ -193,8 +209,9 @@ class Server:
         self._t0 = None
         self._clock = clock
         self.prep_workers = prep_workers
-        # per chunk: (time, members, steps, admitted bytes, ms, cells,
-        #             device bytes of the stepped members and the batch)
+        # per round: (time, members, steps (most of any member), admitted
+        #             bytes, ms, cells, device bytes of the stepped members and
+        #             the batch, cell-steps, members stepped alone)
         self.log = []
         self.peak_admitted = 0.0
 
@@ -210,7 +227,8 @@ class Server:
                                    precision=self.precision)
         run = solver._wake_plan(None, None, self.compute_plane)
         mem = device_bytes(solver.Nx, solver.Ny, solver.Nz, run["Nt"], self.dtype.itemsize,
-                           nmat=len(solver._mt.table()), half=run["h"], group=self.group)
+                           nmat=len(solver._mt.table()), half=run["h"],
+                           beam_rows=max(self.group, self.chunk_steps))
         req.prep_s = time.monotonic() - t0
         return {"req": req, "solver": solver, "wake": wake, "run": run, "done": 0, "mem": mem}
 
@@ -223,6 +241,7 @@ class Server:
         s._wake_attach(run)
         s._wake_device_beam(run)
         s.ctx.sync()
+        it["solo"] = self.two_step and s.ctx.kernel_in_use() == _lib.KERNEL_FUSED2
         it["attach_s"] = time.monotonic() - t0
         return it
 
@@ -319,6 +338,8 @@ class Server:
                         it["req"].finished = self.now()
                         used -= it["mem"]
                         finished += 1
+                        if self.on_done is not None:
+                            self.on_done(it["req"])
                 # admission against the budget, in arrival order; a request
                 # that does not fit waits, later smaller ones may pass it
                 waiting.sort(key=lambda it: it["req"].arrival)
@@ -329,6 +350,8 @@ class Server:
                                                      f"than the {self.budget:.0f} byte budget"}
                         it["req"].finished = self.now()
                         finished += 1
+                        if self.on_done is not None:
+                            self.on_done(it["req"])
                     elif used + it["mem"] <= self.budget:
                         it["req"].admitted = self.now()
                         used += it["mem"]
@@ -348,21 +371,37 @@ class Server:
                     else:
                         time.sleep(0.002)
                     continue
-                n = min(self.chunk_steps, min(it["run"]["Nt"] - it["done"] for it in active))
-                members = [it["solver"] for it in active]
-                if stepper is None:
-                    stepper = BatchStepper(members, graph=self.graph, group=self.group,
-                                           tile=self.tile)
-                else:
-                    stepper.set_members(members)
-                ms = stepper.step(n) if n > 0 else 0.0
+                solo = [it for it in active if it["solo"]]
+                batched = [it for it in active if not it["solo"]]
+                t_round = time.perf_counter()
+                for it in solo:  # asynchronous, each on its solver's stream
+                    it["n"] = min(self.chunk_steps, it["run"]["Nt"] - it["done"])
+                    it["solver"]._advance(it["n"])
+                ms = 0.0
+                if batched:
+                    n = min(self.chunk_steps, min(it["run"]["Nt"] - it["done"] for it in batched))
+                    members = [it["solver"] for it in batched]
+                    if stepper is None:
+                        stepper = BatchStepper(members, graph=self.graph, group=self.group,
+                                               tile=self.tile)
+                    else:
+                        stepper.set_members(members)
+                    ms = stepper.step(n) if n > 0 else 0.0
+                    for it in batched:
+                        it["n"] = n
+                for it in solo:
+                    it["solver"].ctx.sync()
+                if solo:
+                    ms = (time.perf_counter() - t_round) * 1e3
                 for it in active:
-                    it["done"] += n
+                    it["done"] += it["n"]
                     it["req"].steps = it["done"]
-                real = stepper.device_bytes() + sum(it["solver"].ctx.device_bytes()[0]
-                                                    for it in active)
-                self.log.append((self.now(), len(active), n, used, ms,
-                                 sum(it["solver"].N for it in active), real))
+                real = sum(it["solver"].ctx.device_bytes()[0] for it in active)
+                if stepper is not None:
+                    real += stepper.device_bytes()
+                self.log.append((self.now(), len(active), max(it["n"] for it in active), used, ms,
+                                 sum(it["solver"].N for it in active), real,
+                                 sum(it["solver"].N * it["n"] for it in active), len(solo)))
                 still = []
                 for it in active:
                     if it["done"] >= it["run"]["Nt"]:
