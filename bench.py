@@ -429,12 +429,17 @@ def bench_batch(args):
     for s, r in zip(solvers, runs):
         s._wake_device_beam(r)
     cells = sum(s.N for s in solvers)
-    st = wa.BatchStepper(solvers, graph=True, group=16, tile=args.tile)
-    tune = st.tune(reps=3) if not args.no_tune and args.tile is None else None
+    st = wa.BatchStepper(solvers, graph=True, group=16, tile=args.tile,
+                         two_step=not args.one_step, kc2=args.kc2)
+    pairs = st.two_step()[0]
+    # the tuner picks the one-step launch's tile (the two-step batch has one)
+    tune = (st.tune(reps=3) if not args.no_tune and args.tile is None and not pairs else None)
     cfg, nblk, ntiles = st.tile()
-    print(f"[bench] batch {len(solvers)} cavities, {cells} cells, setup {setup_s:.1f}s, "
-          f"tile {cfg} blocks {nblk} tiles {ntiles} tune {tune}", file=sys.stderr, flush=True)
     st.step(args.warmup)
+    kc2 = st.two_step()[1]
+    print(f"[bench] batch {len(solvers)} cavities, {cells} cells, setup {setup_s:.1f}s, "
+          f"two-step {pairs} (kc {kc2}), one-step tile {cfg} blocks {nblk} tiles {ntiles} "
+          f"tune {tune}", file=sys.stderr, flush=True)
     ms = st.step(args.steps)
     value = cells * args.steps / (ms * 1e-3) / 1e9
     # the same steps one context at a time (the plain per-solver path)
@@ -452,14 +457,26 @@ def bench_batch(args):
     seq_s = time.perf_counter() - t0
     seq_value = cells * seq_steps / seq_s / 1e9
     ms_step = ms / args.steps
-    alg = 12 * 4 * cells
-    dims = {0: (320, 3, 64), 1: (128, 4, 32), 2: (64, 8, 32), 3: (256, 4, 32), 4: (64, 4, 32),
-            5: (256, 4, 64)}.get(cfg)
     traffic, tsrc = (None, None)
-    if dims and len(solvers) == 64:
-        traffic, tsrc = traffic_from_profiles(f"fused_batch<float, {dims[0]}, {dims[1]}, {dims[2]}>",
-                                              "f32", "batch64", cfg)
-    achieved = alg / (ms_step * 1e-3) / 1e9
+    if pairs:
+        # one two-step launch pair (lean chunks, then face chunks of every
+        # member) per two steps: 12 field words per cell read and written
+        kname = "fused2_batch"
+        alg = 2 * 12 * 4 * cells
+        launch_ms = 2 * ms_step
+        if len(solvers) == 64:
+            traffic, tsrc = traffic_from_profiles("fused2_batch<float, 32, 16>", "f32", "batch64",
+                                                  f"f2b-kc{kc2}")
+    else:
+        kname = "fused_batch"
+        alg = 12 * 4 * cells
+        launch_ms = ms_step
+        dims = {0: (320, 3, 64), 1: (128, 4, 32), 2: (64, 8, 32), 3: (256, 4, 32), 4: (64, 4, 32),
+                5: (256, 4, 64)}.get(cfg)
+        if dims and len(solvers) == 64:
+            traffic, tsrc = traffic_from_profiles(
+                f"fused_batch<float, {dims[0]}, {dims[1]}, {dims[2]}>", "f32", "batch64", cfg)
+    achieved = alg / (launch_ms * 1e-3) / 1e9
     sizes = sorted(s.N for s in solvers)
     return {
         "metric": "Gcell-steps/s",
@@ -476,16 +493,20 @@ def bench_batch(args):
         "data": "synthetic",
         "config": {
             "workload": f"batch of {len(solvers)} rectangular PEC cavities (seed 3), "
-                        f"{sizes[0]}..{sizes[-1]} cells each, {cells} cells in all; one "
-                        "launch per step in hipGraph groups of 16; wakesolve inner loop",
-            "tile": cfg, "blocks": nblk, "tiles": ntiles,
+                        f"{sizes[0]}..{sizes[-1]} cells each, {cells} cells in all; "
+                        + ("two-step sweeps: one launch pair per two steps"
+                           if pairs else "one launch per step")
+                        + " in hipGraph groups of 16 steps; wakesolve inner loop",
+            "two_step": pairs, "kc": kc2 if pairs else None,
+            "tile": None if pairs else cfg, "blocks": None if pairs else nblk,
+            "tiles": None if pairs else ntiles,
             "one_at_a_time_gcells": round(seq_value, 3),
             "speedup_vs_one_at_a_time": round(value / seq_value, 3),
         },
-        "roofline": {"bound": "hbm", "kernel": "fused_batch", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_source": f"profiles/{tsrc}" if tsrc else None,
-                     "alg_bytes_per_launch": alg, "avg_launch_ms": round(ms_step, 5)},
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": round(launch_ms, 5)},
     }
 
 
@@ -811,6 +832,10 @@ def main():
     ap.add_argument("--ncav", type=int, default=64, help="batch: number of cavities")
     ap.add_argument("--nreq", type=int, default=24, help="serve: number of requests")
     ap.add_argument("--chunk", type=int, default=256, help="serve: steps per batch call")
+    ap.add_argument("--one-step", action="store_true",
+                    help="batch: one-step launches only (no two-step pairs)")
+    ap.add_argument("--kc2", type=int, default=0,
+                    help="batch: planes per z chunk of the two-step sweep (0: modelled)")
     ap.add_argument("--n-large", type=int, default=1024, help="large: cells per side")
     ap.add_argument("--no-plain", action="store_true",
                     help="serve: skip the plain (sparse-operator) FP64 GPU baseline")

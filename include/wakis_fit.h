@@ -291,7 +291,14 @@ int fit_impedance_noalloc(fit_ctx *ctx, const double *x, const double *lam,
  * batch steps many independent contexts (same device and dtype, fused
  * sweep) with ONE launch per step (a block per tile of every member); each member's arithmetic is
  * that of its own fit_step with the same tile configuration, so results are
- * the same alone and in a batch.  Members keep their own state (fields,
+ * the same alone and in a batch.  FP32 batches whose members all suit the
+ * two-step sweep (nx >= 32, ny >= 16, J zero or sigma*E) step pairs of steps
+ * with it instead (fit_batch_set_two_step): one launch per pair over the
+ * lean chunks of all members, then their face chunks, E and H read and
+ * written once per two steps; an odd last step of a call takes the one-step
+ * launch.
+ * Such a member's results are those of the two-step sweep, which agree with
+ * the one-step sweep to FP32 rounding.  Members keep their own state (fields,
  * beam, wake probe); after fit_batch_step every member is as if it had run
  * fit_step(nsteps, conductive[i]).  Members may be replaced between calls
  * (continuous batching).  Members with host beam values (FIT_BEAM_VALUES)
@@ -319,6 +326,13 @@ int fit_batch_tune(fit_batch *batch, int32_t reps, double *ms);
  * advance); with on != 0 a group is captured once into a HIP graph and
  * replayed. */
 int fit_batch_set_graph(fit_batch *batch, int32_t on, int32_t group);
+/* Two-step pairs on (default) or off for this batch, and their planes per
+ * z chunk: kc = 0 picks it by the launch model (the least rounds x (kc + 3)
+ * over all members' blocks), 8 .. 4096 sets it. */
+int fit_batch_set_two_step(fit_batch *batch, int32_t on, int32_t kc);
+/* Whether the next fit_batch_step steps pairs with the two-step sweep (*on)
+ * and the planes per chunk of the last two-step table (*kc, 0 before). */
+int fit_batch_two_step(fit_batch *batch, int32_t *on, int32_t *kc);
 
 #ifdef __cplusplus
 }

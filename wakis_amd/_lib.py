@@ -94,6 +94,8 @@ SIGNATURES = {
     "fit_batch_destroy": [_P],
     "fit_batch_step": [_P, _I64, _PD],
     "fit_batch_device_bytes": [_P, C.POINTER(_I64)],
+    "fit_batch_set_two_step": [_P, _I32, _I32],
+    "fit_batch_two_step": [_P, _PI32, _PI32],
     "fit_batch_set_tile": [_P, _I32],
     "fit_batch_get_tile": [_P, _PI32, _PI32, _PI32],
     "fit_batch_tune": [_P, _I32, _PD],
@@ -443,6 +445,16 @@ class Batch:
         out = np.zeros(1, dtype=np.float64)
         check(self.lib.fit_batch_step(self.h, int(nsteps), dptr(out)), "fit_batch_step")
         return float(out[0])
+
+    def set_two_step(self, on: bool, kc: int = 0):
+        check(self.lib.fit_batch_set_two_step(self.h, int(bool(on)), int(kc)),
+              "fit_batch_set_two_step")
+
+    def two_step(self):
+        """(pairs by the two-step sweep on the next step call, planes per chunk)."""
+        on, kc = C.c_int32(0), C.c_int32(0)
+        check(self.lib.fit_batch_two_step(self.h, C.byref(on), C.byref(kc)), "fit_batch_two_step")
+        return bool(on.value), int(kc.value)
 
     def device_bytes(self) -> int:
         """Device bytes of the batch's own tables."""

@@ -22,13 +22,20 @@ class BatchStepper:
     """Steps a set of device solvers together; members may change between
     calls (``set_members``)."""
 
-    def __init__(self, solvers, graph: bool = True, group: int = 16, tile=None):
+    def __init__(self, solvers, graph: bool = True, group: int = 16, tile=None,
+                 two_step: bool = True, kc2: int = 0):
+        """two_step: FP32 batches whose members all suit the two-step sweep
+        (nx >= 32, ny >= 16) advance pairs of steps with it (one read and
+        write of E and H per two steps); False keeps the one-step launch.
+        kc2: planes per z chunk of the two-step sweep (0: by its launch
+        model)."""
         self.solvers = list(solvers)
         for s in self.solvers:
             s._require_device()
         self._b = _lib.Batch([s.ctx for s in self.solvers],
                              [int(bool(s.use_conductivity)) for s in self.solvers])
         self._b.set_graph(graph, group)
+        self._b.set_two_step(two_step, kc2)
         if tile is not None:
             self._b.set_tile(int(tile))
 
@@ -58,6 +65,10 @@ class BatchStepper:
     def tile(self):
         return self._b.get_tile()
 
+    def two_step(self):
+        """(pairs by the two-step sweep on the next step call, planes per chunk)."""
+        return self._b.two_step()
+
     def device_bytes(self) -> int:
         """Device bytes of the batch's own tables (the members' are theirs)."""
         return self._b.device_bytes()
@@ -67,13 +78,13 @@ class BatchStepper:
 
 
 def wakesolve_batch(solvers, wakelength=None, compute_plane="longitudinal", graph=True,
-                    group=16, tile=None, tune=False, max_chunk=None):
+                    group=16, tile=None, tune=False, max_chunk=None, two_step=True):
     """``wakesolve`` of every solver in one batch: the same setup, device
     beam, E_z probe and wake post-processing as ``SolverFIT3D.wakesolve``,
     with the time steps of all members taken by batched launches.  A member
     leaves the batch when it has taken its own Nt steps, so each solver ends
-    in exactly the state its own wakesolve would leave.  Returns the
-    seconds spent stepping."""
+    in exactly the state its own wakesolve would leave (with two_step, to
+    FP32 rounding: see BatchStepper).  Returns the seconds spent stepping."""
     solvers = list(solvers)
     runs = [s._wake_setup(wakelength, None, compute_plane) for s in solvers]
     for s, r in zip(solvers, runs):
@@ -81,7 +92,7 @@ def wakesolve_batch(solvers, wakelength=None, compute_plane="longitudinal", grap
     done = [0] * len(solvers)
     active = [i for i, r in enumerate(runs) if r["Nt"] > 0]
     stepper = BatchStepper([solvers[i] for i in active] or solvers[:1], graph=graph,
-                           group=group, tile=tile)
+                           group=group, tile=tile, two_step=two_step)
     if tune and active:
         stepper.tune()
     t0 = time.time()

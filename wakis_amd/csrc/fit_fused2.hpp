@@ -766,6 +766,133 @@ inline int pick_kc2(int ncol, int nz, int slots) {
   return best;
 }
 
+// ---- batched two-step sweep --------------------------------------------
+// Every member of a batch advances two steps per launch: blocks [0, ntl)
+// take the lean chunks of all members, blocks [ntl, ntl + ntf) their face
+// chunks (or every chunk of a member too short for a lean chunk), so the
+// short face blocks fill the tail of the long lean ones.  Block t of a part
+// takes tile t - tstart[m] of the member m with tstart[m] <= t < tstart[m+1]
+// (one table per part); the member's arguments of the current buffer parity
+// come from a device table, with the step index and beam rows of pair g of
+// the launch group taken from the device step counter, so a group can be
+// captured and replayed.  The tile bodies are those of the single-grid
+// launch.
+template <typename T>
+struct Batch2Member {
+  Fused2Args<T> f[2];  // f[p]: buffers after p (mod 2) two-step launches
+  int64_t step0;       // the member's step index when the table was built
+  const T *btab;       // beam rows of the group's steps (G x nz), or null
+};
+
+// 32 x 16 columns, a (36 x 20)-cell window (720 threads): less overhang than
+// 64-wide tiles on the narrow grids of a batch, and fewer window cells per
+// tile cell (1.41 against 1.59).
+constexpr int F2B_TX = 32, F2B_TY = 16;
+
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__((TX + 4) * (TY + 4))
+    fused2_batch(const Batch2Member<T> *__restrict__ mem, const int *__restrict__ tl,
+                 const int *__restrict__ tf, int nmem, int ntl, int ntf,
+                 const int64_t *__restrict__ dstep, int g) {
+  __shared__ Smem2<T, TX, TY> sm;
+  T *smt = (T *)fit_dyn_lds;
+  const bool lean = (int)blockIdx.x < ntl;
+  // each XCD takes a contiguous run of the part's tiles (xcd_tile over the
+  // whole part), so neighbouring tiles of a member share that XCD's L2
+  const int t = lean ? xcd_tile((int)blockIdx.x, ntl) : xcd_tile((int)blockIdx.x - ntl, ntf);
+  const int *tstart = lean ? tl : tf;
+  if (t < (lean ? ntl : ntf)) {
+    const int64_t s = *dstep + 2 * g;  // first step of the pair (even)
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tstart[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const Batch2Member<T> &m = mem[lo];
+    Fused2Args<T> f = m.f[(s >> 1) & 1];
+    f.a.step = m.step0 + s;
+    if (m.btab) {
+      f.beam1 = m.btab + (int64_t)2 * g * f.a.nz;
+      f.beam2 = f.beam1 + f.a.nz;
+    }
+    const int b = t - tstart[lo];
+    if (lean) fused2_tile_lean2b<T, TX, TY, false>(f, b, sm, smt);
+    else fused2_tile<T, TX, TY, false, false>(f, f.nremap + b, sm, smt);
+  }
+}
+
+// Beam rows of a launch group of G steps: member m, step q (0 .. G-1), plane k.
+template <typename T>
+__global__ void batch2_beam_table(const Batch2Member<T> *__restrict__ mem, int nmem,
+                                  const int64_t *__restrict__ dstep, int G, T *const *btabs) {
+  const int m = blockIdx.y;
+  if (m >= nmem || !btabs[m]) return;
+  const StepArgs<T> &a0 = mem[m].f[0].a;
+  const int64_t cnt = (int64_t)G * a0.nz;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int q = (int)(t / a0.nz), k = (int)(t % a0.nz);
+    StepArgs<T> a = a0;
+    a.step = mem[m].step0 + *dstep + q;
+    btabs[m][t] = beam_gauss(a, k);
+  }
+}
+
+// Planes per chunk of a batch: the least modelled time rounds x (kc + 8)
+// over the blocks of all members (a block's set-up and pipeline fill cost
+// about 8 planes: on 64 cavities of 120-320 planes, chunks of 64 and 96
+// planes ran 3 % faster than chunks of 32 and 48, and 128 (which leaves the
+// shorter members without a lean chunk) 13 % slower).
+inline int pick_kc2_batch(const int *ncol, const int *nz, int n, int slots) {
+  static const int cand[] = {32, 48, 64, 96, 128, 192, 256, 384};
+  int nzmax = 0;
+  for (int m = 0; m < n; ++m) nzmax = nz[m] > nzmax ? nz[m] : nzmax;
+  int best = 64;
+  double bt = 1e300;
+  const int xslots = slots / 8 > 0 ? slots / 8 : 1;
+  for (int kc : cand) {
+    if (kc > nzmax && kc != cand[0]) break;
+    long long blocks = 0;
+    for (int m = 0; m < n; ++m) {
+      const int mid = nz[m] - 8;
+      blocks += (long long)ncol[m] * (mid >= kc ? (mid + kc - 1) / kc : (nz[m] + kc - 1) / kc);
+    }
+    const long long per_xcd = (blocks + 7) / 8;
+    const long long rounds = (per_xcd + xslots - 1) / xslots;
+    const double t = (double)rounds * (kc + 8);
+    if (t < bt) {
+      bt = t;
+      best = kc;
+    }
+  }
+  return best;
+}
+
+// Chunking of one member of a batch: kc planes per chunk, 4-plane face
+// chunks when a lean chunk fits (as launch_fused2 chunks a single grid).
+template <typename T, int TX, int TY>
+inline void chunk2_layout(Fused2Args<T> &f, int kc) {
+  const StepArgs<T> &a = f.a;
+  f.ntx = (a.nx + TX - 1) / TX;
+  f.nty = (a.ny + TY - 1) / TY;
+  const int ncol = f.ntx * f.nty;
+  f.kc = kc;
+  constexpr int KH = 4;
+  if (a.nz - 2 * KH >= kc) {
+    f.kh = KH;
+    f.nmid = (a.nz - 2 * KH + kc - 1) / kc;
+    f.nchunk = f.nmid + 2;
+    f.nremap = ncol * f.nmid;
+  } else {
+    f.kh = 0;
+    f.nmid = 0;
+    f.nchunk = (a.nz + kc - 1) / kc;
+    f.nremap = 0;
+  }
+  f.nb = ncol * f.nchunk;
+}
+
 // Whether the two-step sweep serves these step arguments: FP32, nx >= 64 and
 // ny >= 16 (a window cell's flat carry then stays within one row and one
 // plane), fields of less than 2^32 bytes, J zero or sigma*E (no J buffer),
@@ -776,11 +903,26 @@ inline bool fused2_supported(const StepArgs<T> &a) {
     (void)a;
     return false;
   } else {
-    return a.jmode != J_BUFFER && a.buf32 && a.nx >= F2_TX && a.ny >= 16 && a.nz >= 2 &&
+    return a.jmode != J_BUFFER && a.buf32 && a.nx >= F2_TX && a.ny >= 16 && a.ny >= F2_TY &&
+           a.nz >= 2 &&
            kernel_spill_free((const void *)fused2_sweep<T, F2_TX, F2_TY, false>) &&
            kernel_spill_free((const void *)fused2_sweep<T, F2_TX, F2_TY, true>) &&
            kernel_spill_free((const void *)fused2_sweep_faces<T, F2_TX, F2_TY, false>) &&
            kernel_spill_free((const void *)fused2_sweep_faces<T, F2_TX, F2_TY, true>);
+  }
+}
+
+// Whether a batch member's step arguments suit the batched two-step sweep
+// (the tile F2B_TX x F2B_TY; the conditions of fused2_supported otherwise).
+template <typename T>
+inline bool fused2_batch_supported(const StepArgs<T> &a) {
+  if constexpr (!cfg2_fits<T, F2B_TX, F2B_TY>()) {
+    (void)a;
+    return false;
+  } else {
+    return a.jmode != J_BUFFER && a.buf32 && a.nx >= F2B_TX && a.ny >= 16 && a.ny >= F2B_TY &&
+           a.nz >= 2 &&
+           kernel_spill_free((const void *)fused2_batch<T, F2B_TX, F2B_TY>);
   }
 }
 
