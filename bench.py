@@ -430,16 +430,18 @@ def bench_batch(args):
         s._wake_device_beam(r)
     cells = sum(s.N for s in solvers)
     st = wa.BatchStepper(solvers, graph=True, group=16, tile=args.tile,
-                         two_step=not args.one_step, kc2=args.kc2)
+                         two_step=not args.one_step, kc2=args.kc2, cfg2=args.cfg2)
     pairs = st.two_step()[0]
-    # the tuner picks the one-step launch's tile (the two-step batch has one)
-    tune = (st.tune(reps=3) if not args.no_tune and args.tile is None and not pairs else None)
+    # the tuners pick the launch's tile shape (and, two-step, the chunk depth)
+    tune = None
+    if not args.no_tune and args.tile is None:
+        tune = st.tune_two_step(reps=3) if pairs else st.tune(reps=3)
     cfg, nblk, ntiles = st.tile()
     st.step(args.warmup)
-    kc2 = st.two_step()[1]
+    _, cfg2, kc2 = st.two_step()
     print(f"[bench] batch {len(solvers)} cavities, {cells} cells, setup {setup_s:.1f}s, "
-          f"two-step {pairs} (kc {kc2}), one-step tile {cfg} blocks {nblk} tiles {ntiles} "
-          f"tune {tune}", file=sys.stderr, flush=True)
+          f"two-step {pairs} (tile cfg {cfg2}, kc {kc2}), one-step tile {cfg} blocks {nblk} "
+          f"tiles {ntiles} tune {tune}", file=sys.stderr, flush=True)
     ms = st.step(args.steps)
     value = cells * args.steps / (ms * 1e-3) / 1e9
     # the same steps one context at a time (the plain per-solver path)
@@ -465,8 +467,9 @@ def bench_batch(args):
         alg = 2 * 12 * 4 * cells
         launch_ms = 2 * ms_step
         if len(solvers) == 64:
-            traffic, tsrc = traffic_from_profiles("fused2_batch<float, 32, 16>", "f32", "batch64",
-                                                  f"f2b-kc{kc2}")
+            tx2, ty2 = ((24, 24) if cfg2 == 1 else (32, 16))
+            traffic, tsrc = traffic_from_profiles(f"fused2_batch<float, {tx2}, {ty2}>", "f32",
+                                                  "batch64", f"f2b-kc{kc2}")
     else:
         kname = "fused_batch"
         alg = 12 * 4 * cells
@@ -497,7 +500,9 @@ def bench_batch(args):
                         + ("two-step sweeps: one launch pair per two steps"
                            if pairs else "one launch per step")
                         + " in hipGraph groups of 16 steps; wakesolve inner loop",
-            "two_step": pairs, "kc": kc2 if pairs else None,
+            "two_step": pairs, "two_step_tile": ({0: "32x16", 1: "24x24"}.get(cfg2)
+                                                 if pairs else None),
+            "kc": kc2 if pairs else None,
             "tile": None if pairs else cfg, "blocks": None if pairs else nblk,
             "tiles": None if pairs else ntiles,
             "one_at_a_time_gcells": round(seq_value, 3),
@@ -836,6 +841,8 @@ def main():
                     help="batch: one-step launches only (no two-step pairs)")
     ap.add_argument("--kc2", type=int, default=0,
                     help="batch: planes per z chunk of the two-step sweep (0: modelled)")
+    ap.add_argument("--cfg2", type=int, default=-1,
+                    help="batch: two-step tile configuration (0: 32x16, 1: 24x24, -1: modelled)")
     ap.add_argument("--n-large", type=int, default=1024, help="large: cells per side")
     ap.add_argument("--no-plain", action="store_true",
                     help="serve: skip the plain (sparse-operator) FP64 GPU baseline")

@@ -295,8 +295,8 @@ int fit_impedance_noalloc(fit_ctx *ctx, const double *x, const double *lam,
  * two-step sweep (nx >= 32, ny >= 16, J zero or sigma*E) step pairs of steps
  * with it instead (fit_batch_set_two_step): one launch per pair over the
  * lean chunks of all members, then their face chunks, E and H read and
- * written once per two steps; an odd last step of a call takes the one-step
- * launch.
+ * written once per two steps, in tiles of 32 x 16 or (ny >= 24) 24 x 24
+ * columns; an odd last step of a call takes the one-step launch.
  * Such a member's results are those of the two-step sweep, which agree with
  * the one-step sweep to FP32 rounding.  Members keep their own state (fields,
  * beam, wake probe); after fit_batch_step every member is as if it had run
@@ -326,13 +326,22 @@ int fit_batch_tune(fit_batch *batch, int32_t reps, double *ms);
  * advance); with on != 0 a group is captured once into a HIP graph and
  * replayed. */
 int fit_batch_set_graph(fit_batch *batch, int32_t on, int32_t group);
-/* Two-step pairs on (default) or off for this batch, and their planes per
- * z chunk: kc = 0 picks it by the launch model (the least rounds x (kc + 3)
- * over all members' blocks), 8 .. 4096 sets it. */
-int fit_batch_set_two_step(fit_batch *batch, int32_t on, int32_t kc);
-/* Whether the next fit_batch_step steps pairs with the two-step sweep (*on)
- * and the planes per chunk of the last two-step table (*kc, 0 before). */
-int fit_batch_two_step(fit_batch *batch, int32_t *on, int32_t *kc);
+/* Two-step pairs on (default) or off for this batch, their tile
+ * configuration (FIT_BATCH2_32x16, FIT_BATCH2_24x24, or -1) and planes per z
+ * chunk (8 .. 4096, or 0).  -1 and 0 pick by the launch model: the least
+ * rounds x (kc + 8) x waves per block over all members' blocks. */
+enum { FIT_BATCH2_32x16 = 0, FIT_BATCH2_24x24 = 1, FIT_BATCH2_NTUNE = 6 };
+int fit_batch_set_two_step(fit_batch *batch, int32_t on, int32_t cfg, int32_t kc);
+/* Whether the next fit_batch_step steps pairs with the two-step sweep (*on),
+ * with which tile configuration (*cfg, -1 if not) and planes per chunk. */
+int fit_batch_two_step(fit_batch *batch, int32_t *on, int32_t *cfg, int32_t *kc);
+/* Autotuner of the two-step batch (state unchanged): times a launch for
+ * each tile configuration every member suits and chunks of 48, 64 and 96
+ * planes, keeps the fastest (*cfg, *kc; -1, 0 if none applies).
+ * ms[cfg * 3 + i] (optional, FIT_BATCH2_NTUNE values): ms per launch of
+ * chunk i, -1 = not run. */
+int fit_batch_tune_two_step(fit_batch *batch, int32_t reps, int32_t *cfg, int32_t *kc,
+                            double *ms);
 
 #ifdef __cplusplus
 }

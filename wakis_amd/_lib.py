@@ -94,8 +94,9 @@ SIGNATURES = {
     "fit_batch_destroy": [_P],
     "fit_batch_step": [_P, _I64, _PD],
     "fit_batch_device_bytes": [_P, C.POINTER(_I64)],
-    "fit_batch_set_two_step": [_P, _I32, _I32],
-    "fit_batch_two_step": [_P, _PI32, _PI32],
+    "fit_batch_set_two_step": [_P, _I32, _I32, _I32],
+    "fit_batch_two_step": [_P, _PI32, _PI32, _PI32],
+    "fit_batch_tune_two_step": [_P, _I32, _PI32, _PI32, _PD],
     "fit_batch_set_tile": [_P, _I32],
     "fit_batch_get_tile": [_P, _PI32, _PI32, _PI32],
     "fit_batch_tune": [_P, _I32, _PD],
@@ -446,15 +447,26 @@ class Batch:
         check(self.lib.fit_batch_step(self.h, int(nsteps), dptr(out)), "fit_batch_step")
         return float(out[0])
 
-    def set_two_step(self, on: bool, kc: int = 0):
-        check(self.lib.fit_batch_set_two_step(self.h, int(bool(on)), int(kc)),
+    def set_two_step(self, on: bool, cfg: int = -1, kc: int = 0):
+        check(self.lib.fit_batch_set_two_step(self.h, int(bool(on)), int(cfg), int(kc)),
               "fit_batch_set_two_step")
 
     def two_step(self):
-        """(pairs by the two-step sweep on the next step call, planes per chunk)."""
-        on, kc = C.c_int32(0), C.c_int32(0)
-        check(self.lib.fit_batch_two_step(self.h, C.byref(on), C.byref(kc)), "fit_batch_two_step")
-        return bool(on.value), int(kc.value)
+        """(pairs by the two-step sweep on the next step call, its tile
+        configuration or -1, planes per chunk)."""
+        on, cfg, kc = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        check(self.lib.fit_batch_two_step(self.h, C.byref(on), C.byref(cfg), C.byref(kc)),
+              "fit_batch_two_step")
+        return bool(on.value), int(cfg.value), int(kc.value)
+
+    def tune_two_step(self, reps: int = 3):
+        """(tile configuration, planes per chunk, {(cfg, kc): ms per launch})."""
+        cfg, kc = C.c_int32(0), C.c_int32(0)
+        ms = np.zeros(6, dtype=np.float64)
+        check(self.lib.fit_batch_tune_two_step(self.h, int(reps), C.byref(cfg), C.byref(kc),
+                                               dptr(ms)), "fit_batch_tune_two_step")
+        times = {(i // 3, (48, 64, 96)[i % 3]): float(v) for i, v in enumerate(ms) if v > 0}
+        return int(cfg.value), int(kc.value), times
 
     def device_bytes(self) -> int:
         """Device bytes of the batch's own tables."""

@@ -23,19 +23,19 @@ class BatchStepper:
     calls (``set_members``)."""
 
     def __init__(self, solvers, graph: bool = True, group: int = 16, tile=None,
-                 two_step: bool = True, kc2: int = 0):
+                 two_step: bool = True, kc2: int = 0, cfg2: int = -1):
         """two_step: FP32 batches whose members all suit the two-step sweep
         (nx >= 32, ny >= 16) advance pairs of steps with it (one read and
         write of E and H per two steps); False keeps the one-step launch.
-        kc2: planes per z chunk of the two-step sweep (0: by its launch
-        model)."""
+        kc2, cfg2: planes per z chunk and tile configuration (0: 32 x 16,
+        1: 24 x 24) of the two-step sweep (0, -1: by its launch model)."""
         self.solvers = list(solvers)
         for s in self.solvers:
             s._require_device()
         self._b = _lib.Batch([s.ctx for s in self.solvers],
                              [int(bool(s.use_conductivity)) for s in self.solvers])
         self._b.set_graph(graph, group)
-        self._b.set_two_step(two_step, kc2)
+        self._b.set_two_step(two_step, cfg2, kc2)
         if tile is not None:
             self._b.set_tile(int(tile))
 
@@ -66,8 +66,15 @@ class BatchStepper:
         return self._b.get_tile()
 
     def two_step(self):
-        """(pairs by the two-step sweep on the next step call, planes per chunk)."""
+        """(pairs by the two-step sweep on the next step call, its tile
+        configuration, planes per chunk)."""
         return self._b.two_step()
+
+    def tune_two_step(self, reps: int = 3):
+        """Time the two-step launch over its tile configurations and chunk
+        depths on this batch and keep the fastest; returns (cfg, kc,
+        {(cfg, kc): ms per launch})."""
+        return self._b.tune_two_step(reps)
 
     def device_bytes(self) -> int:
         """Device bytes of the batch's own tables (the members' are theirs)."""

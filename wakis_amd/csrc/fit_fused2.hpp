@@ -784,10 +784,17 @@ struct Batch2Member {
   const T *btab;       // beam rows of the group's steps (G x nz), or null
 };
 
-// 32 x 16 columns, a (36 x 20)-cell window (720 threads): less overhang than
-// 64-wide tiles on the narrow grids of a batch, and fewer window cells per
-// tile cell (1.41 against 1.59).
-constexpr int F2B_TX = 32, F2B_TY = 16;
+// Tiles of the batched launch: 32 x 16 columns (a 36 x 20 window, 720
+// threads) or 24 x 24 (28 x 28, 784 threads).  Against the 64 x 8 tile of the
+// single-grid launch both leave less overhang on the narrow grids of a batch
+// and have fewer window cells per tile cell (1.41 and 1.36 against 1.59);
+// 24 x 24 needs ny >= 24.
+enum { F2B_CFG_32x16 = 0, F2B_CFG_24x24 = 1, F2B_NCFG = 2 };
+
+inline void f2b_dims(int cfg, int &tx, int &ty) {
+  tx = cfg == F2B_CFG_24x24 ? 24 : 32;
+  ty = cfg == F2B_CFG_24x24 ? 24 : 16;
+}
 
 template <typename T, int TX, int TY>
 __global__ void __launch_bounds__((TX + 4) * (TY + 4))
@@ -839,33 +846,43 @@ __global__ void batch2_beam_table(const Batch2Member<T> *__restrict__ mem, int n
   }
 }
 
-// Planes per chunk of a batch: the least modelled time rounds x (kc + 8)
-// over the blocks of all members (a block's set-up and pipeline fill cost
-// about 8 planes: on 64 cavities of 120-320 planes, chunks of 64 and 96
-// planes ran 3 % faster than chunks of 32 and 48, and 128 (which leaves the
-// shorter members without a lean chunk) 13 % slower).
-inline int pick_kc2_batch(const int *ncol, const int *nz, int n, int slots) {
+// Modelled time of a batched two-step launch with tiles TX x TY and chunks
+// of kc planes: rounds of resident blocks (an eighth per XCD) x (kc + 8) x
+// the block's waves (a block's set-up and pipeline fill cost about 8 planes:
+// on 64 cavities of 120-320 planes, chunks of 64 and 96 planes ran 3 %
+// faster than chunks of 32 and 48, and 128, which leaves the shorter
+// members without a lean chunk, 13 % slower).
+inline double batch2_cost(const int *nx, const int *ny, const int *nz, int n, int tx, int ty,
+                          int kc, int slots) {
+  long long blocks = 0;
+  for (int m = 0; m < n; ++m) {
+    const long long ncol = (long long)((nx[m] + tx - 1) / tx) * ((ny[m] + ty - 1) / ty);
+    const int mid = nz[m] - 8;
+    blocks += ncol * (mid >= kc ? (mid + kc - 1) / kc : (nz[m] + kc - 1) / kc);
+  }
+  const int xslots = slots / 8 > 0 ? slots / 8 : 1;
+  const long long rounds = ((blocks + 7) / 8 + xslots - 1) / xslots;
+  const int waves = ((tx + 4) * (ty + 4) + 63) / 64;
+  return (double)rounds * (kc + 8) * waves;
+}
+
+// Planes per chunk for a tile shape: the least batch2_cost.
+inline int pick_kc2_batch(const int *nx, const int *ny, const int *nz, int n, int tx, int ty,
+                          int slots, double *cost = nullptr) {
   static const int cand[] = {32, 48, 64, 96, 128, 192, 256, 384};
   int nzmax = 0;
   for (int m = 0; m < n; ++m) nzmax = nz[m] > nzmax ? nz[m] : nzmax;
   int best = 64;
   double bt = 1e300;
-  const int xslots = slots / 8 > 0 ? slots / 8 : 1;
   for (int kc : cand) {
     if (kc > nzmax && kc != cand[0]) break;
-    long long blocks = 0;
-    for (int m = 0; m < n; ++m) {
-      const int mid = nz[m] - 8;
-      blocks += (long long)ncol[m] * (mid >= kc ? (mid + kc - 1) / kc : (nz[m] + kc - 1) / kc);
-    }
-    const long long per_xcd = (blocks + 7) / 8;
-    const long long rounds = (per_xcd + xslots - 1) / xslots;
-    const double t = (double)rounds * (kc + 8);
+    const double t = batch2_cost(nx, ny, nz, n, tx, ty, kc, slots);
     if (t < bt) {
       bt = t;
       best = kc;
     }
   }
+  if (cost) *cost = bt;
   return best;
 }
 
@@ -913,17 +930,22 @@ inline bool fused2_supported(const StepArgs<T> &a) {
 }
 
 // Whether a batch member's step arguments suit the batched two-step sweep
-// (the tile F2B_TX x F2B_TY; the conditions of fused2_supported otherwise).
-template <typename T>
-inline bool fused2_batch_supported(const StepArgs<T> &a) {
-  if constexpr (!cfg2_fits<T, F2B_TX, F2B_TY>()) {
+// with tiles TX x TY (the conditions of fused2_supported otherwise).
+template <typename T, int TX, int TY>
+inline bool fused2_batch_supported_t(const StepArgs<T> &a) {
+  if constexpr (!cfg2_fits<T, TX, TY>()) {
     (void)a;
     return false;
   } else {
-    return a.jmode != J_BUFFER && a.buf32 && a.nx >= F2B_TX && a.ny >= 16 && a.ny >= F2B_TY &&
-           a.nz >= 2 &&
-           kernel_spill_free((const void *)fused2_batch<T, F2B_TX, F2B_TY>);
+    return a.jmode != J_BUFFER && a.buf32 && a.nx >= TX && a.ny >= 16 && a.ny >= TY &&
+           a.nz >= 2 && kernel_spill_free((const void *)fused2_batch<T, TX, TY>);
   }
+}
+
+template <typename T>
+inline bool fused2_batch_supported(const StepArgs<T> &a, int cfg) {
+  return cfg == F2B_CFG_24x24 ? fused2_batch_supported_t<T, 24, 24>(a)
+                              : fused2_batch_supported_t<T, 32, 16>(a);
 }
 
 // Launches two steps; false (nothing launched) when not supported.
