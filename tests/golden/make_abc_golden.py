@@ -27,7 +27,9 @@ import sys
 import types
 from types import SimpleNamespace
 
-import numpy as np
+sys.dont_write_bytecode = True  # nothing is written under the reference tree
+
+import numpy as np  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF = "/root/reference/wakis"
