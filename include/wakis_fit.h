@@ -92,6 +92,15 @@ int fit_set_geometry(fit_ctx *ctx, const double *packed, int64_t count);
  * (wakis/solverFIT3D.py:337-344, update_tensors 378-436). */
 int fit_set_materials(fit_ctx *ctx, const uint8_t *ids, int32_t nmat,
                       const double *table);
+/* Materials with more than 256 distinct rows (per-cell ieps/imu/sigma Fields
+ * of any content, re-derived by update_tensors, wakis/solverFIT3D.py:378-436):
+ * ids[N] int32 in [0, nmat), nmat <= N, same table layout.  The table stays
+ * in global memory and every step of such a context runs the split kernels
+ * (with FIT_KERNEL_AUTO; FIT_KERNEL_FUSED/FUSED2 then fail with FIT_ESTATE),
+ * which read the cell's row per update: 4 bytes of id per cell and update
+ * beyond fit_set_materials.  fit_set_materials switches back. */
+int fit_set_materials_wide(fit_ctx *ctx, const int32_t *ids, int32_t nmat,
+                           const double *table);
 
 /* Boundary masks, bit f set = face f masked, f = xlo,xhi,ylo,yhi,zlo,zhi
  * (bits 0..5).  pec_cols: tangential-E column mask of C (PEC/PML faces,
@@ -155,6 +164,14 @@ int fit_wake_probe(fit_ctx *ctx, int32_t ixt, int32_t iyt, int32_t h,
                    int64_t nt_cap);
 /* Copy the probe history (window-major, then z, then time; ctx dtype). */
 int fit_wake_history(fit_ctx *ctx, void *host, int64_t count);
+/* Load a recorded E_z history from the host, for wake post-processing
+ * without a time loop (WakeSolver.solve on an existing Ez.h5 read by
+ * read_Ez, wakis/wakeSolver.py:211-255, 1317-1364): as fit_wake_probe(ctx,
+ * ixt, iyt, h, nt), then the (2h+1)^2 x nz x nt values of host (same layout
+ * as fit_wake_history: window w = (di+h)*(2h+1) + (dj+h), then z, then time;
+ * ctx dtype) are copied into the history.  Synchronous. */
+int fit_wake_history_load(fit_ctx *ctx, int32_t ixt, int32_t iyt, int32_t h,
+                          int64_t nt, const void *host);
 
 /* Frequency-domain field monitor (wakis/field_monitors.py:14-131: the
  * FieldMonitor that wakesolve updates after every step, wakis/routines.py:
@@ -253,14 +270,18 @@ int fit_sync(fit_ctx *ctx);
  * 311-421).  For every window w < nwin (taken in order from window_ids) and
  * every sample n < ns:
  *   acc = carry ? acc : 0;  for k < nzw: c = cols[k*ns + n];
- *     if c >= 0: acc = acc + hist[w][k0 + k][c] * dz[k];
- *   acc = acc / qnorm;  wp[w*ns + n] = acc.
+ *     if c >= 0: acc = acc + hist[w][r(k)][c] * dz[k];
+ *   acc = acc / qnorm;  wp[w*ns + n] = acc,
+ * with the history row r(k) = k0 + k, or k0 + nzw - 1 - k when zrev != 0
+ * (the counter-moving integral of wakis/wakeSolver.py:500-513, which walks z
+ * from the top; the caller passes dz[k] negated, as the reference weights
+ * that branch's terms by -dz[k]).
  * cols[nzw*ns] holds the time column of (n, k) (-1 = no contribution),
  * computed by the host exactly as the reference's int((...)/dt). */
 int fit_wake_potential(fit_ctx *ctx, int32_t nwin, const int32_t *window_ids,
                        int32_t carry, int32_t ns, int32_t nzw, int32_t k0,
-                       const int32_t *cols, const double *dz, double qnorm,
-                       double *wp);
+                       int32_t zrev, const int32_t *cols, const double *dz,
+                       double qnorm, double *wp);
 
 /* Impedance DFT (wakis/wakeSolver.py:676-703): bins j < K of the N-point
  * zero-padded (or truncated) DFT of x[len] and of lam[len], both FP64,

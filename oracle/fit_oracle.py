@@ -37,7 +37,8 @@ What is restated (citations are reference files, path:line):
 * ``OPlaneWave``        wakis/sources.py:142-248.
 * ``OWake``             wakis/wakeSolver.py:133-209 (fmax, ti), 311-421
                         ``calc_long_WP``, 423-551 ``calc_long_WP_3d`` (the
-                        WP carry-over between the 9 windows is kept),
+                        WP carry-over between the 9 windows is kept;
+                        its counter-moving branch 500-513),
                         553-626 ``calc_trans_WP``, 628-720 ``calc_long_Z``,
                         722-785 ``calc_trans_Z``, 842-874
                         ``calc_lambdas_analytic``.
@@ -49,7 +50,10 @@ Pinning (tests/test_oracle_*.py): reference golden vectors of
 tests/test_001_pec_cubic_cavity.py:18-67 (WP, Z, cumsums), TE101 frequency of
 tests/test_002_analytic_resonator.py:144, PML reflection bound of
 tests/test_005_pml_boundaries.py:99, discrete div(B) conservation (S.C = 0),
-brute-force loop vs vectorised WP, closed-form WP of a travelling Ez, and
+brute-force loop vs vectorised WP, closed-form WP of a travelling Ez, the
+counter-moving WP by the z-mirror identity (on a z axis symmetric about 0,
+the counter-moving WP of the z-reversed history equals minus the co-moving
+WP of the history, bit for bit, for any dz array), and
 calc_long_Z and calc_trans_Z against an independent direct DFT,
 calc_trans_WP against the Panofsky-Wenzel closed form of a field linear in
 x and y, and ``emsolve``'s ABC view aliasing against input/output vectors
@@ -812,7 +816,8 @@ class OWake:
     """WakeSolver: beam parameters, WP(s) and Z(f) post-processing."""
 
     def __init__(self, wakelength=None, q=1e-9, sigmaz=1e-3, beta=1.0, xsource=0.0,
-                 ysource=0.0, xtest=0.0, ytest=0.0, ti=None, fmax=None, skip_cells=0):
+                 ysource=0.0, xtest=0.0, ytest=0.0, ti=None, fmax=None, skip_cells=0,
+                 counter_moving=False):
         # wakeSolver.py:133-173
         self.q, self.sigmaz, self.beta = q, sigmaz, beta
         self.v = self.beta * c_light
@@ -828,7 +833,7 @@ class OWake:
         self.n_transverse_cells = 1
         self.lambdas = None
         self.chargedist = None
-        self.counter_moving = False
+        self.counter_moving = counter_moving
         self.Ez_hf = None
 
     # read_Ez (wakeSolver.py:1317-1364) from the in-memory dict
@@ -884,7 +889,19 @@ class OWake:
                 for n in range(nt):
                     Ez = self.Ez_hf[keys[n]]
                     Ezt[:, n] = Ez[Ez.shape[0] // 2 + i, Ez.shape[1] // 2 + j, zz]
-                if loops:
+                if self.counter_moving and loops:
+                    # wakeSolver.py:500-513
+                    for n in range(len(s)):
+                        for k in range(nz):
+                            ts = ((z[-k - 1] - s[n]) / (-1 * self.v) - zmax / (-1 * self.v)
+                                  - self.t[0] + ti)
+                            it = int(ts / dt)
+                            if it < nt:
+                                WP[n] = WP[n] + (Ezt[-k - 1, it]) * (-1 * self.dz[k])
+                elif self.counter_moving:
+                    WP = _wp_accumulate_counter(WP, Ezt, z, s, self.v, zmax, self.t[0], ti,
+                                                dt, nt, self.dz)
+                elif loops:
                     for n in range(len(s)):
                         for k in range(nz):
                             ts = (z[k] + s[n]) / self.v - zmin / self.v - self.t[0] + ti
@@ -995,6 +1012,21 @@ class OWake:
         elif compute_plane.lower() == "longitudinal":
             self.calc_long_WP()
             self.calc_long_Z()
+
+
+def _wp_accumulate_counter(WP, Ezt, z, s, v, zmax, t0, ti, dt, nt, dz):
+    """The counter-moving branch of wakeSolver.py:500-513, vectorised over n
+    with the loop's per-n order over k: term k samples z[-k-1] at the time
+    measured from zmax with the velocity negated, weighted by -dz[k] (dz
+    indexed from the bottom while z is walked from the top)."""
+    WP = WP.copy()
+    for k in range(len(z)):
+        ts = (z[-k - 1] - s) / (-1 * v) - zmax / (-1 * v) - t0 + ti
+        it = np.trunc(ts / dt).astype(np.int64)
+        ok = it < nt
+        idx = np.where(it < 0, it + Ezt.shape[1], it)
+        WP[ok] = WP[ok] + (Ezt[-k - 1, idx[ok]]) * (-1 * dz[k])
+    return WP
 
 
 def _wp_accumulate(WP, Ezt, z, s, v, zmin, t0, ti, dt, nt, dz):

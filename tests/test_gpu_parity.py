@@ -87,6 +87,71 @@ def test_conductive_background_and_update_tensors():
     assert max(err.values()) <= TOL64, err
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("tensor", ["all", "ieps"])
+def test_graded_material_tensors_beyond_256_rows(dtype, tensor):
+    """Per-cell material tensors of any content through update_tensors
+    (wakis/solverFIT3D.py:378-436): a dielectric graded in z and x (eps_r
+    1..3, a different value in nearly every cell, several thousand distinct
+    rows) with a conductive slab, stepped with J = sigma*E and a beam line;
+    the solver takes the wide material ids (split kernels) and matches the
+    oracle.  tensor="ieps" rebuilds only the permittivity (sigma is left as
+    the background gives it)."""
+    import wakis_amd as wa
+
+    shape = (14, 12, 20)
+    print(f"[case] graded tensors {np.dtype(dtype).name} tensor={tensor}", flush=True)
+    osv, psv = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, bg=[1.0, 1.0, 10.0], seed=11,
+                         dtype=dtype, init=False)
+    Nx, Ny, Nz = shape
+    i = np.arange(Nx)[:, None, None]
+    k = np.arange(Nz)[None, None, :]
+    j = np.arange(Ny)[None, :, None]
+    epsr = 1.0 + 2.0 * (k + 0.37 * i + 0.011 * j) / (Nz + 0.37 * Nx)
+    ieps_m = np.broadcast_to(1.0 / (epsr * 8.8541878128e-12), shape)
+    sig_m = np.where((k >= 8) & (k < 11), 25.0, 10.0) * np.ones(shape)
+    for s in (osv, psv):
+        f = s.ieps
+        for d in "xyz":
+            f[:, :, :, d] = ieps_m
+        s.ieps = f
+        if tensor == "all":
+            g = s.sigma
+            for d in "xyz":
+                g[:, :, :, d] = sig_m
+            s.sigma = g
+        s.update_tensors(tensor)
+    assert psv.ctx.kernel_in_use() == wa._lib.KERNEL_SPLIT
+    rows = len(psv._mt.table())
+    assert rows > 256, rows
+    ob = OBeam(q=1e-9, sigmaz=0.01, beta=1.0)
+    pb = wa.Beam(q=1e-9, sigmaz=0.01, beta=1.0)
+    for n in range(30):
+        ob.update(osv, n * osv.dt)
+        pb.update(psv, n * psv.dt)
+        osv.one_step()
+        psv.one_step()
+    err = compare_fields(osv, psv)
+    print(rows, err)
+    assert max(err.values()) <= (TOL64 if dtype == np.float64 else TOL32), err
+
+
+def test_wide_materials_refuse_fused_kernels():
+    """An explicit fused kernel on a context with more than 256 material
+    rows is refused; batches refuse such members."""
+    import wakis_amd as wa
+
+    shape = (8, 8, 10)
+    print("[case] wide materials refuse fused", flush=True)
+    _, psv = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, init=False, kernel="fused")
+    f = psv.ieps
+    f.fromarray(np.linspace(1.0, 2.0, 3 * psv.N) / 8.8541878128e-12)
+    psv.ieps = f
+    psv.update_tensors()
+    with pytest.raises(wa._lib.FitError, match="256 material rows"):
+        psv.one_step()
+
+
 def test_beam_user_loop_fp64():
     """Beam.update + one_step in a user loop (wakis/sources.py:76-110)."""
     import wakis_amd as wa

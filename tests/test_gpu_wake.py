@@ -399,3 +399,106 @@ def test_run_log_keys_like_reference(tmp_path):
                                                       "Grid Logs"]
     grid_log = js.loads(parts[3].split("\n", 1)[1])
     assert grid_log["stl_materials"] == {"cavity": [1.0, 1.0]} and grid_log["Nx"] == 16
+
+
+@pytest.mark.parametrize("skip", [0, 2])
+def test_counter_moving_wakesolve(skip):
+    """WakeSolver(counter_moving=True): the wake integral of
+    wakis/wakeSolver.py:500-513 on the device after a wakesolve, against the
+    oracle's (whose counter-moving branch is pinned by the z-mirror identity,
+    tests/test_oracle.py), FP64 within 1e-10."""
+    shape = (14, 13, 36)
+    kw = dict(WAKE_KW, skip_cells=skip, counter_moving=True)
+    print(f"[case] counter-moving wakesolve skip={skip} shape={shape}", flush=True)
+    osv, psv = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, bg="pec", solids=_cavity(shape),
+                         init=False, cfln=1.0, wake_kw=kw)
+    osv.wakesolve(compute_plane="both")
+    psv.wakesolve(compute_plane="both")
+    ow, pw = osv.wake, psv.wake
+    assert np.abs(ow.WP_3d).max() > 0
+    errs = {"WP3d": rel(pw.WP_3d, ow.WP_3d), "WP": rel(pw.WP, ow.WP), "Z": rel(pw.Z, ow.Z),
+            "WPx": rel(pw.WPx, ow.WPx), "Zx": rel(pw.Zx, ow.Zx)}
+    print(errs)
+    assert max(errs.values()) <= 1e-10, errs
+
+
+def _mirror_history(nz=20, nt=90, seed=9, dtype=np.float32):
+    rng = np.random.default_rng(seed)
+    zc = (np.arange(nz) - (nz - 1) / 2) * 0.004
+    dz = rng.uniform(0.003, 0.005, nz)
+    dt = np.float32(6.1e-12)
+    hf = {"x": np.arange(3.0) * 1e-3, "y": np.arange(3.0) * 1e-3, "z": zc,
+          "dx": np.full(3, 1e-3), "dy": np.full(3, 1e-3), "dz": dz,
+          "t": np.arange(0, nt * dt, dt)}
+    for n in range(nt):
+        hf["#" + str(n).zfill(5)] = rng.standard_normal((3, 3, nz)).astype(dtype)
+    return hf
+
+
+@pytest.mark.parametrize("skip", [0, 3])
+def test_counter_moving_mirror_identity_on_device(skip):
+    """On a z axis symmetric about 0, the device's counter-moving WP of the
+    z-reversed history is minus its co-moving WP of the history, bit for bit,
+    and both equal the oracle's."""
+    import wakis_amd as wa
+
+    print(f"[case] counter-moving mirror identity skip={skip}", flush=True)
+    hf = _mirror_history()
+    rev = dict(hf)
+    for key in hf:
+        if key.startswith("#"):
+            rev[key] = np.ascontiguousarray(hf[key][:, :, ::-1])
+    kw = dict(wakelength=0.05, sigmaz=0.005, skip_cells=skip)
+    co = wa.WakeSolver(save=False, **kw)
+    cm = wa.WakeSolver(save=False, counter_moving=True, **kw)
+    co.Ez_hf, cm.Ez_hf = hf, rev
+    co.calc_long_WP_3d()
+    cm.calc_long_WP_3d()
+    assert np.abs(co.WP_3d).max() > 0
+    assert np.array_equal(cm.WP_3d, -co.WP_3d)
+    ocm = OWake(counter_moving=True, **kw)
+    ocm.Ez_hf = rev
+    ocm.calc_long_WP_3d()
+    assert np.array_equal(cm.WP_3d, ocm.WP_3d)
+
+
+@pytest.mark.parametrize("source", ["dict", "npz", "npz_f32"])
+def test_standalone_solve_from_recorded_history(source, tmp_path):
+    """WakeSolver.solve without a time loop (wakis/wakeSolver.py:211-255 on
+    a history read by read_Ez, 1317-1364): the oracle's recorded E_z
+    datasets ('#%05d' keys plus x, y, z, dx, dy, dz, t), given as a dict
+    (Ez_hf) or an .npz archive (Ez_file), are moved to the device and give
+    the oracle's WP_3d bit for bit and its WPx, WPy, Z, Zx, Zy within 1e-10."""
+    import wakis_amd as wa
+
+    shape = (12, 12, 30)
+    print(f"[case] standalone solve from {source}", flush=True)
+    dtype = np.float32 if source == "npz_f32" else np.float64
+    osv, _ = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, bg="pec", solids=_cavity(shape),
+                       init=False, cfln=1.0, wake_kw=WAKE_KW, dtype=dtype)
+    osv.wakesolve(compute_plane="both")
+    ow = osv.wake
+    hf = dict(ow.Ez_hf) if ow.Ez_hf is not None else dict(osv.Ez_hf)
+    assert any(k.startswith("#") for k in hf)
+    if source == "dict":
+        pw = wa.WakeSolver(save=False, **WAKE_KW)
+        pw.Ez_hf = hf
+    else:
+        path = str(tmp_path / "Ez.npz")
+        np.savez(path, **hf)
+        pw = wa.WakeSolver(save=False, Ez_file=path, **WAKE_KW)
+    pw.solve(compute_plane="both")
+    assert np.array_equal(pw.s, ow.s)
+    assert np.array_equal(pw.WP_3d, ow.WP_3d)
+    errs = {"WPx": rel(pw.WPx, ow.WPx), "WPy": rel(pw.WPy, ow.WPy), "Z": rel(pw.Z, ow.Z),
+            "Zx": rel(pw.Zx, ow.Zx), "Zy": rel(pw.Zy, ow.Zy)}
+    print(errs)
+    assert max(errs.values()) <= 1e-10, errs
+    # the longitudinal-only path reads window (2, 2) of the 3x3 datasets
+    pl = wa.WakeSolver(save=False, **WAKE_KW_LONG)
+    pl.Ez_hf = hf
+    pl.solve(compute_plane="longitudinal")
+    ol = OWake(**WAKE_KW_LONG)
+    ol.Ez_hf = hf
+    ol.calc_long_WP()
+    assert np.array_equal(pl.WP, ol.WP)

@@ -190,6 +190,95 @@ def test_bucket_columns_match_reference_expression():
             assert cols[k, n] == (it if it < nt else -1)
 
 
+def test_counter_moving_bucket_columns_match_reference_expression():
+    """counter=True: int(ts/dt) of the counter-moving branch,
+    wakeSolver.py:500-508, term k at z[-k-1] with times from zmax."""
+    import wakis_amd as wa
+
+    w = wa.WakeSolver(wakelength=0.05, sigmaz=0.005, save=False, skip_cells=2,
+                      counter_moving=True)
+    z = np.sort(np.random.default_rng(3).uniform(-0.04, 0.04, 18))
+    zc = (z[1:] + z[:-1]) / 2
+    dt = np.float32(2.7e-12)
+    nt = 300
+    w.attach_device_history(None, {"x": np.arange(3.0), "y": np.arange(3.0), "z": zc,
+                                   "dx": 0, "dy": 0, "dz": np.diff(z),
+                                   "t": np.arange(0, nt * dt, dt), "nsteps": nt, "half": 1})
+    cols, s, zs = w.bucket_columns(nt, counter=True)
+    assert (cols >= 0).any()
+    for n in range(0, len(s), 7):
+        for k in range(len(zs)):
+            ts = ((zs[-k - 1] - s[n]) / (-1 * w.v) - zc.max() / (-1 * w.v) - w.t[0] + w.ti)
+            it = int(ts / (w.t[2] - w.t[1]))
+            assert cols[k, n] == (it if it < nt else -1)
+
+
+class _HistoryCapture:
+    """Stands in for the device context: keeps what would be uploaded."""
+
+    made = []
+
+    def __init__(self, nx, ny, nz, dtype=np.float64, device=0):
+        self.shape, self.dtype = (nx, ny, nz), np.dtype(dtype)
+        _HistoryCapture.made.append(self)
+
+    def wake_history_load(self, ixt, iyt, h, hist):
+        self.probe, self.hist = (ixt, iyt, h), np.array(hist)
+
+
+@pytest.mark.parametrize("win", [3, 5, 1])
+def test_recorded_history_layout(win, monkeypatch, tmp_path):
+    """read_Ez's datasets become the device history [w][k][t] with
+    w = (di+1)*3 + (dj+1) over the centred 3x3 window (rows shape//2 + i, as
+    wakeSolver.py:492-498 indexes a dataset), or the line of a 1-D dataset;
+    the first min(len(t), number of datasets) steps are kept; an .npz
+    archive gives the same history as the dict."""
+    import wakis_amd as wa
+    from wakis_amd import _lib
+
+    monkeypatch.setattr(_lib, "Context", _HistoryCapture)
+    rng = np.random.default_rng(2)
+    nz, nkeys = 7, 11
+    t = np.arange(9) * 1e-12  # fewer times than datasets: nt = 9
+    hf = {"x": np.arange(3.0), "y": np.arange(3.0), "z": np.arange(nz) * 1e-3,
+          "dx": np.ones(3), "dy": np.ones(3), "dz": np.full(nz, 1e-3), "t": t}
+    shape = (win, win, nz) if win > 1 else (nz,)
+    for n in range(nkeys):
+        hf["#" + str(n).zfill(5)] = rng.standard_normal(shape).astype(np.float32)
+    w = wa.WakeSolver(save=False)
+    w.Ez_hf = hf
+    w._load_host_history()
+    cap = _HistoryCapture.made[-1]
+    assert cap.dtype == np.float32 and w._meta["nsteps"] == 9
+    if win > 1:
+        c = win // 2
+        assert cap.shape == (3, 3, nz) and cap.probe == (1, 1, 1)
+        for n in range(9):
+            e = hf["#" + str(n).zfill(5)]
+            for di in (-1, 0, 1):
+                for dj in (-1, 0, 1):
+                    assert np.array_equal(cap.hist[(di + 1) * 3 + (dj + 1), :, n],
+                                          e[c + di, c + dj, :])
+    else:
+        assert cap.probe == (0, 0, 0) and cap.hist.shape == (1, nz, 9)
+        for n in range(9):
+            assert np.array_equal(cap.hist[0, :, n], hf["#" + str(n).zfill(5)])
+    path = str(tmp_path / "Ez.npz")
+    np.savez(path, **hf)
+    w2 = wa.WakeSolver(save=False, Ez_file=path)
+    w2._need()
+    assert np.array_equal(_HistoryCapture.made[-1].hist, cap.hist)
+    assert np.array_equal(w2.t, t) and np.array_equal(w2.dz, hf["dz"])
+
+
+def test_missing_history_raises():
+    import wakis_amd as wa
+
+    w = wa.WakeSolver(save=False, Ez_file="/nonexistent/Ez.npz")
+    with pytest.raises(RuntimeError, match="no E_z history"):
+        w.calc_long_WP_3d()
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_slice_solids_and_tensor_material_equal_oracle(dtype):
     """Index-slice solids and a diagonal-tensor material {"eps": (ex, ey, ez)}

@@ -183,6 +183,60 @@ def test_wp_vectorised_equals_loops():
         assert np.array_equal(a.WP, b.WP)
 
 
+def _mirror_history(nz=10, nt=60, seed=9, dtype=np.float32):
+    """A 3x3 history on a z axis symmetric about 0 (z[nz-1-k] == -z[k]
+    exactly) with an arbitrary, non-symmetric dz array."""
+    rng = np.random.default_rng(seed)
+    zc = (np.arange(nz) - (nz - 1) / 2) * 0.004
+    assert np.array_equal(zc[::-1], -zc)
+    dz = rng.uniform(0.003, 0.005, nz)
+    dt = np.float32(6.1e-12)
+    hf = {"x": np.arange(3.0), "y": np.arange(3.0), "z": zc, "dx": np.ones(3), "dy": np.ones(3),
+          "dz": dz, "t": np.arange(0, nt * dt, dt)}
+    for n in range(nt):
+        hf["#" + str(n).zfill(5)] = rng.standard_normal((3, 3, nz)).astype(dtype)
+    return hf
+
+
+def test_counter_moving_wp_vectorised_equals_loops():
+    """The vectorised counter-moving accumulation equals the reference's
+    double loop of wakeSolver.py:500-513 bit for bit, with skip_cells and
+    the carry-over between windows."""
+    hf = _mirror_history(nz=9, nt=40, seed=4)
+    for skip in (0, 2):
+        a = OWake(wakelength=0.02, sigmaz=0.004, skip_cells=skip, counter_moving=True)
+        b = OWake(wakelength=0.02, sigmaz=0.004, skip_cells=skip, counter_moving=True)
+        a.Ez_hf = b.Ez_hf = hf
+        a.calc_long_WP_3d(loops=True)
+        b.calc_long_WP_3d(loops=False)
+        assert np.array_equal(a.WP_3d, b.WP_3d)
+        assert np.abs(a.WP_3d).max() > 0
+
+
+@pytest.mark.parametrize("skip", [0, 2])
+def test_counter_moving_wp_mirror_identity(skip):
+    """On a z axis symmetric about 0, the counter-moving integral of the
+    z-reversed history is minus the co-moving integral of the history, bit
+    for bit: the counter-moving time (z[-k-1] - s)/(-v) - zmax/(-v) equals
+    the co-moving (z[k] + s)/v - zmin/v exactly there, term k reads row
+    nz-1-k of the reversed history (row k of the original) and is weighted
+    by -dz[k].  With a non-symmetric dz this fails for a dz indexed from the
+    top, a dropped sign, a time measured from zmin, or an unreversed z walk."""
+    hf = _mirror_history()
+    rev = dict(hf)
+    for key in hf:
+        if key.startswith("#"):
+            rev[key] = np.ascontiguousarray(hf[key][:, :, ::-1])
+    co = OWake(wakelength=0.03, sigmaz=0.005, skip_cells=skip)
+    cm = OWake(wakelength=0.03, sigmaz=0.005, skip_cells=skip, counter_moving=True)
+    co.Ez_hf, cm.Ez_hf = hf, rev
+    co.calc_long_WP_3d()
+    cm.calc_long_WP_3d()
+    assert np.abs(co.WP_3d).max() > 0
+    assert np.array_equal(cm.WP_3d, -co.WP_3d)
+    assert np.array_equal(cm.s, co.s)
+
+
 def test_wp_closed_form_travelling_wave():
     """Ez(z, t) = g(z - v t) sampled on the time grid integrates to
     WP(s) = -sum_k g(-s) dz / (q 1e12) up to the time bucketing: with

@@ -53,6 +53,7 @@ SIGNATURES = {
     "fit_destroy": [_P],
     "fit_set_geometry": [_P, _PD, _I64],
     "fit_set_materials": [_P, _P, _I32, _PD],
+    "fit_set_materials_wide": [_P, _PI32, _I32, _PD],
     "fit_set_bc": [_P, _U32, _U32],
     "fit_set_dt": [_P, _D],
     "fit_set_jmode": [_P, _I32],
@@ -66,6 +67,7 @@ SIGNATURES = {
     "fit_apply_abc": [_P, _U32],
     "fit_wake_probe": [_P, _I32, _I32, _I32, _I64],
     "fit_wake_history": [_P, _P, _I64],
+    "fit_wake_history_load": [_P, _I32, _I32, _I32, _I64, _P],
     "fit_monitor_set": [_P, _I32],
     "fit_monitor_accumulate": [_P, _PD],
     "fit_monitor_phases": [_P, _PD, _I64],
@@ -85,7 +87,8 @@ SIGNATURES = {
     "fit_get_tile": [_P, _PI32, _PI32, _PI32, _PI32],
     "fit_tune": [_P, _I32, _PD],
     "fit_sync": [_P],
-    "fit_wake_potential": [_P, _I32, _PI32, _I32, _I32, _I32, _I32, _PI32, _PD, _D, _PD],
+    "fit_wake_potential": [_P, _I32, _PI32, _I32, _I32, _I32, _I32, _I32, _PI32, _PD, _D,
+                           _PD],
     "fit_impedance": [_P, _PD, _PD, _I64, _I64, _I32, _D, _PD, _PD, _PD, _PD, _PD, _PD],
     "fit_impedance_noalloc": [_P, _PD, _PD, _I64, _I64, _I32, _D, _PD, _PD, _PD, _PD, _PD,
                               _PD],
@@ -212,6 +215,14 @@ class Context:
             assert ids.size == self.N
             p = C.c_void_p(ids.ctypes.data)
         check(self.lib.fit_set_materials(self.h, p, nmat, dptr(table)), "fit_set_materials")
+
+    def set_materials_wide(self, ids, table: np.ndarray):
+        """More than 256 material rows: int32 ids per cell (Fortran order)."""
+        table = np.ascontiguousarray(table, dtype=np.float64)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        assert ids.size == self.N
+        check(self.lib.fit_set_materials_wide(self.h, iptr(ids), table.shape[0], dptr(table)),
+              "fit_set_materials_wide")
 
     def set_bc(self, pec: int, pmc: int):
         check(self.lib.fit_set_bc(self.h, pec, pmc), "fit_set_bc")
@@ -380,16 +391,29 @@ class Context:
         check(self.lib.fit_wake_history(self.h, vptr(out), out.size), "fit_wake_history")
         return out.reshape(nw, self.nz, nt)
 
+    def wake_history_load(self, ixt: int, iyt: int, h: int, hist: np.ndarray):
+        """Load a recorded history hist[(2h+1)^2, nz, nt] (fit_wake_history_load)."""
+        nw = (2 * h + 1) ** 2
+        hist = np.ascontiguousarray(hist, dtype=self.np_dtype)
+        if hist.ndim != 3 or hist.shape[0] != nw or hist.shape[1] != self.nz:
+            raise ValueError(f"history shape {hist.shape}: expected ({nw}, {self.nz}, nt)")
+        nt = hist.shape[2]
+        check(self.lib.fit_wake_history_load(self.h, ixt, iyt, h, nt, vptr(hist)),
+              "fit_wake_history_load")
+        self.probe = (ixt, iyt, h, int(nt))
+
     def wake_potential(self, window_ids, carry: bool, cols: np.ndarray, k0: int,
-                       dz: np.ndarray, qnorm: float) -> np.ndarray:
+                       dz: np.ndarray, qnorm: float, zrev: bool = False) -> np.ndarray:
+        """fit_wake_potential; zrev walks the history rows from the top of the
+        window (dz[k] is passed as given: the counter-moving caller negates it)."""
         w = np.ascontiguousarray(window_ids, dtype=np.int32)
         cols = np.ascontiguousarray(cols, dtype=np.int32)
         nzw, ns = cols.shape
         dz = np.ascontiguousarray(dz[:nzw], dtype=np.float64)
         out = np.empty(len(w) * ns, dtype=np.float64)
         check(self.lib.fit_wake_potential(self.h, len(w), iptr(w), int(bool(carry)), ns, nzw,
-                                          int(k0), iptr(cols), dptr(dz), float(qnorm),
-                                          dptr(out)), "fit_wake_potential")
+                                          int(k0), int(bool(zrev)), iptr(cols), dptr(dz),
+                                          float(qnorm), dptr(out)), "fit_wake_potential")
         return out.reshape(len(w), ns)
 
     def impedance(self, x: np.ndarray, lam: np.ndarray, N: int, K: int, ds: float,

@@ -115,8 +115,7 @@ template <typename T>
 __global__ void materialize_j(StepArgs<T> a) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= a.N) return;
-  const int m = a.mat ? a.mat[n] : 0;
-  const T *mt = a.mtab + m * MAT_STRIDE;
+  const T *mt = mat_row(a, n);
   T *J = const_cast<T *>(a.J);
   // jmode < 0: J is in its buffer already (only the beam line is applied)
   if (a.jmode >= 0) {
@@ -170,12 +169,17 @@ __global__ void monitor_accumulate(const T *__restrict__ E, int64_t N, int nf,
 template <typename T>
 __global__ void wake_potential(const T *__restrict__ hist, int64_t hist_nt, int nzh,
                                const int32_t *__restrict__ wids, int nwin, int carry,
-                               int ns, int nzw, int k0, const int32_t *__restrict__ cols,
+                               int ns, int nzw, int k0, int zrev,
+                               const int32_t *__restrict__ cols,
                                const double *__restrict__ dz, double qnorm,
                                double *__restrict__ wp) {
 #pragma clang fp contract(off)
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= ns) return;
+  // history row of term k: k0 + k, or walking z down from the top row of the
+  // window (k0 + nzw - 1 - k) for the counter-moving integral
+  const int r0 = zrev ? k0 + nzw - 1 : k0;
+  const int rs = zrev ? -1 : 1;
   double acc = 0.0;
   for (int w = 0; w < nwin; ++w) {
     if (!carry) acc = 0.0;
@@ -183,7 +187,7 @@ __global__ void wake_potential(const T *__restrict__ hist, int64_t hist_nt, int 
     for (int k = 0; k < nzw; ++k) {
       const int c = cols[(int64_t)k * ns + n];
       if (c >= 0) {
-        const double e = (double)hw[(int64_t)(k0 + k) * hist_nt + c];
+        const double e = (double)hw[(int64_t)(r0 + rs * k) * hist_nt + c];
         const double prod = e * dz[k];
         acc = acc + prod;
       }

@@ -48,6 +48,10 @@ struct fit_ctx {
   void *E_alt = nullptr, *H_alt = nullptr;  // second buffers of the fused sweep
   uint8_t *mat = nullptr;
   bool mat_uniform = true;
+  // wide material ids (fit_set_materials_wide): one int32 per cell, the table
+  // in global memory; stepped by the split kernels only
+  int32_t *mat32 = nullptr;
+  bool wide = false;
   void *mtab = nullptr;
   int nmat = 0;
   std::vector<double> mtab_host;  // nmat * 9 FP64 (1/eps, 1/mu, sigma)
@@ -103,6 +107,7 @@ int64_t resident_bytes(const fit_ctx *c) {
   for (const void *p : {c->E, c->H, c->J, c->E_alt, c->H_alt})
     if (p) b += fb;
   if (c->mat) b += c->N;
+  if (c->mat32) b += 4 * c->N;
   if (c->geo) b += c->geo_count * (int64_t)c->esz;
   if (c->jline) b += (int64_t)c->nz * c->esz;
   if (c->zc) b += (int64_t)c->nz * 8;
@@ -133,8 +138,8 @@ int ensure_staging(fit_ctx *c, size_t bytes) {
 int upload_mtab(fit_ctx *c) {
   if (!c->mat_set) return 0;
   const int nm = c->nmat;
-  std::vector<double> t(nm * fit::MAT_STRIDE);
-  for (int m = 0; m < nm; ++m) {
+  std::vector<double> t((size_t)nm * fit::MAT_STRIDE);
+  for (size_t m = 0; m < (size_t)nm; ++m) {
     const double *r = &c->mtab_host[m * 9];
     for (int d = 0; d < 3; ++d) {
       t[m * 9 + d] = c->dt * r[3 + d];      // dt/mu
@@ -164,7 +169,8 @@ fit::StepArgs<T> make_args(fit_ctx *c) {
   a.E = (T *)c->E;
   a.H = (T *)c->H;
   a.J = (const T *)c->J;
-  a.mat = c->mat_uniform ? nullptr : c->mat;
+  a.mat = (c->mat_uniform || c->wide) ? nullptr : c->mat;
+  a.mat32 = c->wide ? c->mat32 : nullptr;
   a.mtab = (const T *)c->mtab;
   a.nmat = c->nmat;
   {
@@ -233,6 +239,13 @@ inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs
 // FIT_KERNEL_AUTO a failed allocation selects the split kernels instead.
 int want_fused(fit_ctx *c, bool *fused) {
   *fused = false;
+  if (c->wide) {
+    // the fused sweeps keep the material table in LDS (at most 256 rows)
+    if (c->kernel == FIT_KERNEL_FUSED || c->kernel == FIT_KERNEL_FUSED2)
+      return fail(FIT_ESTATE, "more than 256 material rows: the fused sweeps do not apply "
+                              "(use FIT_KERNEL_AUTO or FIT_KERNEL_SPLIT)");
+    return 0;
+  }
   if (!fit::fused_supported(c->kernel, c->nx, c->ny, c->nz)) return 0;
   if (!c->E_alt || !c->H_alt) {
     const size_t fb = (size_t)3 * c->N * c->esz;
@@ -522,7 +535,7 @@ int fit_destroy(fit_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ps[] = {c->E,   c->H,     c->J,  c->E_alt, c->H_alt,   c->mat,  c->mtab,
                 c->geo, c->jline, c->zc, c->hist,  c->staging, c->btab, c->mon,
-                c->mon_ph};
+                c->mon_ph, c->mat32};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -558,18 +571,45 @@ int fit_set_materials(fit_ctx *c, const uint8_t *ids, int32_t nmat, const double
   if (!c || !table) return fail(FIT_EINVAL, "null argument");
   if (nmat < 1 || nmat > 256) return fail(FIT_EINVAL, "nmat must be in [1, 256]");
   if (int rc = set_device(c)) return rc;
-  c->mtab_host.assign(table, table + (size_t)nmat * 9);
-  c->nmat = nmat;
   if (ids) {
     for (int64_t n = 0; n < c->N; ++n)
       if (ids[n] >= nmat) return fail(FIT_EINVAL, "material id out of range");
+  }
+  c->mtab_host.assign(table, table + (size_t)nmat * 9);
+  c->nmat = nmat;
+  if (ids) {
     HIPCHK(hipMemcpy(c->mat, ids, (size_t)c->N, hipMemcpyHostToDevice));
     c->mat_uniform = false;
   } else {
     HIPCHK(hipMemset(c->mat, 0, (size_t)c->N));
     c->mat_uniform = true;
   }
+  if (c->mat32) HIPCHK(hipFree(c->mat32));
+  c->mat32 = nullptr;
+  c->wide = false;
   c->mat_set = true;
+  return upload_mtab(c);
+}
+
+int fit_set_materials_wide(fit_ctx *c, const int32_t *ids, int32_t nmat, const double *table) {
+  if (!c || !ids || !table) return fail(FIT_EINVAL, "null argument");
+  if (nmat < 1 || (int64_t)nmat > c->N) return fail(FIT_EINVAL, "nmat must be in [1, N]");
+  for (int64_t n = 0; n < c->N; ++n)
+    if (ids[n] < 0 || ids[n] >= nmat) return fail(FIT_EINVAL, "material id out of range");
+  if (int rc = set_device(c)) return rc;
+  if (!c->mat32) {
+    if (hipMalloc((void **)&c->mat32, (size_t)c->N * 4) != hipSuccess) {
+      c->mat32 = nullptr;
+      return fail(FIT_ENOMEM, "material id allocation failed");
+    }
+  }
+  HIPCHK(hipMemcpy(c->mat32, ids, (size_t)c->N * 4, hipMemcpyHostToDevice));
+  c->mtab_host.assign(table, table + (size_t)nmat * 9);
+  c->nmat = nmat;
+  c->wide = true;
+  c->mat_uniform = false;
+  c->mat_set = true;
+  note_peak(c);
   return upload_mtab(c);
 }
 
@@ -797,6 +837,17 @@ int fit_wake_history(fit_ctx *c, void *host, int64_t count) {
   if (count != (int64_t)nw * c->nz * c->hist_nt) return fail(FIT_EINVAL, "history count mismatch");
   if (int rc = set_device(c)) return rc;
   HIPCHK(hipMemcpyAsync(host, c->hist, (size_t)count * c->esz, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fit_wake_history_load(fit_ctx *c, int32_t ixt, int32_t iyt, int32_t h, int64_t nt,
+                          const void *host) {
+  if (!c || !host) return fail(FIT_EINVAL, "null argument");
+  if (nt <= 0) return fail(FIT_EINVAL, "history needs at least one time column");
+  if (int rc = fit_wake_probe(c, ixt, iyt, h, nt)) return rc;
+  const size_t bytes = c->hist_bytes;
+  HIPCHK(hipMemcpyAsync(c->hist, host, bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1162,8 +1213,8 @@ int fit_sync(fit_ctx *c) {
 }
 
 int fit_wake_potential(fit_ctx *c, int32_t nwin, const int32_t *wids, int32_t carry, int32_t ns,
-                       int32_t nzw, int32_t k0, const int32_t *cols, const double *dz,
-                       double qnorm, double *wp) {
+                       int32_t nzw, int32_t k0, int32_t zrev, const int32_t *cols,
+                       const double *dz, double qnorm, double *wp) {
   if (!c || !wids || !cols || !dz || !wp) return fail(FIT_EINVAL, "null argument");
   if (!c->hist) return fail(FIT_ESTATE, "no probe history");
   const int nwh = (2 * c->hw + 1) * (2 * c->hw + 1);
@@ -1198,11 +1249,11 @@ int fit_wake_potential(fit_ctx *c, int32_t nwin, const int32_t *wids, int32_t ca
     if (c->dtype == FIT_F64)
       hipLaunchKernelGGL(fit::wake_potential<double>, dim3(nblocks(ns, 256)), dim3(256), 0,
                          c->stream, (const double *)c->hist, c->hist_nt, c->nz, dw, nwin, carry,
-                         ns, nzw, k0, dcols, ddz, qnorm, dwp);
+                         ns, nzw, k0, zrev != 0, dcols, ddz, qnorm, dwp);
     else
       hipLaunchKernelGGL(fit::wake_potential<float>, dim3(nblocks(ns, 256)), dim3(256), 0,
                          c->stream, (const float *)c->hist, c->hist_nt, c->nz, dw, nwin, carry,
-                         ns, nzw, k0, dcols, ddz, qnorm, dwp);
+                         ns, nzw, k0, zrev != 0, dcols, ddz, qnorm, dwp);
     e = hipGetLastError();
   }
   if (e == hipSuccess)

@@ -71,6 +71,7 @@ struct StepArgs {
   T *H;            // 3N
   const T *J;      // 3N (J_BUFFER only)
   const uint8_t *mat;  // N material ids (nullptr: all 0)
+  const int32_t *mat32;  // N wide material ids (more than 256 rows), else nullptr
   const T *mtab;   // nmat * MAT_STRIDE
   int nmat;
   Geo<T> g;
@@ -225,6 +226,14 @@ __device__ __forceinline__ void gstore(P *p, V x, const bool nt) {
 }
 __device__ __forceinline__ int bload_u8(Rsrc r, uint32_t v, uint32_t s) {
   return (int)__builtin_amdgcn_raw_buffer_load_b8(r, (int)v, (int)s, 0);
+}
+
+// Material row of cell n: the wide ids when set (split kernels only), else
+// the 8-bit ids, else row 0.
+template <typename T>
+__device__ __forceinline__ const T *mat_row(const StepArgs<T> &a, int64_t n) {
+  const int64_t m = a.mat32 ? (int64_t)a.mat32[n] : (a.mat ? (int64_t)a.mat[n] : 0);
+  return a.mtab + m * MAT_STRIDE;
 }
 
 // Wake probe: E_z of the state before the update of this step

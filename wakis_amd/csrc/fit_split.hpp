@@ -57,8 +57,7 @@ __global__ void __launch_bounds__(SPLIT_BX *SPLIT_BY) h_update_split(StepArgs<T>
       uy_pz = u_at(a, 1, ii, jj, kk);
     }
   }
-  const int m = a.mat ? a.mat[n] : 0;
-  const T *mt = a.mtab + m * MAT_STRIDE;
+  const T *mt = mat_row(a, n);
   const T cx = masked(a.pmc, 0, i, j, k, nx, ny, nz) ? T(0) : mt[0] * a.g.iAx[j + (int64_t)ny * k];
   const T cy = masked(a.pmc, 1, i, j, k, nx, ny, nz) ? T(0) : mt[1] * a.g.iAy[i + (int64_t)nx * k];
   const T cz = masked(a.pmc, 2, i, j, k, nx, ny, nz) ? T(0) : mt[2] * a.g.iAz[i + (int64_t)nx * j];
@@ -103,8 +102,7 @@ __global__ void __launch_bounds__(SPLIT_BX *SPLIT_BY) e_update_split(StepArgs<T>
       wy_mz = w_at(a, 1, ii, jj, kk);
     }
   }
-  const int m = a.mat ? a.mat[n] : 0;
-  const T *mt = a.mtab + m * MAT_STRIDE;
+  const T *mt = mat_row(a, n);
   T *E = a.E;
   const T ex = E[n], ey = E[a.N + n], ez = E[2 * a.N + n];
 

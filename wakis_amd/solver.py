@@ -83,16 +83,18 @@ class _MaterialTable:
         sub = self.ids[region]
         if sub.size == 0:
             return
+        idt = self.ids.dtype
         lo, hi = int(sub.min()), int(sub.max())
         old = (np.array([lo]) if lo == hi else
-               lo + np.flatnonzero(np.bincount(sub.ravel(order="K") - np.uint16(lo))))
-        lut = np.arange(len(self.rows), dtype=np.uint16)
+               lo + np.flatnonzero(np.bincount((sub.ravel(order="K") - idt.type(lo))
+                                               .astype(np.int64))))
+        lut = np.arange(len(self.rows), dtype=idt)
         for o in old:
             nid = self.intern(fn(self.rows[int(o)]))
-            if nid > np.iinfo(np.uint16).max:
+            if nid > np.iinfo(idt).max:
                 raise ValueError("too many distinct materials")
             if nid >= len(lut):
-                lut = np.concatenate([lut, np.arange(len(lut), nid + 1, dtype=np.uint16)])
+                lut = np.concatenate([lut, np.arange(len(lut), nid + 1, dtype=idt)])
             lut[o] = nid
         self.ids[region] = lut[sub]
 
@@ -393,14 +395,18 @@ class SolverFIT3D(RoutinesMixin):
             self.dt = tau.min()
 
     def _upload_materials(self):
+        """Material ids and rows to the device: 8-bit ids and a table the
+        fused sweeps keep in LDS for up to 256 rows; beyond that (per-cell
+        tensors, e.g. a graded ieps given through update_tensors) int32 ids
+        with the table in global memory, stepped by the split kernels."""
         tab = self._mt.table()
-        if len(tab) > 256:
-            raise ValueError(f"{len(tab)} distinct materials; the device table holds 256")
         ids = self._mt.ids
         if len(tab) == 1:
             self.ctx.set_materials(None, tab)
-        else:
+        elif len(tab) <= 256:
             self.ctx.set_materials(np.reshape(ids.astype(np.uint8), -1, order="F"), tab)
+        else:
+            self.ctx.set_materials_wide(np.reshape(ids.astype(np.int32), -1, order="F"), tab)
 
     def _material_field(self, col):
         if self._mat_fields is None:
@@ -481,22 +487,35 @@ class SolverFIT3D(RoutinesMixin):
 
     def update_tensors(self, tensor="all"):
         """Re-derive the material ids from the (possibly edited) ieps, imu,
-        sigma Fields and upload them (wakis/solverFIT3D.py:378-436); like the
-        reference this also disables the first-step ghost zeroing."""
+        sigma Fields and upload them (wakis/solverFIT3D.py:378-436): tensor
+        "ieps", "imu" or "sigma" takes only that Field's values (the others
+        keep the values in use, as the reference rebuilds only that
+        diagonal), "all" takes all three; like the reference this also
+        disables the first-step ghost zeroing.  Any number of distinct rows
+        is accepted (above 256 the split kernels step the solver)."""
         if self._mat_fields is not None:
+            take = ("ieps", "imu", "sigma") if tensor == "all" else (tensor,)
+            cur_tab = self._mt.table()
+            cur_ids = np.reshape(self._mt.ids, -1, order="F")
             cols = []
-            for name in ("ieps", "imu", "sigma"):
-                a = np.asarray(self._mat_fields[name].toarray()).astype(self.dtype)
-                cols += [a[d * self.N:(d + 1) * self.N].astype(np.float64) for d in range(3)]
+            for c0, name in ((0, "ieps"), (3, "imu"), (6, "sigma")):
+                if name in take:
+                    a = np.asarray(self._mat_fields[name].toarray()).astype(self.dtype)
+                    cols += [a[d * self.N:(d + 1) * self.N].astype(np.float64)
+                             for d in range(3)]
+                else:
+                    cols += [cur_tab[cur_ids, c0 + d] for d in range(3)]
             stacked = np.stack(cols, axis=1)
             rows, inv = np.unique(stacked, axis=0, return_inverse=True)
             mt = _MaterialTable((self.Nx, self.Ny, self.Nz), rows[0], self.dtype)
             mt.rows = [tuple(r) for r in rows]
             mt.index = {r: i for i, r in enumerate(mt.rows)}
-            mt.ids = np.reshape(inv.reshape(-1).astype(np.uint16), (self.Nx, self.Ny, self.Nz),
+            idt = np.uint16 if len(rows) <= np.iinfo(np.uint16).max + 1 else np.int32
+            mt.ids = np.reshape(inv.reshape(-1).astype(idt), (self.Nx, self.Ny, self.Nz),
                                 order="F")
             self._mt = mt
-            self._mat_fields = None
+            if tensor == "all":
+                self._mat_fields = None
         self._upload_materials()
         self.ctx.set_dt(float(self.dt))
         self.step_0 = False

@@ -82,13 +82,94 @@ class WakeSolver:
         self.dx, self.dy, self.dz = meta["dx"], meta["dy"], meta["dz"]
         self.t = meta["t"]
 
+    # -- recorded histories (wakis/wakeSolver.py:1317-1364) --------------------
+    Ez_hf = None
+
+    def read_Ez(self, filename=None, return_value=False):
+        """Read a recorded E_z history (wakis/wakeSolver.py:1317-1364): the
+        reference's ``Ez.h5`` layout, one dataset per step keyed ``'#%05d'``
+        (a (2h+1, 2h+1, nz) window or an nz line) plus ``x, y, z, dx, dy,
+        dz, t``.  An ``.npz`` archive with those keys is read with numpy
+        (no pickles); an HDF5 file needs h5py.  Sets ``Ez_hf`` and the axis
+        attributes; the history is moved to the device when a wake is
+        computed."""
+        if filename is None:
+            filename = self.Ez_file
+        if str(filename).endswith(".npz"):
+            with np.load(filename, allow_pickle=False) as z:
+                hf = {k: np.array(z[k]) for k in z.files}
+        else:
+            try:
+                import h5py
+            except ImportError as e:
+                raise ImportError(
+                    f"reading {filename} needs h5py; an .npz archive with the same keys "
+                    "is read without it") from e
+            with h5py.File(filename, "r") as f:
+                hf = {k: np.array(f[k]) for k in f.keys()}
+        self.Ez_hf = hf
+        self.Ez_file = filename
+        self._axes_from(hf)
+        if return_value:
+            return hf
+
+    def _axes_from(self, hf):
+        for key, attr in (("x", "xf"), ("y", "yf"), ("z", "zf"), ("dx", "dx"), ("dy", "dy"),
+                          ("dz", "dz"), ("t", "t")):
+            if key in hf.keys():
+                setattr(self, attr, np.array(hf[key]))
+
+    def _load_host_history(self, device=0):
+        """Move the datasets of ``Ez_hf`` to a device history: the first
+        min(len(t), number of datasets) steps (the reference's nt), the
+        centred 3x3 window of a 3-D dataset (rows shape//2 - 1 .. shape//2 + 1,
+        as wakis/wakeSolver.py:492-498 indexes it) or the line of a 1-D one."""
+        from ._lib import Context
+
+        hf = self.Ez_hf
+        self._axes_from(hf)
+        keys = sorted(k for k in hf.keys() if isinstance(k, str) and k.startswith("#"))
+        if not keys:
+            raise KeyError("no '#%05d' datasets in the E_z history")
+        nt = min(len(self.t), len(keys))
+        first = np.asarray(hf[keys[0]])
+        if first.ndim == 3:
+            h = 1
+            a0, b0 = first.shape[0] // 2, first.shape[1] // 2
+            if a0 < 1 or b0 < 1 or a0 + 1 >= first.shape[0] or b0 + 1 >= first.shape[1]:
+                raise ValueError(f"dataset shape {first.shape}: needs a window of 3 or more")
+        elif first.ndim == 1:
+            h = 0
+        else:
+            raise ValueError(f"dataset shape {first.shape}: expected (a, b, nz) or (nz,)")
+        nz = first.shape[-1]
+        dtype = np.float32 if first.dtype == np.float32 else np.float64
+        nw = (2 * h + 1) ** 2
+        hist = np.empty((nw, nz, nt), dtype=dtype)
+        for n in range(nt):
+            e = np.asarray(hf[keys[n]])
+            if h == 1:
+                hist[:, :, n] = e[a0 - 1:a0 + 2, b0 - 1:b0 + 2, :].reshape(nw, nz)
+            else:
+                hist[0, :, n] = e
+        side = 2 * h + 1 if h else 2
+        ctx = Context(side, side, nz, dtype=dtype, device=device)
+        ctx.wake_history_load(h, h, h, hist)
+        self._ctx = ctx
+        self._meta = {"half": h, "nsteps": nt, "x": self.xf, "y": self.yf, "z": self.zf,
+                      "dx": self.dx, "dy": self.dy, "dz": self.dz, "t": self.t, "src": hf}
+
     def _need(self):
-        if self._ctx is None:
-            raise RuntimeError(
-                "no E_z history on the device: run SolverFIT3D.wakesolve first "
-                "(the Ez.h5 file path needs h5py, which is not used here)")
-        if self.counter_moving:
-            raise NotImplementedError("counter_moving wakes are not part of this build")
+        stale = (self.Ez_hf is not None and self._meta is not None
+                 and self._meta.get("src") is not self.Ez_hf)
+        if self._ctx is None or self._meta is None or stale:
+            if self.Ez_hf is None:
+                if self.Ez_file is None or not os.path.exists(self.Ez_file):
+                    raise RuntimeError(
+                        "no E_z history: run SolverFIT3D.wakesolve first, or give a recorded "
+                        "history (Ez_hf, or Ez_file / read_Ez with an .npz or .h5 file)")
+                self.read_Ez()
+            self._load_host_history()
 
     def solve(self, compute_plane=None, **kwargs):
         """wakis/wakeSolver.py:211-255."""
@@ -124,16 +205,24 @@ class WakeSolver:
         s = np.arange(-self.ti * self.v, wakelength, dt * self.v)
         return nt, dt, zmin, z, s
 
-    def bucket_columns(self, ncols_alloc):
+    def bucket_columns(self, ncols_alloc, counter=False):
         """cols[k, n]: the E_z time column the reference reads for (s_n, z_k),
         or -1.  it = int(ts/dt); it >= nt contributes nothing; a negative it
         indexes from the end of the reference's Ezt array (ncols_alloc
-        columns, only the first nt of which are filled)."""
+        columns, only the first nt of which are filled).  counter=True: the
+        counter-moving time of wakis/wakeSolver.py:500-506, where term k
+        samples z[-k-1] with the velocity negated and times counted from
+        zmax."""
         nt_t, dt, zmin, z, s = self._axes()
+        zmax = np.max(self.zf)
         nt = min(nt_t, int(self._meta["nsteps"]))
         cols = np.full((len(z), len(s)), -1, dtype=np.int32)
         for k in range(len(z)):
-            ts = (z[k] + s) / self.v - zmin / self.v - self.t[0] + self.ti
+            if counter:
+                ts = ((z[-k - 1] - s) / (-1 * self.v) - zmax / (-1 * self.v)
+                      - self.t[0] + self.ti)
+            else:
+                ts = (z[k] + s) / self.v - zmin / self.v - self.t[0] + self.ti
             it = np.trunc(ts / dt).astype(np.int64)
             c = np.where(it < 0, it + ncols_alloc, it)
             ok = (it < nt) & (c >= 0) & (c < nt)
@@ -148,9 +237,16 @@ class WakeSolver:
         self._need()
         if self._meta["half"] != 1:
             raise ValueError("calc_long_WP_3d needs the 3x3 probe window")
-        cols, s, z = self.bucket_columns(len(self.t))
-        wp = self._ctx.wake_potential(np.arange(9), True, cols, self.skip_cells, self.dz,
-                                      self.q * 1e12)
+        counter = bool(self.counter_moving)
+        cols, s, z = self.bucket_columns(len(self.t), counter=counter)
+        dz = np.asarray(self.dz, dtype=np.float64)[:len(z)]
+        if counter:
+            # wakis/wakeSolver.py:500-513: Ezt[-k-1, it] * (-1 * dz[k])
+            wp = self._ctx.wake_potential(np.arange(9), True, cols, self.skip_cells, -1 * dz,
+                                          self.q * 1e12, zrev=True)
+        else:
+            wp = self._ctx.wake_potential(np.arange(9), True, cols, self.skip_cells, dz,
+                                          self.q * 1e12)
         i0 = j0 = self.n_transverse_cells
         self.WP_3d = wp.reshape(2 * i0 + 1, 2 * j0 + 1, len(s))
         self.s = s
@@ -159,9 +255,13 @@ class WakeSolver:
 
     def calc_long_WP(self, Ezt=None, **kwargs):
         """wakis/wakeSolver.py:311-421 (one window; for a 3x3 history the
-        reference reads window index (2, 2))."""
+        reference reads window index (2, 2)).  ``Ezt`` is stored and, as in
+        the reference, the sum is taken over the recorded history; this
+        routine has no counter-moving branch."""
         for key, val in kwargs.items():
             setattr(self, key, val)
+        if Ezt is not None:
+            self.Ezt = Ezt
         self._need()
         if self._meta["half"] == 0 and self.skip_cells != 0:
             # the reference copies the whole E_z line into an array of the
@@ -240,8 +340,12 @@ class WakeSolver:
         return ds, N, K, ffft[mask]
 
     def _ctx_for_dft(self):
+        """The device context the DFT runs on: the one holding the history,
+        else a small context of its own (results read from files)."""
         if self._ctx is None:
-            raise RuntimeError("no device context: run SolverFIT3D.wakesolve first")
+            from ._lib import Context
+
+            self._ctx = Context(2, 2, 2, dtype=np.float64)
         return self._ctx
 
     def calc_long_Z(self, samples=1001, fmax=None, **kwargs):
