@@ -39,6 +39,87 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def cpu_info():
+    """Host CPU model, the logical CPUs the OS reports (nproc) and those this
+    process may run on."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "usable_cpus": usable}
+
+
+def _oracle_for(spec):
+    """Oracle solver (FP64 scipy.sparse) of a CPU-baseline sample: a z-slab
+    of the pillbox grid, or one cavity of the batch workload."""
+    from oracle.fit_oracle import OGrid, OSolver
+    from wakis_amd.workloads import cavity_batch, lossy_pillbox
+
+    if spec["kind"] == "pillbox_slab":
+        # the reference's geomspace PML grading (the oracle restates the
+        # reference; the grading changes sigma values only, not the work)
+        w = lossy_pillbox(256, 256, spec["nz"], pml_profile="geomspace")
+    else:
+        w = cavity_batch(n=64, seed=3)[spec["index"]]
+    g, sv = w["grid"], w["solver"]
+    og = OGrid(x=g["x"], y=g["y"], z=g["z"])
+    return OSolver(og, cfln=sv["cfln"], bc_low=sv["bc_low"], bc_high=sv["bc_high"],
+                   solids=g["solids"], bg=sv["bg"], n_pml=sv["n_pml"], dtype=np.float64)
+
+
+def cpu_worker(spec):
+    """One CPU-baseline process: build the sample's oracle, report ready,
+    wait for the start line on stdin, time spec["steps"] steps."""
+    osv = _oracle_for(spec)
+    osv.one_step()
+    print(json.dumps({"ready": True}), flush=True)
+    sys.stdin.readline()
+    t0 = time.perf_counter()
+    for _ in range(spec["steps"]):
+        osv.one_step()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"cells": osv.N, "steps": spec["steps"], "seconds": dt}), flush=True)
+
+
+def cpu_parallel(specs):
+    """len(specs) oracle processes started together (each builds its sample,
+    then all start stepping at once); returns the sum of their cell-step
+    rates and the per-worker results.  The workers touch no GPU."""
+    import subprocess
+
+    procs = []
+    for spec in specs:
+        cmd = [sys.executable, "-u", os.path.abspath(__file__), "--workload", "cpu-worker",
+               "--cpu-spec", json.dumps(spec)]
+        env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1",
+                   MKL_NUM_THREADS="1")
+        procs.append(subprocess.Popen(cmd, stdin=subprocess.PIPE, stdout=subprocess.PIPE,
+                                      env=env, text=True))
+    try:
+        for p in procs:
+            line = p.stdout.readline()
+            if not line or not json.loads(line).get("ready"):
+                raise RuntimeError("cpu worker failed to start")
+        for p in procs:
+            p.stdin.write("go\n")
+            p.stdin.flush()
+        res = [json.loads(p.stdout.readline()) for p in procs]
+        for p in procs:
+            p.wait(timeout=120)
+    finally:
+        for p in procs:
+            if p.poll() is None:
+                p.kill()
+    rate = sum(r["cells"] * r["steps"] / r["seconds"] for r in res)
+    return rate, res
+
+
+def cpu_procs():
+    """Processes for the multi-process CPU figure: the usable CPUs, at most 16
+    (the host share of one GPU)."""
+    return max(1, min(cpu_info()["usable_cpus"], 16))
+
+
 def traffic_from_profiles(kernel, dtype, grid, tile):
     """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE and
     WRITE_SIZE (each scaled by what the counter reads for a copy kernel of
@@ -81,7 +162,9 @@ def cpu_baseline(nz_sample=32, steps=150):
         "cores": 1,
         "kind": "port",
         "sample": f"oracle FP64 scipy.sparse one_step, 256x256x{nz_sample} z-slab of the pillbox "
-                  f"grid, {steps} steps, {dt:.2f} s on 1 core of {cpu_model()}",
+                  f"grid, {steps} steps, {dt:.2f} s on 1 core of {cpu_model()} (scipy's "
+                  "sparse products and numpy's vector operations are single-threaded)",
+        **cpu_info(),
     }
 
 
@@ -481,6 +564,7 @@ def bench_batch(args):
                 f"fused_batch<float, {dims[0]}, {dims[1]}, {dims[2]}>", "f32", "batch64", cfg)
     achieved = alg / (launch_ms * 1e-3) / 1e9
     sizes = sorted(s.N for s in solvers)
+    out_cpu = None if args.no_cpu else cpu_baseline_batch(solvers)
     return {
         "metric": "Gcell-steps/s",
         "value": round(value, 3),
@@ -512,6 +596,86 @@ def bench_batch(args):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_source": f"profiles/{tsrc}" if tsrc else None,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": round(launch_ms, 5)},
+        **({"cpu_baseline": out_cpu} if out_cpu else {}),
+    }
+
+
+def cpu_baseline_batch(solvers, target=2.5e8):
+    """The batch's cavities on the CPU oracle (FP64 scipy.sparse): one
+    process alone on the median cavity, and P = cpu_procs() processes at once
+    on P cavities spread over the size range, each timing about `target`
+    cell-steps; value = the P processes' summed cell-step rate."""
+    order = np.argsort([s.N for s in solvers])
+    med = int(order[len(order) // 2])
+    nsteps = lambda i: max(5, int(round(target / solvers[i].N)))
+    r1, res1 = cpu_parallel([{"kind": "cavity", "index": med, "steps": nsteps(med)}])
+    P = cpu_procs()
+    idx = [int(order[(q * len(order)) // P]) for q in range(P)]
+    rp, resp = cpu_parallel([{"kind": "cavity", "index": i, "steps": nsteps(i)} for i in idx])
+    return {
+        "value": rp / 1e9,
+        "unit": "Gcell-steps/s",
+        "cores": P,
+        "kind": "port",
+        "single_process_gcells": r1 / 1e9,
+        "sample": f"oracle FP64 scipy.sparse one_step on the batch's cavities: {P} processes at "
+                  f"once on cavities {idx} (about {target:.1e} cell-steps each, "
+                  f"{max(r['seconds'] for r in resp):.1f} s), summed rate; one process alone "
+                  f"on the median cavity {med}: {r1 / 1e9:.4f} Gcell-steps/s",
+        **cpu_info(),
+    }
+
+
+def fifo_servers(arrivals, service, servers):
+    """Completion times of first-come-first-served requests on `servers`
+    identical servers."""
+    import heapq
+
+    free = [0.0] * servers
+    heapq.heapify(free)
+    fin = np.empty(len(arrivals))
+    for q in np.argsort(arrivals, kind="stable"):
+        t = max(float(arrivals[q]), heapq.heappop(free))
+        fin[q] = t + float(service[q])
+        heapq.heappush(free, fin[q])
+    return fin
+
+
+def cpu_baseline_serve(arrivals, cells, nts, nz_sample=32, target=2.5e8):
+    """Solves/hour of the CPU oracle on the same arrival trace, extrapolated
+    from measured stepping rates: one process (one request at a time) and
+    P = cpu_procs() processes (P requests at a time, first come first
+    served), each request's service time cells x Nt / (per-process rate).
+    The rates are measured on 256 x 256 x nz_sample slabs of the pillbox
+    grid (FP64 oracle, about `target` cell-steps per process); whole
+    requests (1e6-1.2e8 cells, hours each on one core) are not run."""
+    spec = {"kind": "pillbox_slab", "nz": nz_sample,
+            "steps": max(5, int(round(target / (256 * 256 * nz_sample))))}
+    r1, _ = cpu_parallel([spec])
+    P = cpu_procs()
+    rp, resp = cpu_parallel([spec] * P)
+    work = np.asarray(cells, dtype=np.float64) * np.asarray(nts, dtype=np.float64)
+    arr = np.asarray(arrivals, dtype=np.float64)
+    out = {}
+    for name, servers, rate in (("one_process", 1, r1), ("p_processes", P, rp / P)):
+        fin = fifo_servers(arr, work / rate, servers)
+        lat = fin - arr
+        out[name] = {"solves_per_h": 3600.0 * len(arr) / (fin.max() - arr.min()),
+                     "p50_s": float(np.percentile(lat, 50)),
+                     "p99_s": float(np.percentile(lat, 99)),
+                     "gcell_steps_per_s": (rate * servers) / 1e9}
+    return {
+        "value": out["p_processes"]["solves_per_h"],
+        "unit": "solves/h",
+        "cores": P,
+        "kind": "port",
+        "one_process": out["one_process"],
+        "p_processes": out["p_processes"],
+        "sample": f"extrapolated: oracle FP64 scipy.sparse stepping rates measured on "
+                  f"256x256x{nz_sample} pillbox slabs ({spec['steps']} steps; 1 process "
+                  f"{r1 / 1e9:.4f}, {P} processes at once {rp / 1e9:.4f} Gcell-steps/s summed), "
+                  f"applied to each request's cells x Nt on the same arrivals, FIFO",
+        **cpu_info(),
     }
 
 
@@ -597,6 +761,7 @@ def bench_serve(args):
                             f"E_z window D2H per step; {args.plain_steps} steps timed per request",
         }
     cells = np.array([r.cells for r in done])
+    out_cpu = None if args.no_cpu else cpu_baseline_serve(arr, cells, nts)
     return {
         "metric": "solves/h",
         "value": round(served_per_h, 1),
@@ -626,6 +791,7 @@ def bench_serve(args):
             "device_busy_frac": round(busy_ms / 1e3 / wall, 3),
             "chunk_steps": args.chunk,
         },
+        **({"cpu_baseline": out_cpu} if out_cpu else {}),
     }
 
 
@@ -827,9 +993,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--tile", type=int, default=None, help="fused tile configuration id")
+    ap.add_argument("--tile2", type=int, default=None,
+                    help="two-step tile shape (0: 64x8, 1: 32x16, 2: 32x8, 3: 16x16)")
     ap.add_argument("--wakelength", type=float, default=None, help="precision: wakelength (m)")
     ap.add_argument("--workload", default="pillbox",
-                    choices=["pillbox", "batch", "serve", "precision", "large", "plain", "cubic"],
+                    choices=["pillbox", "batch", "serve", "precision", "large", "plain", "cubic",
+                             "cpu-worker"],
                     help="pillbox: one 256x256x768 solve (the headline); batch: 64 cavities in "
                          "one launch per step; serve: Poisson impedance requests; large: "
                          "the 1024^3 stratified box, FP32 vs FP64; cubic: the PEC cubic "
@@ -852,7 +1021,11 @@ def main():
                     help="pillbox: also time the plain (sparse-operator) FP64 GPU method")
     ap.add_argument("--plain-target", default="serve", choices=["serve", "pillbox"],
                     help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-spec", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.workload == "cpu-worker":
+        cpu_worker(json.loads(args.cpu_spec))
+        return
     if args.steps is None:
         args.steps = 2000 if args.workload == "large" else 1000
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -888,6 +1061,8 @@ def main():
     grid, wake, solver = build(wl, dtype=dtype, kernel=args.kernel)
     if args.tile is not None:
         solver.ctx.set_tile(args.tile)
+    if args.tile2 is not None:
+        solver.ctx.set_tile2(args.tile2)
     setup_s = time.perf_counter() - t0
     ctx = solver.ctx
     N = solver.N
@@ -906,12 +1081,16 @@ def main():
     solver.step_0 = False
     cond = solver.use_conductivity
 
-    tune_ms = None
+    tune_ms = tune2_ms = None
     if not args.no_tune and args.tile is None and solver.ctx.kernel_in_use() == 2:
         tune_ms = solver.tune(reps=3)
+    if not args.no_tune and args.tile2 is None and solver.ctx.kernel_in_use() == 3:
+        tune2_ms = [round(v, 4) for v in ctx.tune2(reps=3)]
     tile = ctx.get_tile()
+    tile2 = ctx.get_tile2()
     print(f"[bench] {wl['name']} {args.dtype} N={N} setup {setup_s:.1f}s warmup {args.warmup} "
-          f"tile {tile} tune {tune_ms}", file=sys.stderr, flush=True)
+          f"tile {tile} tune {tune_ms} tile2 {tile2} tune2 {tune2_ms}", file=sys.stderr,
+          flush=True)
     ctx.step(args.warmup, cond)
     ctx.sync()
     # one timed window: HIP events around every launch of exactly --steps
@@ -939,8 +1118,8 @@ def main():
         # algorithmically (the per-cell-step figure), while the launch itself
         # reads and writes E and H once (12 words per cell)
         kname, kms, alg_bytes = "fused2_sweep", k1 / n_two, 2 * 12 * esz * N
-        traffic, tsrc = traffic_from_profiles(f"fused2_sweep<{ctype}, 64, 8>", args.dtype,
-                                              f"{nx}x{ny}x{nz}", "f2")
+        traffic, tsrc = traffic_from_profiles(f"fused2_sweep<{ctype}, {tile2[1]}, {tile2[2]}>",
+                                              args.dtype, f"{nx}x{ny}x{nz}", "f2")
         extra = {"traffic_source": f"profiles/{tsrc}" if tsrc else None,
                  "steps_per_launch": 2,
                  "launch_min_hbm_bytes": 12 * esz * N,
@@ -995,7 +1174,10 @@ def main():
                         "(beam + E_z probe + step)",
             "grid": [nx, ny, nz],
             "kernel": "fused2" if two else ("fused" if fused else "split"),
-            "tile": {"cfg": tile[0], "tx": tile[1], "ty": tile[2], "kc": tile[3]} if fused else None,
+            "tile": ({"two_step_cfg": tile2[0], "tx": tile2[1], "ty": tile2[2],
+                      "tune2_ms": tune2_ms} if two else
+                     {"cfg": tile[0], "tx": tile[1], "ty": tile[2], "kc": tile[3]} if fused
+                     else None),
             "step_alg_gbs": round(step_alg, 1),
             "step_hbm_frac": round(step_alg / HBM_PEAK_GBS, 4),
             "copy_kernel_gbs": round(copy_gbs, 1),

@@ -198,7 +198,7 @@ int fit_monitor_read(fit_ctx *ctx, int32_t f, int32_t comp, double *host);
  * conductive != 0 selects J := sigma*E after each step.  Asynchronous.
  * With FIT_KERNEL_AUTO or FIT_KERNEL_FUSED2, pairs of steps run as ONE
  * two-step sweep (E and H read and written once per two steps) on FP32
- * contexts with nx >= 64 and ny >= 16 (fields below 2^32 bytes), when J is
+ * contexts with nx >= 16 and ny >= 16 (fields below 2^32 bytes), when J is
  * 0 or sigma*E
  * (not FIT_J_BUFFER) and no field-monitor update is staged for the pair;
  * every other step runs the one-step sweep.  Each step's beam value, J and
@@ -260,6 +260,28 @@ int fit_get_tile(fit_ctx *ctx, int32_t *cfg, int32_t *tx, int32_t *ty, int32_t *
  * context's grid and data (reps launches each; the state is not changed)
  * and select the fastest; ms[FIT_NTILE] receives the average times. */
 int fit_tune(fit_ctx *ctx, int32_t reps, double *ms);
+/* Tile shape of the two-step sweep (no reference counterpart: a launch
+ * choice of this library): FIT_TILE2_* columns per workgroup, -1 = the
+ * default (the shape of least modelled cost: window cells per tile cell,
+ * tile overhang over the grid and measured time per window cell).  A shape
+ * the grid cannot take (nx < tile width, ny < max(16, tile height)) falls
+ * back to the default.  Every shape evaluates the same per-cell
+ * expressions.  fit_get_tile2
+ * reports the shape the next two-step sweep will use (-1: none, 0 x 0);
+ * fit_tune2 times every shape that applies on this context's grid and data
+ * (reps two-step launches each; the state is not changed), selects the
+ * fastest and writes the average times to ms[FIT_NTILE2] (-1 where a shape
+ * does not apply). */
+enum {
+  FIT_TILE2_64x8 = 0,
+  FIT_TILE2_32x16 = 1,
+  FIT_TILE2_32x8 = 2,
+  FIT_TILE2_16x16 = 3,
+  FIT_NTILE2 = 4
+};
+int fit_set_tile2(fit_ctx *ctx, int32_t cfg);
+int fit_get_tile2(fit_ctx *ctx, int32_t *cfg, int32_t *tx, int32_t *ty);
+int fit_tune2(fit_ctx *ctx, int32_t reps, double *ms);
 /* The step kernel the next fit_step will use (FIT_KERNEL_SPLIT or
  * FIT_KERNEL_FUSED; AUTO resolves to FUSED when its second field buffers
  * fit in device memory). */

@@ -1,6 +1,7 @@
 """The two-step fused sweep (two leapfrog steps per launch, FP32; used by
-fit_step for pairs of steps on grids with nx >= 64 and ny >= 16) against the FP32 oracle (relative L2 <= 1e-4, the FP32 bar of the other
-parity tests) and against the one-step fused sweep on the same context setup
+fit_step for pairs of steps on grids with nx >= 24 and ny >= 16, in tiles of
+64 x 8, 32 x 16 or 24 x 24 columns) against the FP32 oracle (relative L2
+<= 1e-4, the FP32 bar of the other parity tests) and against the one-step fused sweep on the same context setup
 (<= 1e-5: the two kernels evaluate the same per-cell expressions).  Cases:
 every boundary kind, the helical x/y wrap across tile and grid edges, odd
 step counts (a single step after the pairs), odd plane counts, non-uniform
@@ -114,3 +115,69 @@ def test_two_step_wakesolve_matches_one_step():
     print(errs, flush=True)
     assert max(errs["E"], errs["H"], errs["WP"], errs["Z"]) <= TOL_KERNEL, errs
     assert errs["E_oracle"] <= TOL_ORACLE and errs["WP_oracle"] <= TOL_ORACLE, errs
+
+
+SHAPE_CASES = [
+    # tiles overhanging the high x and y edges for every shape
+    ("periodic", (72, 40, 30), ["periodic"] * 3, ["periodic"] * 3, False, False),
+    # narrower than 64 cells: 64 x 8 does not apply
+    ("mixed_narrow_lean", (50, 24, 41), ["periodic", "pec", "pec"],
+     ["periodic", "pmc", "pml"], True, True),
+    ("pml_conductive", (64, 32, 44), ["pec", "pec", "pml"], ["pec", "pec", "pml"], True, False),
+]
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3], ids=["64x8", "32x16", "32x8", "16x16"])
+@pytest.mark.parametrize("name,shape,bl,bh,solids,nonuniform", SHAPE_CASES,
+                         ids=[c[0] for c in SHAPE_CASES])
+def test_two_step_tile_shapes(cfg, name, shape, bl, bh, solids, nonuniform):
+    """Every tile shape of the single-grid two-step sweep (fit_set_tile2)
+    against the FP32 oracle and the one-step sweep; a shape the grid cannot
+    take falls back to the default shape."""
+    from wakis_amd import _lib
+
+    nsteps = 9
+    print(f"[case] two-step shape {cfg} {name} {shape}", flush=True)
+    osv, p2 = _pair(shape, bl, bh, solids, nonuniform, "fused2")
+    _, p1 = _pair(shape, bl, bh, solids, nonuniform, "fused")
+    p2.ctx.set_tile2(cfg)
+    tx = (64, 32, 32, 16)[cfg]
+    if shape[0] >= tx:
+        assert p2.ctx.get_tile2()[0] == cfg
+    else:  # falls back to the modelled default
+        assert p2.ctx.get_tile2()[0] not in (-1, cfg)
+    assert p2.ctx.kernel_in_use() == _lib.KERNEL_FUSED2
+    for _ in range(nsteps):
+        osv.one_step()
+    p2._advance(nsteps)
+    p1._advance(nsteps)
+    assert p2.ctx.launch_counts() == (1, 4)
+    errs = {}
+    for f in "EHJ":
+        o = getattr(osv, f).toarray()
+        a2 = getattr(p2, f).toarray()
+        a1 = getattr(p1, f).toarray()
+        errs[f] = (rel(a2, o), rel(a2, a1))
+    print(errs, flush=True)
+    for f, (eo, ek) in errs.items():
+        assert eo <= TOL_ORACLE and ek <= TOL_KERNEL, (f, eo, ek)
+
+
+def test_two_step_tuner_keeps_state():
+    """fit_tune2 times the shapes without changing the fields; the tuned run
+    agrees with an untuned one (the shapes evaluate the same per-cell
+    expressions; chunk boundaries, and with them the planes the lean path
+    takes, may differ)."""
+    shape = (72, 40, 30)
+    print("[case] two-step tuner", flush=True)
+    _, pa = _pair(shape, ["pec"] * 3, ["pec"] * 3, True, False, "fused2")
+    _, pb = _pair(shape, ["pec"] * 3, ["pec"] * 3, True, False, "fused2")
+    e0 = pa.E.toarray().copy()
+    ms = pa.ctx.tune2(reps=2)
+    assert np.array_equal(pa.E.toarray(), e0)
+    assert (ms > 0).all(), ms
+    assert pa.ctx.get_tile2()[0] == int(np.argmin(ms))
+    pa._advance(6)
+    pb._advance(6)
+    for f in "EH":
+        assert rel(getattr(pa, f).toarray(), getattr(pb, f).toarray()) <= TOL_KERNEL, f

@@ -20,6 +20,7 @@ J_ZERO, J_SIGMA_E, J_BUFFER = 0, 1, 2
 BEAM_OFF, BEAM_VALUES, BEAM_GAUSS = 0, 1, 2
 KERNEL_AUTO, KERNEL_SPLIT, KERNEL_FUSED, KERNEL_FUSED2 = 0, 1, 2, 3
 NTILE = 12
+NTILE2 = 4  # FIT_NTILE2: two-step tile shapes 64x8, 32x16, 32x8, 16x16
 
 
 class FitError(RuntimeError):
@@ -86,6 +87,9 @@ SIGNATURES = {
     "fit_set_chunk_rsrc": [_P, _I32],
     "fit_get_tile": [_P, _PI32, _PI32, _PI32, _PI32],
     "fit_tune": [_P, _I32, _PD],
+    "fit_set_tile2": [_P, _I32],
+    "fit_get_tile2": [_P, _PI32, _PI32, _PI32],
+    "fit_tune2": [_P, _I32, _PD],
     "fit_sync": [_P],
     "fit_wake_potential": [_P, _I32, _PI32, _I32, _I32, _I32, _I32, _I32, _PI32, _PD, _D,
                            _PD],
@@ -353,6 +357,25 @@ class Context:
         the per-configuration average ms."""
         out = np.zeros(NTILE, dtype=np.float64)
         check(self.lib.fit_tune(self.h, int(reps), dptr(out)), "fit_tune")
+        return out
+
+    def set_tile2(self, cfg: int):
+        """Tile shape of the two-step sweep (0: 64x8, 1: 32x16, 2: 32x8,
+        3: 16x16, -1: the modelled default)."""
+        check(self.lib.fit_set_tile2(self.h, int(cfg)), "fit_set_tile2")
+
+    def get_tile2(self):
+        """(cfg, tx, ty) of the next two-step sweep; cfg -1 when none applies."""
+        c, x, y = C.c_int32(), C.c_int32(), C.c_int32()
+        check(self.lib.fit_get_tile2(self.h, C.byref(c), C.byref(x), C.byref(y)),
+              "fit_get_tile2")
+        return c.value, x.value, y.value
+
+    def tune2(self, reps: int = 3):
+        """Time every two-step tile shape that applies, select the fastest;
+        returns the per-shape average ms (-1 where a shape does not apply)."""
+        out = np.zeros(NTILE2, dtype=np.float64)
+        check(self.lib.fit_tune2(self.h, int(reps), dptr(out)), "fit_tune2")
         return out
 
     def sync(self):

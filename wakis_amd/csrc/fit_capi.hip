@@ -74,6 +74,9 @@ struct fit_ctx {
   int kernel = FIT_KERNEL_AUTO;
   bool chunk_rsrc = false;  // fit_set_chunk_rsrc
   int fused_cfg = -1;  // tile configuration of the fused sweep (-1: default)
+  int f2_cfg = -1;     // tile shape of the two-step sweep (fit_set_tile2; -1: default)
+  int f2_def = -1;     // the modelled default shape, computed for f2_def_nmat rows
+  int f2_def_nmat = -1;
   void *btab = nullptr;  // beam line table of the steps of one fit_step call
   int64_t btab_steps = 0;
   // field monitor: complex128 accumulators (3 * mon_nf * N) and the staged
@@ -272,6 +275,21 @@ void monitor_launch(fit_ctx *c, const double *phase) {
                      c->stream, (const T *)c->E, c->N, c->mon_nf, phase, (double2 *)c->mon);
 }
 
+// Tile shape of the two-step sweep for these step arguments: the one set by
+// fit_set_tile2 when it applies, else the default shape (-1: none applies).
+template <typename T>
+int f2_cfg_of(fit_ctx *c, const fit::StepArgs<T> &a) {
+  if (c->f2_cfg >= 0 && fit::fused2_supported<T>(a, c->f2_cfg)) return c->f2_cfg;
+  if (c->f2_def >= 0 && c->f2_def_nmat == c->nmat && fit::fused2_supported<T>(a, c->f2_def))
+    return c->f2_def;
+  const int d = fit::fused2_default_cfg<T>(a);
+  if (d >= 0) {
+    c->f2_def = d;
+    c->f2_def_nmat = c->nmat;
+  }
+  return d;
+}
+
 // ev (optional): 3 events per step [before first kernel, between, after].
 template <typename T>
 int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) {
@@ -301,8 +319,10 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
   const bool pairs = fused && (c->kernel == FIT_KERNEL_AUTO || c->kernel == FIT_KERNEL_FUSED2);
   for (int s = 0; s < nsteps;) {
     fit::StepArgs<T> a = make_args<T>(c);
-    if (pairs && s + 1 < nsteps && !(c->mon && s < c->mon_ph_steps) &&
-        fit::fused2_supported<T>(a)) {
+    const int f2 = pairs && s + 1 < nsteps && !(c->mon && s < c->mon_ph_steps)
+                       ? f2_cfg_of<T>(c, a)
+                       : -1;
+    if (f2 >= 0) {
       const T *jl = (const T *)c->jline;
       const T *beam1 = nullptr, *beam2 = nullptr;
       if (gauss) {
@@ -315,7 +335,7 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
       const bool js1 = c->jmode == FIT_J_SIGMA_E;
       const bool js2 = conductive ? true : js1;
       if (ev) HIPCHK(hipEventRecord(ev[3 * s], c->stream));
-      if (!fit::launch_fused2<T>(a, (T *)c->E_alt, (T *)c->H_alt, beam1, beam2, js1, js2,
+      if (!fit::launch_fused2<T>(a, f2, (T *)c->E_alt, (T *)c->H_alt, beam1, beam2, js1, js2,
                                  c->stream))
         return fail(FIT_ESTATE, "two-step sweep not launched");
       std::swap(c->E, c->E_alt);
@@ -428,6 +448,49 @@ int tune_impl(fit_ctx *c, int reps, double *ms_out) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (!rc) c->fused_cfg = best;
+  return rc;
+}
+
+// Times each tile shape of the two-step sweep (reps launches, written into
+// the second buffers without swapping, no probe and no beam: the state is
+// unchanged) and keeps the fastest.
+template <typename T>
+int tune2_impl(fit_ctx *c, int reps, double *ms_out) {
+  fit::StepArgs<T> a = make_args<T>(c);
+  a.hist = nullptr;
+  const bool js = c->jmode == FIT_J_SIGMA_E;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int best = -1;
+  double best_ms = 1e300;
+  int rc = 0;
+  for (int cfg = 0; cfg < fit::F2_NCFG && !rc; ++cfg) {
+    if (!fit::fused2_supported<T>(a, cfg)) {
+      if (ms_out) ms_out[cfg] = -1.0;
+      continue;
+    }
+    fit::launch_fused2<T>(a, cfg, (T *)c->E_alt, (T *)c->H_alt, nullptr, nullptr, js, js,
+                          c->stream);
+    if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(FIT_EHIP, "event");
+    for (int r = 0; r < reps && !rc; ++r)
+      fit::launch_fused2<T>(a, cfg, (T *)c->E_alt, (T *)c->H_alt, nullptr, nullptr, js, js,
+                            c->stream);
+    float t = 0;
+    if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+      rc = fail(FIT_EHIP, "tuning timing");
+    if (hipGetLastError() != hipSuccess && !rc) rc = fail(FIT_EHIP, "tuning launch");
+    const double ms = (double)t / reps;
+    if (ms_out) ms_out[cfg] = ms;
+    if (!rc && ms < best_ms) {
+      best_ms = ms;
+      best = cfg;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!rc && best >= 0) c->f2_cfg = best;
   return rc;
 }
 
@@ -1176,6 +1239,38 @@ int fit_tune(fit_ctx *c, int32_t reps, double *ms) {
   if (!fused) return fail(FIT_ESTATE, "the fused sweep is not in use");
   if (int rc = set_device(c)) return rc;
   return c->dtype == FIT_F64 ? tune_impl<double>(c, reps, ms) : tune_impl<float>(c, reps, ms);
+}
+
+int fit_set_tile2(fit_ctx *c, int32_t cfg) {
+  if (!c) return fail(FIT_EINVAL, "null context");
+  if (cfg < -1 || cfg >= fit::F2_NCFG) return fail(FIT_EINVAL, "bad two-step tile shape");
+  c->f2_cfg = cfg;
+  return 0;
+}
+
+int fit_get_tile2(fit_ctx *c, int32_t *cfg, int32_t *tx, int32_t *ty) {
+  if (!c || !cfg) return fail(FIT_EINVAL, "null argument");
+  *cfg = -1;
+  if (c->dtype == FIT_F32 && c->geo_set && c->mat_set && c->dt_set) {
+    if (int rc = set_device(c)) return rc;
+    *cfg = f2_cfg_of<float>(c, make_args<float>(c));
+  }
+  int a = 0, b = 0;
+  if (*cfg >= 0) fit::f2_dims(*cfg, a, b);
+  if (tx) *tx = a;
+  if (ty) *ty = b;
+  return 0;
+}
+
+int fit_tune2(fit_ctx *c, int32_t reps, double *ms) {
+  if (int rc = check_ready(c)) return rc;
+  if (reps < 1) return fail(FIT_EINVAL, "reps < 1");
+  if (c->dtype != FIT_F32) return fail(FIT_ESTATE, "the two-step sweep is FP32 only");
+  bool fused = false;
+  if (int rc = want_fused(c, &fused)) return rc;
+  if (!fused) return fail(FIT_ESTATE, "the fused sweeps are not in use");
+  if (int rc = set_device(c)) return rc;
+  return tune2_impl<float>(c, reps, ms);
 }
 
 int fit_kernel_in_use(fit_ctx *c, int32_t *kernel) {

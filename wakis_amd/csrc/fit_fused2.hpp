@@ -748,7 +748,7 @@ constexpr bool cfg2_fits() {
 // round lasts about a chunk plus its 3 planes of pipeline fill; the least
 // modelled time rounds x (kc + 3) among a few lengths.
 inline int pick_kc2(int ncol, int nz, int slots) {
-  static const int cand[] = {32, 48, 64, 96, 128, 192, 256, 384};
+  static const int cand[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384};
   int best = 64;
   double bt = 1e300;
   const int xslots = slots / 8 > 0 ? slots / 8 : 1;
@@ -910,23 +910,57 @@ inline void chunk2_layout(Fused2Args<T> &f, int kc) {
   f.nb = ncol * f.nchunk;
 }
 
-// Whether the two-step sweep serves these step arguments: FP32, nx >= 64 and
-// ny >= 16 (a window cell's flat carry then stays within one row and one
-// plane), fields of less than 2^32 bytes, J zero or sigma*E (no J buffer),
-// the kernels spill-free.
-template <typename T>
-inline bool fused2_supported(const StepArgs<T> &a) {
-  if constexpr (!cfg2_fits<T, F2_TX, F2_TY>()) {
+// Single-grid tile shapes of the two-step sweep: 64 x 8 columns (a 68 x 12
+// window, 816 threads), 32 x 16 (36 x 20, 720), 32 x 8 (36 x 12, 432) or
+// 16 x 16 (20 x 20, 400).  64 x 8 has the longest contiguous rows and the
+// least time per window cell; the others leave less overhang on grids whose
+// width is not a multiple of 64, and the two small ones give more blocks
+// (and two resident per CU) on small grids.  (24 x 24 and 28 x 12 were
+// slower than one of these on every grid measured, 48^2 to 512^2 cells.)
+enum { F2_CFG_64x8 = 0, F2_CFG_32x16 = 1, F2_CFG_32x8 = 2, F2_CFG_16x16 = 3, F2_NCFG = 4 };
+
+inline void f2_dims(int cfg, int &tx, int &ty) {
+  static const int d[F2_NCFG][2] = {{64, 8}, {32, 16}, {32, 8}, {16, 16}};
+  tx = d[cfg][0];
+  ty = d[cfg][1];
+}
+
+// Whether the two-step sweep with tiles TX x TY serves these step
+// arguments: FP32, nx >= TX and ny >= max(16, TY) (a window cell's flat
+// carry then stays within one row and one plane), fields of less than 2^32
+// bytes, J zero or sigma*E (no J buffer), the kernels spill-free.
+template <typename T, int TX, int TY>
+inline bool fused2_supported_t(const StepArgs<T> &a) {
+  if constexpr (!cfg2_fits<T, TX, TY>()) {
     (void)a;
     return false;
   } else {
-    return a.jmode != J_BUFFER && a.buf32 && a.nx >= F2_TX && a.ny >= 16 && a.ny >= F2_TY &&
+    return a.jmode != J_BUFFER && a.buf32 && a.nx >= TX && a.ny >= 16 && a.ny >= TY &&
            a.nz >= 2 &&
-           kernel_spill_free((const void *)fused2_sweep<T, F2_TX, F2_TY, false>) &&
-           kernel_spill_free((const void *)fused2_sweep<T, F2_TX, F2_TY, true>) &&
-           kernel_spill_free((const void *)fused2_sweep_faces<T, F2_TX, F2_TY, false>) &&
-           kernel_spill_free((const void *)fused2_sweep_faces<T, F2_TX, F2_TY, true>);
+           kernel_spill_free((const void *)fused2_sweep<T, TX, TY, false>) &&
+           kernel_spill_free((const void *)fused2_sweep<T, TX, TY, true>) &&
+           kernel_spill_free((const void *)fused2_sweep_faces<T, TX, TY, false>) &&
+           kernel_spill_free((const void *)fused2_sweep_faces<T, TX, TY, true>);
   }
+}
+
+template <typename T>
+inline bool fused2_supported(const StepArgs<T> &a, int cfg) {
+  switch (cfg) {
+    case F2_CFG_64x8: return fused2_supported_t<T, 64, 8>(a);
+    case F2_CFG_32x16: return fused2_supported_t<T, 32, 16>(a);
+    case F2_CFG_32x8: return fused2_supported_t<T, 32, 8>(a);
+    case F2_CFG_16x16: return fused2_supported_t<T, 16, 16>(a);
+    default: return false;
+  }
+}
+
+// Whether any tile shape of the two-step sweep serves these arguments.
+template <typename T>
+inline bool fused2_supported(const StepArgs<T> &a) {
+  for (int c = 0; c < F2_NCFG; ++c)
+    if (fused2_supported<T>(a, c)) return true;
+  return false;
 }
 
 // Whether a batch member's step arguments suit the batched two-step sweep
@@ -948,16 +982,17 @@ inline bool fused2_batch_supported(const StepArgs<T> &a, int cfg) {
                               : fused2_batch_supported_t<T, 32, 16>(a);
 }
 
-// Launches two steps; false (nothing launched) when not supported.
-template <typename T>
-inline bool launch_fused2(const StepArgs<T> &a, T *En, T *Hn, const T *beam1, const T *beam2,
-                          bool js1, bool js2, hipStream_t s) {
-  if constexpr (!cfg2_fits<T, F2_TX, F2_TY>()) {
+// Launches two steps with tiles TX x TY; false (nothing launched) when not
+// supported.
+template <typename T, int TX, int TY>
+inline bool launch_fused2_t(const StepArgs<T> &a, T *En, T *Hn, const T *beam1, const T *beam2,
+                            bool js1, bool js2, hipStream_t s) {
+  if constexpr (!cfg2_fits<T, TX, TY>()) {
     (void)a; (void)En; (void)Hn; (void)beam1; (void)beam2; (void)js1; (void)js2; (void)s;
     return false;
   } else {
-    if (!fused2_supported<T>(a)) return false;
-    constexpr int TX = F2_TX, TY = F2_TY, NT = (TX + 4) * (TY + 4);
+    if (!fused2_supported_t<T, TX, TY>(a)) return false;
+    constexpr int NT = (TX + 4) * (TY + 4);
     Fused2Args<T> f;
     f.a = a;
     f.En = En;
@@ -1005,6 +1040,74 @@ inline bool launch_fused2(const StepArgs<T> &a, T *En, T *Hn, const T *beam1, co
                          f);
     return true;
   }
+}
+
+template <typename T>
+inline bool launch_fused2(const StepArgs<T> &a, int cfg, T *En, T *Hn, const T *beam1,
+                          const T *beam2, bool js1, bool js2, hipStream_t s) {
+  switch (cfg) {
+    case F2_CFG_64x8: return launch_fused2_t<T, 64, 8>(a, En, Hn, beam1, beam2, js1, js2, s);
+    case F2_CFG_32x16: return launch_fused2_t<T, 32, 16>(a, En, Hn, beam1, beam2, js1, js2, s);
+    case F2_CFG_32x8: return launch_fused2_t<T, 32, 8>(a, En, Hn, beam1, beam2, js1, js2, s);
+    case F2_CFG_16x16: return launch_fused2_t<T, 16, 16>(a, En, Hn, beam1, beam2, js1, js2, s);
+    default: return false;
+  }
+}
+
+// Modelled time of a two-step launch with a shape: the blocks (columns x
+// chunks of the kc that pick_kc2 takes) run in rounds of the resident
+// blocks, a round lasting kc + 3 planes of one block, and a block's plane
+// time is its window cells x the shape's measured time per window cell
+// relative to 64 x 8 (launches on a 256 x 256 x 300 grid, no overhang:
+// 64 x 8 0.378 ms, 32 x 16 0.390, 32 x 8 0.432, 16 x 16 0.472) x its
+// resident blocks per CU (blocks sharing a CU share its throughput).  On
+// large grids this is the window cells over the covered (overhanging)
+// columns; on small ones the rounds and chunk lengths decide.
+template <typename T, int TX, int TY>
+inline double fused2_time_t(const StepArgs<T> &a, int dev, double rel) {
+  constexpr int NT = (TX + 4) * (TY + 4);
+  int ncu = 0;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (ncu <= 0) ncu = 256;
+  const int slots = resident_slots(dev, (const void *)fused2_sweep<T, TX, TY, false>, NT,
+                                   fused_mt_bytes<T>(a.nmat));
+  const int bpc = slots / ncu > 0 ? slots / ncu : 1;
+  const int ncol = ((a.nx + TX - 1) / TX) * ((a.ny + TY - 1) / TY);
+  const int kc = pick_kc2(ncol, a.nz, slots);
+  const long long blocks = (long long)ncol * ((a.nz + kc - 1) / kc);
+  const int xslots = slots / 8 > 0 ? slots / 8 : 1;
+  const long long rounds = ((blocks + 7) / 8 + xslots - 1) / xslots;
+  return (double)rounds * (kc + 3) * NT * rel * bpc;
+}
+
+template <typename T>
+inline double fused2_time(const StepArgs<T> &a, int cfg, int dev) {
+  switch (cfg) {
+    case F2_CFG_64x8: return fused2_time_t<T, 64, 8>(a, dev, 1.0);
+    case F2_CFG_32x16: return fused2_time_t<T, 32, 16>(a, dev, 1.17);
+    case F2_CFG_32x8: return fused2_time_t<T, 32, 8>(a, dev, 1.08);
+    case F2_CFG_16x16: return fused2_time_t<T, 16, 16>(a, dev, 1.27);
+    default: return 1e300;
+  }
+}
+
+// The tile shape used when none is set: the least fused2_time among the
+// shapes that apply (-1: none).
+template <typename T>
+inline int fused2_default_cfg(const StepArgs<T> &a) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  int best = -1;
+  double bc = 1e300;
+  for (int c = 0; c < F2_NCFG; ++c) {
+    if (!fused2_supported<T>(a, c)) continue;
+    const double t = fused2_time<T>(a, c, dev);
+    if (t < bc) {
+      bc = t;
+      best = c;
+    }
+  }
+  return best;
 }
 
 }  // namespace fit
