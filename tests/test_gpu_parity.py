@@ -276,6 +276,32 @@ def test_fused_multi_tile_fp64(name, bl, bh, tile):
     assert max(err.values()) <= TOL64, err
 
 
+@pytest.mark.parametrize("tile", list(range(6, 12)))
+def test_row_sweep_j_buffer_fp64(tile):
+    """The row-wave tiles (ids 6-11) in FP64 with J read from its buffer on
+    every step (a non-conductive solver given a J field), on a grid whose
+    last z chunk runs the tail loop for several planes: the variants that
+    run (spill-free builds; a variant that spills is never launched) match
+    the oracle within 1e-10 on E, H and J."""
+    shape = (70, 13, 70)  # chunks of 32 planes: [0, 32), [32, 64), [64, 70)
+    print(f"[case] row sweep J buffer fp64 tile={tile} shape={shape}", flush=True)
+    Nx, Ny, Nz = shape
+    solids = [(block_mask(shape, (3, 2, 5), (Nx - 4, Ny - 3, Nz - 6)), "vacuum"),
+              (block_mask(shape, (20, 4, 30), (50, 10, 66)), [3.0, 1.2])]
+    osv, psv = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, solids=solids, seed=23,
+                         kernel="fused", tile=tile)
+    runs = psv.ctx.tune(reps=1)[tile] > 0
+    psv.ctx.set_tile(tile)
+    print("runs as itself:", runs, flush=True)
+    for _ in range(8):
+        osv.one_step()
+        psv.one_step()
+    assert psv.ctx.get_state()[0] == 2  # FIT_J_BUFFER throughout
+    err = compare_fields(osv, psv)
+    print(err)
+    assert max(err.values()) <= TOL64, err
+
+
 def test_fused_equals_split_bitwise_structure():
     """Fused and split kernels agree to rounding on a mid-size grid (FP64)."""
     shape = (70, 20, 40)
