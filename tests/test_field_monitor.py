@@ -152,3 +152,74 @@ def test_device_monitor_in_wakesolve():
         a, b, c = pm.get_components()[k], om.get_components()[k], pm2.get_components()[k]
         assert crel(a, c) <= 1e-12, k
         assert crel(a, b) <= 1e-10, k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype,tile", [(np.float64, 1), (np.float64, 4), (np.float64, 8),
+                                        (np.float32, 5), (np.float32, 2)],
+                         ids=["fp64-t1", "fp64-t4", "fp64-row8", "fp32-t5", "fp32-t2"])
+def test_fused_monitor_update(dtype, tile):
+    """The monitor update fused into the one-step sweep (every chunk on the
+    general tile path while a monitor update is staged) on a grid of several
+    tiles and z chunks: accumulators against the oracle's reference update
+    after each step (FP64 1e-10, FP32 1e-4) and against the separate
+    accumulation pass of the split kernels (1e-12 FP64).  Tile 8 is a
+    row-sweep tile, which keeps the separate pass."""
+    import wakis_amd as wa
+    from helpers import block_mask
+
+    shape = (70, 20, 76)
+    print(f"[case] fused monitor {np.dtype(dtype).name} tile={tile}", flush=True)
+    kw = dict(q=1e-9, sigmaz=0.01, beta=1.0, wakelength=0.04)
+    solids = [(block_mask(shape, (2, 2, 0), (68, 18, 76)), "vacuum"),
+              (block_mask(shape, (20, 5, 30), (40, 12, 50)), [3.0, 1.0, 30.0])]
+    bc = (["pec"] * 3, ["pec", "pec", "pml"])
+    osv, pf = make_pair(*shape, *bc, dtype=dtype, init=False, solids=solids, bg="pec",
+                        wake_kw=kw, kernel="fused", tile=tile, n_pml=4)
+    _, ps = make_pair(*shape, *bc, dtype=dtype, init=False, solids=solids, bg="pec",
+                      wake_kw=kw, kernel="split", n_pml=4)
+    freqs = [0.7e9, 2.2e9]
+    om = OFieldMonitor(freqs)
+    osv.wakesolve(compute_plane="longitudinal", callback=lambda s, t: om.update(s.E, s.dt))
+    mf, ms = wa.FieldMonitor(freqs), wa.FieldMonitor(freqs)
+    pf.wakesolve(compute_plane="longitudinal", use_field_monitor=True, field_monitor=mf)
+    ps.wakesolve(compute_plane="longitudinal", use_field_monitor=True, field_monitor=ms)
+    assert pf.ctx.launch_counts()[1] == 0  # one-step sweeps while the monitor runs
+    tol = 1e-10 if dtype == np.float64 else 1e-4
+    for k in ("Ex", "Ey", "Ez"):
+        a, b, c = mf.get_components()[k], om.get_components()[k], ms.get_components()[k]
+        assert np.abs(b).max() > 0, k
+        assert crel(a, b) <= tol, (k, crel(a, b))
+        if dtype == np.float64:
+            assert crel(a, c) <= 1e-12, (k, crel(a, c))
+
+
+@pytest.mark.gpu
+def test_monitor_staging_after_frequency_count_change():
+    """Phases staged for one frequency, then, after fit_monitor_set with two,
+    for two: the staged table is re-sized by bytes, and the fused updates
+    equal a separate update after each step (regression: a table sized for
+    one frequency's rows was reused for two)."""
+    shape = (40, 12, 30)
+    print("[case] monitor restaging", flush=True)
+    _, pa = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, seed=5, kernel="fused")
+    _, pb = make_pair(*shape, ["pec"] * 3, ["pec"] * 3, seed=5, kernel="fused")
+    for s in (pa, pb):
+        s.ctx.zero_ghosts()
+        s.step_0 = False
+    n = 6
+    for nf in (1, 2):
+        ph = np.exp(-2j * np.pi * np.outer(np.arange(n) + 1.0, 0.05 * (np.arange(nf) + 1)))
+        pa.ctx.monitor_set(nf)
+        pb.ctx.monitor_set(nf)
+        pa.ctx.monitor_phases(ph)
+        pa.ctx.step(n, False)
+        for q in range(n):
+            pb.ctx.step(1, False)
+            pb.ctx.monitor_accumulate(ph[q])
+        for f in range(nf):
+            for comp in range(3):
+                a = pa.ctx.monitor_read(f, comp)
+                b = pb.ctx.monitor_read(f, comp)
+                assert np.abs(b).max() > 0
+                assert crel(a, b) <= 1e-12, (nf, f, comp, crel(a, b))

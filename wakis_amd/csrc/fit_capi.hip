@@ -359,8 +359,18 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
       a.jline = (const T *)c->btab + (int64_t)s * c->nz;
     }
     if (ev) HIPCHK(hipEventRecord(ev[3 * s], c->stream));
+    const bool mon_step = c->mon && s < c->mon_ph_steps;
+    bool mon_fused = false;
     if (fused) {
       const int cfg = c->fused_cfg >= 0 ? c->fused_cfg : fit::sweep_default(c->nx, c->ny, c->nz, c->dtype == FIT_F64);
+      // the field monitor update of this step runs inside the sweep (its
+      // general tile path) when the sweep is a fused_sweep tile
+      if (mon_step && cfg < fit::FUSED_NCFG) {
+        a.mon = (double2 *)c->mon;
+        a.mph = c->mon_ph + (int64_t)s * 2 * c->mon_nf;
+        a.mnf = c->mon_nf;
+        mon_fused = true;
+      }
       if (!fit::launch_sweep<T>(a, (T *)c->E_alt, (T *)c->H_alt, c->stream, cfg))
         return fail(FIT_ESTATE, "fused sweep tile configuration " + std::to_string(cfg) +
                                     " spills to scratch memory in this build; not launched");
@@ -375,7 +385,7 @@ int step_impl(fit_ctx *c, int nsteps, int conductive, hipEvent_t *ev = nullptr) 
       hipLaunchKernelGGL(fit::e_update_split<T>, grid, block, 0, c->stream, a);
     }
     if (ev) HIPCHK(hipEventRecord(ev[3 * s + 2], c->stream));
-    if (c->mon && s < c->mon_ph_steps) monitor_launch<T>(c, c->mon_ph + (int64_t)s * 2 * c->mon_nf);
+    if (mon_step && !mon_fused) monitor_launch<T>(c, c->mon_ph + (int64_t)s * 2 * c->mon_nf);
     HIPCHK(hipGetLastError());
     c->launches[0] += 1;
     c->step += 1;
@@ -941,24 +951,26 @@ int fit_monitor_set(fit_ctx *c, int32_t nf) {
 
 namespace {
 int monitor_stage(fit_ctx *c, const double *rows, int64_t nrows) {
-  if (c->mon_ph_cap < nrows) {
+  // the table is sized in bytes: a later fit_monitor_set may change nf
+  const size_t need = (size_t)nrows * 2 * c->mon_nf * 8;
+  if (c->mon_ph_bytes < need) {
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->mon_ph) HIPCHK(hipFree(c->mon_ph));
     c->mon_ph = nullptr;
     c->mon_ph_cap = 0;
     c->mon_ph_bytes = 0;
-    if (hipMalloc((void **)&c->mon_ph, (size_t)nrows * 2 * c->mon_nf * 8) != hipSuccess) {
+    if (hipMalloc((void **)&c->mon_ph, need) != hipSuccess) {
       c->mon_ph = nullptr;
       (void)hipGetLastError();
       return fail(FIT_ENOMEM, "field monitor phase table allocation failed");
     }
-    c->mon_ph_cap = nrows;
-    c->mon_ph_bytes = (size_t)nrows * 2 * c->mon_nf * 8;
+    c->mon_ph_bytes = need;
     note_peak(c);
   }
+  c->mon_ph_cap = (int64_t)(c->mon_ph_bytes / ((size_t)2 * c->mon_nf * 8));
   // the previous use of the table (launches of an earlier call) must be done
   HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpy(c->mon_ph, rows, (size_t)nrows * 2 * c->mon_nf * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->mon_ph, rows, need, hipMemcpyHostToDevice));
   return 0;
 }
 }  // namespace

@@ -97,6 +97,11 @@ struct StepArgs {
   T *hist;
   int64_t hist_nt;
   int ixt, iyt, hw;
+  // field monitor update fused into the step (one-step fused sweep): the
+  // complex128 accumulators acc[f][c][N] and this step's nf phases, or null
+  double2 *mon;
+  const double *mph;
+  int mnf;
 };
 
 // Boundary mask of component `comp` at (i, j, k).  The reference builds it as
@@ -234,6 +239,31 @@ template <typename T>
 __device__ __forceinline__ const T *mat_row(const StepArgs<T> &a, int64_t n) {
   const int64_t m = a.mat32 ? (int64_t)a.mat32[n] : (a.mat ? (int64_t)a.mat[n] : 0);
   return a.mtab + m * MAT_STRIDE;
+}
+
+// FieldMonitor update of cell n with the step's new E (wakis/field_monitors.py:
+// 96-102): acc[f][c][n] += E_c * phase_f in complex128, E promoted to (E, 0)
+// and the product rounded as numpy's complex multiply, (e*pr - 0*pi,
+// e*pi + 0*pr), then the two adds, no contraction into an FMA (the same
+// arithmetic as the monitor_accumulate kernel).
+template <typename T>
+__device__ __forceinline__ void monitor_cell(const StepArgs<T> &a, int64_t n, T ex, T ey, T ez) {
+#pragma clang fp contract(off)
+  const T ev[3] = {ex, ey, ez};
+  for (int f = 0; f < a.mnf; ++f) {
+    const double pr = a.mph[2 * f], pi = a.mph[2 * f + 1];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double e = (double)ev[c];
+      const double re = e * pr - 0.0 * pi;
+      const double im = e * pi + 0.0 * pr;
+      double2 *p = a.mon + ((int64_t)f * 3 + c) * a.N + n;
+      double2 v = *p;
+      v.x = v.x + re;
+      v.y = v.y + im;
+      *p = v;
+    }
+  }
 }
 
 // Wake probe: E_z of the state before the update of this step

@@ -148,7 +148,7 @@ __device__ __forceinline__ WinCell win_cell(int gi, int gj, int nx, int ny) {
 // JB: the J buffer mode (J read from memory) or not (J = 0 or sigma*E); the
 // two are separate code, so the J loads never meet the other mode's values
 // in a register (a join there would make every wave wait for all its loads).
-template <typename T, int TX, int TY, int KC, bool JB, bool NTS>
+template <typename T, int TX, int TY, int KC, bool JB, bool NTS, bool MON = true>
 __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile,
                                            FusedSmem<T, TX, TY> &sm, T *__restrict__ smt) {
   using G = Tile<TX, TY>;
@@ -508,22 +508,29 @@ __device__ __forceinline__ void fused_tile(const FusedArgs<T> &f, const int tile
       const T cx = ((wz - ww[2][lj - 1][li]) - wy) + wyp;
       const T cy = ((wx - wxp) - wz) + ww[2][lj][li - 1];
       const T cz = ((wy - ww[1][lj][li - 1]) - wx) + ww[0][lj - 1][li];
+      T enx, eny, enz;
       if (JB) {
         const T jx = Jx[n];
         const T jy = (Jx + N)[n];
         const T jz = beam_col ? jk : (Jx + 2 * (int64_t)N)[n];
-        gstore(&Enx[n], eox + mt[3] * (ax * cx - jx), NTS);
-        gstore(&Eny[n], eoy + mt[4] * (ay * cy - jy), NTS);
-        gstore(&Enz[n], eoz + mt[5] * (az * cz - jz), NTS);
+        enx = eox + mt[3] * (ax * cx - jx);
+        eny = eoy + mt[4] * (ay * cy - jy);
+        enz = eoz + mt[5] * (az * cz - jz);
       } else {
         const T js = a.jmode == J_SIGMA_E ? T(1) : T(0);
         const T jx = js * (mt[6] * eox);
         const T jy = js * (mt[7] * eoy);
         const T jz = beam_col ? jk : js * (mt[8] * eoz);
-        gstore(&Enx[n], eox + mt[3] * (ax * cx - jx), NTS);
-        gstore(&Eny[n], eoy + mt[4] * (ay * cy - jy), NTS);
-        gstore(&Enz[n], eoz + mt[5] * (az * cz - jz), NTS);
+        enx = eox + mt[3] * (ax * cx - jx);
+        eny = eoy + mt[4] * (ay * cy - jy);
+        enz = eoz + mt[5] * (az * cz - jz);
       }
+      gstore(&Enx[n], enx, NTS);
+      gstore(&Eny[n], eny, NTS);
+      gstore(&Enz[n], enz, NTS);
+      // field monitor fused into the step: accumulate the new E here instead
+      // of reading it back in a separate pass
+      if (MON && a.mon) monitor_cell(a, n, enx, eny, enz);
     }
     wxp = wx;
     wyp = wy;
@@ -1014,20 +1021,23 @@ inline int pick_kc(int KC, int ncol, int nz, int slots) {
 // holds) the planes of every chunk do, per field component.  (In FP64 the
 // lean tile holds 209 VGPRs against the general tile's 189: two waves per
 // SIMD either way.)  NTS: non-temporal field stores.
-template <typename T, int TX, int TY, int KC, bool NTS, bool PCR = false>
+// MON: the general path may carry a fused field-monitor update (a.mon set:
+// every chunk then takes the general path); false for the batched launch,
+// whose members have no monitor.
+template <typename T, int TX, int TY, int KC, bool NTS, bool PCR = false, bool MON = true>
 __device__ __forceinline__ void fused_dispatch(const FusedArgs<T> &f, int tile,
                                                FusedSmem<T, TX, TY> &sm, T *smt) {
   {
     int k0, k1;
     chunk_span(f, tile / (f.ntx * f.nty), k0, k1);
-    if ((PCR || f.a.buf32) && k0 >= 3 && k1 + 4 <= f.a.nz) {
+    if ((PCR || f.a.buf32) && k0 >= 3 && k1 + 4 <= f.a.nz && !(MON && f.a.mon)) {
       if (f.a.jmode == J_BUFFER) fused_tile_inner<T, TX, TY, KC, true, NTS, PCR>(f, tile, sm, smt);
       else fused_tile_inner<T, TX, TY, KC, false, NTS, PCR>(f, tile, sm, smt);
       return;
     }
   }
-  if (f.a.jmode == J_BUFFER) fused_tile<T, TX, TY, KC, true, NTS>(f, tile, sm, smt);
-  else fused_tile<T, TX, TY, KC, false, NTS>(f, tile, sm, smt);
+  if (f.a.jmode == J_BUFFER) fused_tile<T, TX, TY, KC, true, NTS, MON>(f, tile, sm, smt);
+  else fused_tile<T, TX, TY, KC, false, NTS, MON>(f, tile, sm, smt);
 }
 
 // Tile order of block b among nb blocks: the hardware dispatches block b to
@@ -1087,7 +1097,8 @@ __global__ void __launch_bounds__(TX *TY)
     f.a.step = m.step0 + s;
     if (m.btab) f.a.jline = m.btab + (int64_t)g * f.a.nz;
     const int b = t - tstart[lo];
-    fused_dispatch<T, TX, TY, KC, false>(f, b < f.nremap ? xcd_tile(b, f.nremap) : b, sm, smt);
+    fused_dispatch<T, TX, TY, KC, false, false, false>(f, b < f.nremap ? xcd_tile(b, f.nremap) : b,
+                                                       sm, smt);
   }
 }
 

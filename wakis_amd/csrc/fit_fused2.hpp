@@ -1065,6 +1065,10 @@ inline bool launch_fused2(const StepArgs<T> &a, int cfg, T *En, T *Hn, const T *
 // columns; on small ones the rounds and chunk lengths decide.
 template <typename T, int TX, int TY>
 inline double fused2_time_t(const StepArgs<T> &a, int dev, double rel) {
+  if constexpr (!cfg2_fits<T, TX, TY>()) {
+    (void)a; (void)dev; (void)rel;
+    return 1e300;
+  } else {
   constexpr int NT = (TX + 4) * (TY + 4);
   int ncu = 0;
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -1078,6 +1082,7 @@ inline double fused2_time_t(const StepArgs<T> &a, int dev, double rel) {
   const int xslots = slots / 8 > 0 ? slots / 8 : 1;
   const long long rounds = ((blocks + 7) / 8 + xslots - 1) / xslots;
   return (double)rounds * (kc + 3) * NT * rel * bpc;
+  }
 }
 
 template <typename T>
