@@ -284,3 +284,80 @@ def test_cubic_pml_profile_closed_form():
     with pytest.raises(ValueError):
         wa.SolverFIT3D(g, bc_low=["pml"] * 3, bc_high=["pml"] * 3, n_pml=n, device=None,
                        pml_profile="linear")
+
+
+@pytest.mark.parametrize("tensor", ["all", "ieps"])
+def test_update_tensors_equals_constructor_materials(tensor):
+    """update_tensors (wakis/solverFIT3D.py:378-436) rebuilds the operator
+    products from the edited ieps Field exactly as the constructor builds
+    them from materials: a vacuum solver whose ieps Field is set to that of
+    a dielectric solid and re-derived by update_tensors steps bit for bit
+    like the solver built with the solid (the ghost zeroing that
+    update_tensors disables is redone by hand).  A dropped iDeps rebuild, a
+    wrong operand order or the E-update's J term reading a stale ieps each
+    fail it."""
+    shape = (7, 6, 9)
+    blk = block_mask(shape, (1, 1, 2), (5, 5, 7))
+    kw = dict(Nx=7, Ny=6, Nz=9, xmin=0, xmax=0.07, ymin=0, ymax=0.06, zmin=0, zmax=0.09)
+    a = OSolver(OGrid(**kw), bc_low=["pec"] * 3, bc_high=["pec"] * 3,
+                solids=[(blk, [4.0, 1.0])])
+    b = OSolver(OGrid(**kw), bc_low=["pec"] * 3, bc_high=["pec"] * 3)
+    b.dt = a.dt
+    assert not np.array_equal(a.ieps.toarray(), b.ieps.toarray())
+    b.ieps.fromarray(a.ieps.toarray())
+    b.update_tensors(tensor)
+    assert not b.step_0
+    b.step_0 = True
+    rng = np.random.default_rng(3)
+    for s in (a, b):
+        s.E.fromarray(np.zeros(3 * s.N))
+    e0 = rng.standard_normal(3 * a.N)
+    j0 = 1e-3 * rng.standard_normal(3 * a.N)
+    for s in (a, b):
+        s.E.fromarray(e0.copy())
+        s.J.fromarray(j0.copy())
+    for _ in range(5):
+        a.one_step()
+        b.one_step()
+    assert np.abs(a.E.toarray()).max() > 0
+    for f in "EH":
+        assert np.array_equal(getattr(a, f).toarray(), getattr(b, f).toarray()), f
+
+
+PW = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "planewave.npz"))
+PW_CASES = {
+    "default_f64": (dict(nodes=3, f=1.5e9, amplitude=2.5, phase=0.3), np.float64),
+    "slices_f32": (dict(xs=slice(1, 4), ys=slice(0, 3), zs=2, nodes=2, f=0.8e9, amplitude=100.0,
+                        phase=np.pi / 2), np.float32),
+    "zslice_beta": (dict(zs=slice(1, 3), f=2.0e9, amplitude=1.0, beta=0.7), np.float64),
+}
+
+
+def test_planewave_case_lists_match():
+    assert list(PW["cases"]) == list(PW_CASES)
+
+
+@pytest.mark.parametrize("name", list(PW_CASES))
+def test_planewave_matches_reference_vectors(name):
+    """OPlaneWave.update and the product's PlaneWave.update on host Fields
+    write the reference's E_x / H_y slices bit for bit
+    (tests/golden/planewave.npz, from wakis/sources.py:142-248 on the
+    reference's own Field), before and after tmax."""
+    import wakis_amd as wa
+
+    nx, ny, nz = (int(v) for v in PW["dims"])
+    kw, dtype = PW_CASES[name]
+    o = SimpleNamespace(Nx=nx, Ny=ny, Nz=nz, E=OField(nx, ny, nz, dtype=dtype),
+                        H=OField(nx, ny, nz, dtype=dtype))
+    p = SimpleNamespace(Nx=nx, Ny=ny, Nz=nz, E=wa.Field(nx, ny, nz, dtype=dtype),
+                        H=wa.Field(nx, ny, nz, dtype=dtype))
+    from oracle.fit_oracle import OPlaneWave
+
+    opw, ppw = OPlaneWave(**kw), wa.PlaneWave(**kw)
+    for q, t in enumerate(PW["times"]):
+        opw.update(o, t)
+        ppw.update(p, t)
+        for f in "EH":
+            ref = PW[f"{name}/{f}/{q}"]
+            assert np.array_equal(getattr(o, f).toarray(), ref), (f, q)
+            assert np.array_equal(np.asarray(getattr(p, f).toarray()), ref), (f, q)
