@@ -139,6 +139,11 @@ int fit_field_write(fit_ctx *ctx, int32_t which, int32_t comp,
                     const int32_t box[6], const void *host);
 /* Fill a whole field with zeros. */
 int fit_field_zero(fit_ctx *ctx, int32_t which);
+/* Watchdog (no counterpart in the reference, whose solver never checks):
+ * *finite = 1 if every value of field `which` is finite, 0 if any is NaN or
+ * infinite; one pass over the 3N values on the context's stream (a 256-byte
+ * flag is allocated on the first call and kept).  Synchronous. */
+int fit_field_finite(fit_ctx *ctx, int32_t which, int32_t *finite);
 /* Write the J the next step would use (per the J mode: 0, sigma*E or the
  * buffer, then the beam line) into the J buffer, so that it can be read. */
 int fit_materialize_j(fit_ctx *ctx);

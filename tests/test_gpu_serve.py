@@ -103,3 +103,31 @@ def test_serve_two_step_members_equal_alone():
         assert np.array_equal(r.result["Z"], alone["Z"]), r.rid
         print(r.rid, r.cells, r.steps, "peak", r.result["device_peak"], "model",
               r.result["device_model"], flush=True)
+
+
+def test_watchdog_and_cancel_on_device():
+    """A request whose fields hold a NaN (written after it is attached) is
+    stopped by the device watchdog at the first round boundary, a request
+    cancelled before it starts never steps, and the others give the same
+    results as alone."""
+    from wakis_amd.serve import Server, solve_alone
+
+    print("[case] serve watchdog + cancel", flush=True)
+    reqs = _requests()
+    clock = VClock()
+    srv = Server(dtype=np.float32, chunk_steps=40, graph=True, group=8, tile=2, clock=clock)
+    clock.server = srv
+
+    def poison(req, solver):
+        if req.rid == 1:
+            solver.E[3, 4, 5, "z"] = np.nan
+
+    srv.after_attach = poison
+    srv.cancel(3)
+    clock.server = srv
+    done = {r.rid: r for r in srv.run(reqs)}
+    assert "non-finite" in done[1].result["error"] and done[1].steps == 40
+    assert "cancelled" in done[3].result["error"] and done[3].steps == 0
+    for rid in (0, 2):
+        _, ref = solve_alone(done[rid], dtype=np.float32, tile=2)
+        assert np.array_equal(done[rid].result["Z"], ref["Z"]), rid

@@ -94,6 +94,16 @@ class FakeContext:
     def device_bytes(self):
         return sum(self.held.values()), self.peak
 
+    # the watchdog: a context whose shape is in `diverge` reports non-finite
+    # fields once it has stepped more than `diverge_after` steps
+    diverge = ()
+    diverge_after = 0
+
+    def field_finite(self, which=0):
+        self._hold("flag", 256)
+        bad = (self.nx, self.ny, self.nz) in self.diverge
+        return not (bad and getattr(self, "steps", 0) > self.diverge_after)
+
     def close(self):
         if not self.closed:
             self.ledger.add(-sum(self.held.values()))
@@ -127,6 +137,8 @@ class FakeBatch:
         self.members = list(solvers)
 
     def step(self, n):
+        for s in self.members:
+            s.ctx.steps = getattr(s.ctx, "steps", 0) + int(n)
         return 0.1 * n
 
     def device_bytes(self):
@@ -248,7 +260,8 @@ def test_model_terms():
     hist = nz * nt * esz
     ns = nt + 2
     finish = 4 + 4 * nz * ns + 8 * nz + 8 * ns + 16 * ns + 48 * 11
-    assert b == fields + metric + tables + hist + 8 * nz * esz + 4096 + finish
+    watchdog = 256
+    assert b == fields + metric + tables + hist + 8 * nz * esz + 4096 + finish + watchdog
     # a 3 x 3 probe window holds nine histories
     b9 = serve.device_bytes(nx, ny, nz, nt, esz, nmat=3, half=1, beam_rows=8, samples=11)
     assert b9 - b == 8 * nz * nt * esz + 32 + 64 * ns
@@ -294,3 +307,101 @@ def test_two_step_members_step_alone_in_even_calls(fake_device):
     assert max(e[8] for e in srv.log) >= 1
     assert sum(e[7] for e in srv.log) == sum(r.cells * r.steps for r in done)
     assert fake_device.peak <= srv.peak_admitted
+
+
+@pytest.mark.timeout(300)
+def test_watchdog_stops_a_diverging_solve(fake_device, monkeypatch):
+    """A solve whose fields turn non-finite is stopped at the next round
+    boundary with an error result and its memory released; the other
+    requests complete."""
+    reqs = _burst(4)
+    from wakis_amd.workloads import build
+
+    _, _, s = build(reqs[1].workload, dtype=np.float32, device=None, precision="mixed")
+    monkeypatch.setattr(FakeContext, "diverge", ((s.Nx, s.Ny, s.Nz),))
+    monkeypatch.setattr(FakeContext, "diverge_after", 60)
+    clock = VClock()
+    srv = serve.Server(dtype=np.float32, chunk_steps=50, group=8, clock=clock)
+    clock.server = srv
+    done = srv.run(reqs)
+    bad = [r for r in done if "error" in r.result]
+    assert [r.rid for r in bad] == [reqs[1].rid], [(r.rid, r.result.get("error")) for r in done]
+    err = bad[0].result["error"]
+    assert "non-finite" in err and "watchdog" in err
+    assert 60 < bad[0].steps <= 110  # caught at the first round boundary past step 60
+    assert all("error" not in r.result for r in done if r.rid != reqs[1].rid)
+    assert fake_device.live == 0
+
+
+@pytest.mark.timeout(300)
+def test_cancel_before_and_during_the_run(fake_device):
+    """cancel() drops a request before preparation, one waiting for
+    admission, and one stopped between rounds; the rest complete and every
+    byte is released."""
+    reqs = _burst(6, arrivals=[0.0, 0.0, 0.0, 0.0, 0.0, 5.0])
+    models = {r.rid: _model(r) for r in reqs}
+    budget = 1.05 * max(models.values())  # one request at a time: the others queue
+    clock = VClock()
+    srv = serve.Server(dtype=np.float32, budget=budget, chunk_steps=50, group=8, clock=clock)
+    clock.server = srv
+    srv.cancel(reqs[5].rid)  # arrives later: never prepared
+    first = []
+
+    # the first request to reach the device is cancelled before its first
+    # step; reqs[3] is cancelled as soon as it is prepared
+    def after_attach(req, solver):
+        if not first:
+            first.append(req.rid)
+            srv.cancel(req.rid)
+
+    prepare = srv._prepare
+
+    def prepare_then_cancel(req):
+        it = prepare(req)
+        if req.rid == reqs[3].rid:
+            srv.cancel(req.rid)
+        return it
+
+    srv.after_attach = after_attach
+    srv._prepare = prepare_then_cancel
+    done = srv.run(reqs)
+    res = {r.rid: r.result for r in done}
+    assert "cancelled before preparation" in res[reqs[5].rid]["error"]
+    assert "cancelled before admission" in res[reqs[3].rid]["error"]
+    if first[0] != reqs[3].rid:
+        assert "cancelled before its first step" in res[first[0]]["error"]
+    for r in done:
+        if r.rid not in (reqs[5].rid, reqs[3].rid, first[0]):
+            assert "error" not in r.result, r.result
+    assert fake_device.live == 0
+    assert all(r.finished >= r.arrival for r in done)
+
+
+@pytest.mark.timeout(300)
+def test_cancel_a_running_solve(fake_device, monkeypatch):
+    """A request cancelled while it is stepping stops at the next round
+    boundary with the steps it took; its memory is released."""
+    reqs = _burst(3)
+    from wakis_amd.workloads import build
+
+    _, _, s = build(reqs[2].workload, dtype=np.float32, device=None, precision="mixed")
+    shape = (s.Nx, s.Ny, s.Nz)
+    clock = VClock()
+    srv = serve.Server(dtype=np.float32, chunk_steps=50, group=8, clock=clock)
+    clock.server = srv
+    step = FakeBatch.step
+
+    def step_then_cancel(self, n):
+        ms = step(self, n)
+        if any((m.Nx, m.Ny, m.Nz) == shape for m in self.members):
+            srv.cancel(reqs[2].rid)
+        return ms
+
+    monkeypatch.setattr(FakeBatch, "step", step_then_cancel)
+    done = srv.run(reqs)
+    res = {r.rid: r for r in done}
+    err = res[reqs[2].rid].result["error"]
+    assert err == "cancelled after 50 steps", err
+    assert res[reqs[2].rid].steps == 50
+    assert all("error" not in res[r.rid].result for r in reqs[:2])
+    assert fake_device.live == 0

@@ -62,6 +62,7 @@ SIGNATURES = {
     "fit_field_read": [_P, _I32, _I32, _PI32, _P],
     "fit_field_write": [_P, _I32, _I32, _PI32, _P],
     "fit_field_zero": [_P, _I32],
+    "fit_field_finite": [_P, _I32, _PI32],
     "fit_materialize_j": [_P],
     "fit_get_state": [_P, _PI32, _PI32],
     "fit_zero_ghosts": [_P],
@@ -295,6 +296,12 @@ class Context:
 
     def materialize_j(self):
         check(self.lib.fit_materialize_j(self.h), "fit_materialize_j")
+
+    def field_finite(self, which: int = FIELD_E) -> bool:
+        """Whether every value of a field is finite (fit_field_finite)."""
+        out = C.c_int32()
+        check(self.lib.fit_field_finite(self.h, int(which), C.byref(out)), "fit_field_finite")
+        return bool(out.value)
 
     # -- stepping ----------------------------------------------------------
     def zero_ghosts(self):

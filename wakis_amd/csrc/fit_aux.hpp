@@ -136,6 +136,17 @@ __global__ void materialize_j(StepArgs<T> a) {
   }
 }
 
+// Watchdog: flag := 1 if any of the n values is NaN or infinite (every
+// writer stores the same value; the host clears the flag before the launch).
+template <typename T>
+__global__ void nonfinite_flag(const T *__restrict__ a, int64_t n, int *__restrict__ flag) {
+  bool bad = false;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x)
+    bad |= !__builtin_isfinite(a[q]);
+  if (bad) *flag = 1;
+}
+
 // FieldMonitor update (wakis/field_monitors.py:96-102): acc[f][c][n] +=
 // E_c[n] * phase_f in complex128, E promoted to (E, 0) and the product
 // rounded as numpy's complex multiply, (e*pr - 0*pi, e*pi + 0*pr), then

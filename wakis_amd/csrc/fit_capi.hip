@@ -78,6 +78,7 @@ struct fit_ctx {
   int f2_def = -1;     // the modelled default shape, computed for f2_def_nmat rows
   int f2_def_nmat = -1;
   void *btab = nullptr;  // beam line table of the steps of one fit_step call
+  int *flag = nullptr;   // watchdog flag (fit_field_finite), 256 bytes
   int64_t btab_steps = 0;
   // field monitor: complex128 accumulators (3 * mon_nf * N) and the staged
   // phases (mon_ph_steps rows of mon_nf complex values) of the next fit_step
@@ -115,6 +116,7 @@ int64_t resident_bytes(const fit_ctx *c) {
   if (c->jline) b += (int64_t)c->nz * c->esz;
   if (c->zc) b += (int64_t)c->nz * 8;
   if (c->btab) b += c->btab_steps * c->nz * (int64_t)c->esz;
+  if (c->flag) b += 256;
   b += (int64_t)(c->mtab_bytes + c->hist_bytes + c->staging_bytes + c->mon_bytes + c->mon_ph_bytes);
   return b;
 }
@@ -608,7 +610,7 @@ int fit_destroy(fit_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ps[] = {c->E,   c->H,     c->J,  c->E_alt, c->H_alt,   c->mat,  c->mtab,
                 c->geo, c->jline, c->zc, c->hist,  c->staging, c->btab, c->mon,
-                c->mon_ph, c->mat32};
+                c->mon_ph, c->mat32, c->flag};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -820,6 +822,39 @@ int fit_field_zero(fit_ctx *c, int32_t which) {
     hipLaunchKernelGGL(fit::fill_zero<float>, dim3(nblocks(cnt, 256)), dim3(256), 0, c->stream,
                        (float *)field_ptr(c, which), cnt);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int fit_field_finite(fit_ctx *c, int32_t which, int32_t *finite) {
+  if (!c || !finite) return fail(FIT_EINVAL, "null argument");
+  if (which < 0 || which > 2) return fail(FIT_EINVAL, "bad field id");
+  if (int rc = set_device(c)) return rc;
+  if (!c->flag) {
+    if (hipMalloc((void **)&c->flag, 256) != hipSuccess) {
+      c->flag = nullptr;
+      (void)hipGetLastError();
+      return fail(FIT_ENOMEM, "watchdog flag allocation failed");
+    }
+    note_peak(c);
+  }
+  if (which == FIT_FIELD_J && c->jmode != FIT_J_BUFFER) {
+    // J is implicit (0 or sigma*E): it is finite whenever E is
+    which = FIT_FIELD_E;
+  }
+  HIPCHK(hipMemsetAsync(c->flag, 0, sizeof(int), c->stream));
+  const int64_t cnt = 3 * c->N;
+  const int nb = (int)std::min<int64_t>(nblocks(cnt, 256), 4096);
+  if (c->dtype == FIT_F64)
+    hipLaunchKernelGGL(fit::nonfinite_flag<double>, dim3(nb), dim3(256), 0, c->stream,
+                       (const double *)field_ptr(c, which), cnt, c->flag);
+  else
+    hipLaunchKernelGGL(fit::nonfinite_flag<float>, dim3(nb), dim3(256), 0, c->stream,
+                       (const float *)field_ptr(c, which), cnt, c->flag);
+  HIPCHK(hipGetLastError());
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *finite = h ? 0 : 1;
   return 0;
 }
 

@@ -154,7 +154,8 @@ def device_bytes(nx, ny, nz, nt, esz, nmat=256, half=0, beam_rows=256, samples=1
     finish      the wake potential's scratch (window ids, nz x ns time columns,
                 dz, the WP rows) and the impedance DFT's (2 ns + 6 K doubles),
                 with ns <= nt + 2 wake samples and K <= samples DFT bins; both
-                are counted although they are held one after the other."""
+                are counted although they are held one after the other
+    watchdog    the 256-byte flag of fit_field_finite."""
     n = nx * ny * nz
     nwin = (2 * half + 1) ** 2
     ns = nt + 2
@@ -164,7 +165,8 @@ def device_bytes(nx, ny, nz, nt, esz, nmat=256, half=0, beam_rows=256, samples=1
     history = nwin * nz * max(nt, 1) * esz
     beam = beam_rows * nz * esz + 4096
     finish = (4 * nwin + 4 * nz * ns + 8 * nz + 8 * nwin * ns) + (16 * ns + 48 * samples)
-    return int(fields + metric + tables + history + beam + finish)
+    watchdog = 256
+    return int(fields + metric + tables + history + beam + finish + watchdog)
 
 
 class Server:
@@ -189,16 +191,31 @@ class Server:
     they do when it is solved alone (chunk_steps must be even) and its
     launches overlap on the device with the other solves' and with the
     batched launch of the remaining solves.  on_done(request) is called in the
-    main loop as each request completes (or is rejected)."""
+    main loop as each request completes (or is rejected, cancelled or
+    stopped by the watchdog).
+
+    watchdog: after every round each stepped solve's E is checked on the
+    device (fit_field_finite); a solve with a NaN or infinite value is
+    stopped, its memory released, and its result is {"error": ...} with the
+    step count.  cancel(rid) (any thread) drops a request before it is
+    prepared or admitted, or stops it at the next round boundary; its result
+    is {"error": "cancelled ..."}.  after_attach(request, solver) runs on the
+    device worker once a request's fields are on the device, before its first
+    step (to set initial fields, for instance)."""
 
     def __init__(self, dtype=np.float32, budget=288e9, chunk_steps=256, graph=True, group=16,
                  tile=None, compute_plane="longitudinal", clock: Callable[[], float] | None = None,
                  prep_workers=2, precision="mixed", device=0, two_step=True,
-                 on_done: Callable[[Request], None] | None = None):
+                 on_done: Callable[[Request], None] | None = None, watchdog=True,
+                 after_attach: Callable[[Request, object], None] | None = None):
         if int(chunk_steps) < 2 or int(chunk_steps) % 2:
             raise ValueError("chunk_steps must be even (pairs of steps start at even indices)")
         self.two_step = bool(two_step)
         self.on_done = on_done
+        self.watchdog = bool(watchdog)
+        self.after_attach = after_attach
+        self._cancel = set()
+        self._cancel_lock = threading.Lock()
         self.dtype = np.dtype(dtype)
         self.precision = precision
         self.budget = float(budget)
@@ -214,6 +231,15 @@ class Server:
         #             the batch, cell-steps, members stepped alone)
         self.log = []
         self.peak_admitted = 0.0
+
+    def cancel(self, rid: int) -> None:
+        """Cancel request rid (safe from any thread); see the class notes."""
+        with self._cancel_lock:
+            self._cancel.add(int(rid))
+
+    def _cancelled(self, req: Request) -> bool:
+        with self._cancel_lock:
+            return req.rid in self._cancel
 
     def now(self) -> float:
         if self._clock is not None:
@@ -240,6 +266,8 @@ class Server:
         s.attach_device(self.device, tile=self.tile)
         s._wake_attach(run)
         s._wake_device_beam(run)
+        if self.after_attach is not None:
+            self.after_attach(it["req"], s)
         s.ctx.sync()
         it["solo"] = self.two_step and s.ctx.kernel_in_use() == _lib.KERNEL_FUSED2
         it["attach_s"] = time.monotonic() - t0
@@ -261,6 +289,11 @@ class Server:
         finally:
             s.ctx.close()
         it["req"].finish_s = time.monotonic() - tf
+        return it
+
+    def _abort(self, it):
+        """Release a stopped request's device memory (device worker)."""
+        it["solver"].ctx.close()
         return it
 
     def run(self, requests):
@@ -292,7 +325,9 @@ class Server:
                 except queue.Empty:
                     continue
                 try:
-                    dev_out.put((kind, self._attach(it) if kind == "attach" else self._finish(it)))
+                    fn = {"attach": self._attach, "finish": self._finish,
+                          "abort": self._abort}[kind]
+                    dev_out.put((kind, fn(it)))
                 except BaseException as e:
                     dev_out.put(("error", e))
 
@@ -309,11 +344,32 @@ class Server:
         used = 0.0       # device_bytes of every admitted, unfinished request
         stepper = None
         finished = 0
+
+        def done(req, error=None):
+            nonlocal finished
+            if error is not None:
+                req.result = {"error": error}
+            req.finished = self.now()
+            finished += 1
+            if self.on_done is not None:
+                self.on_done(req)
+
+        def stop_active(it, error):
+            # a resident request stopped before its last step
+            nonlocal in_flight
+            it["error"] = error
+            dev_in.put(("abort", it))
+            in_flight += 1
+
         try:
             while finished < len(reqs):
                 now = self.now()
                 while nxt < len(reqs) and reqs[nxt].arrival <= now:
-                    todo.put(reqs[nxt])
+                    if self._cancelled(reqs[nxt]):
+                        received += 1  # never prepared
+                        done(reqs[nxt], "cancelled before preparation")
+                    else:
+                        todo.put(reqs[nxt])
                     nxt += 1
                 while True:
                     try:
@@ -323,7 +379,10 @@ class Server:
                     if "error" in item:
                         raise item["error"]
                     received += 1
-                    waiting.append(item)
+                    if self._cancelled(item["req"]):
+                        done(item["req"], "cancelled before admission")
+                    else:
+                        waiting.append(item)
                 while True:
                     try:
                         kind, it = dev_out.get_nowait()
@@ -333,25 +392,26 @@ class Server:
                         raise it
                     in_flight -= 1
                     if kind == "attach":
-                        active.append(it)
-                    else:
-                        it["req"].finished = self.now()
+                        if self._cancelled(it["req"]):
+                            stop_active(it, "cancelled before its first step")
+                        else:
+                            active.append(it)
+                    elif kind == "abort":
                         used -= it["mem"]
-                        finished += 1
-                        if self.on_done is not None:
-                            self.on_done(it["req"])
+                        done(it["req"], it["error"])
+                    else:
+                        used -= it["mem"]
+                        done(it["req"])
                 # admission against the budget, in arrival order; a request
                 # that does not fit waits, later smaller ones may pass it
                 waiting.sort(key=lambda it: it["req"].arrival)
                 keep = []
                 for it in waiting:
-                    if it["mem"] > self.budget:
-                        it["req"].result = {"error": f"needs {it['mem']} device bytes, more "
-                                                     f"than the {self.budget:.0f} byte budget"}
-                        it["req"].finished = self.now()
-                        finished += 1
-                        if self.on_done is not None:
-                            self.on_done(it["req"])
+                    if self._cancelled(it["req"]):
+                        done(it["req"], "cancelled before admission")
+                    elif it["mem"] > self.budget:
+                        done(it["req"], f"needs {it['mem']} device bytes, more than the "
+                                        f"{self.budget:.0f} byte budget")
                     elif used + it["mem"] <= self.budget:
                         it["req"].admitted = self.now()
                         used += it["mem"]
@@ -404,7 +464,12 @@ class Server:
                                  sum(it["solver"].N * it["n"] for it in active), len(solo)))
                 still = []
                 for it in active:
-                    if it["done"] >= it["run"]["Nt"]:
+                    if self._cancelled(it["req"]):
+                        stop_active(it, f"cancelled after {it['done']} steps")
+                    elif self.watchdog and not it["solver"].ctx.field_finite(_lib.FIELD_E):
+                        stop_active(it, f"non-finite fields after {it['done']} steps "
+                                        "(the solve diverged); stopped by the watchdog")
+                    elif it["done"] >= it["run"]["Nt"]:
                         dev_in.put(("finish", it))
                         in_flight += 1
                     else:
