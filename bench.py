@@ -692,7 +692,7 @@ def bench_serve(args):
               f"(arrived {r.arrival:6.2f}s)", file=sys.stderr, flush=True)
 
     srv = Server(dtype=np.float32, budget=288e9, chunk_steps=args.chunk, graph=True, group=16,
-                 on_done=done_line)
+                 on_done=done_line, two_step=not args.all_batched)
     t0 = time.perf_counter()
     done = srv.run(reqs)
     wall = time.perf_counter() - t0
@@ -706,6 +706,17 @@ def bench_serve(args):
     print(f"[bench] device stepping {busy_ms / 1e3:.1f}s of {wall:.1f}s wall, "
           f"{cell_steps / (busy_ms * 1e-3) / 1e9:.1f} Gcell-steps/s while stepping, "
           f"{len(srv.log)} chunks", file=sys.stderr, flush=True)
+    if args.served_only:
+        return {"metric": "solves/h", "value": round(served_per_h, 1), "unit": "solves/h",
+                "n_gpus": 1, "steps": int(sum(r.steps for r in done)), "warmup": 0,
+                "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                "config": {"workload": f"{len(done)} Poisson requests (served only, no baseline)",
+                           "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+                           "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+                           "stepping_gcell_steps_per_s":
+                               round(cell_steps / (busy_ms * 1e-3) / 1e9, 2),
+                           "all_batched": bool(args.all_batched)}}
     # one at a time in FP64, same arrivals, FIFO: (a) this library's FP64
     # kernel; (b) the reference's plain GPU method (sparse operators), its
     # stepping timed on each request's grid and scaled to the request's Nt,
@@ -885,10 +896,15 @@ def bench_large(args):
         # untimed warmup and tile search on the zero fields (they stay zero:
         # no source, J = sigma*E = 0), then the bunch and the probe from step 0
         tune_ms = None
+        kin = ctx.kernel_in_use()
         if args.tile is not None and dtype == np.float32:
             ctx.set_tile(args.tile)  # the FP32 run's tile (profiling runs)
-        elif not args.no_tune and ctx.kernel_in_use() == _lib.KERNEL_FUSED:
+        elif args.tile2 is not None and kin == _lib.KERNEL_FUSED2:
+            ctx.set_tile2(args.tile2)
+        elif not args.no_tune and kin == _lib.KERNEL_FUSED:
             tune_ms = s.tune(reps=2)
+        elif not args.no_tune and kin == _lib.KERNEL_FUSED2:
+            tune_ms = [round(v, 4) for v in ctx.tune2(reps=2)]
         ctx.step(4, cond)
         ctx.sync()
         beam = Beam(q=wake.q, sigmaz=wake.sigmaz, beta=wake.beta)
@@ -899,8 +915,10 @@ def bench_large(args):
         ctx.zero_ghosts()
         s.step_0 = False
         tile = ctx.get_tile()
+        tile2 = ctx.get_tile2() if kin == _lib.KERNEL_FUSED2 else None
         print(f"[bench] large {name}: {s.N} cells, setup {setup_s:.1f}s, dt {float(s.dt):.6e}, "
-              f"tile {tile}, tune {tune_ms}", file=sys.stderr, flush=True)
+              f"kernel {kin}, tile {tile}, tile2 {tile2}, tune {tune_ms}", file=sys.stderr,
+              flush=True)
         total_ms, done = 0.0, 0
         chunk = 250
         while done < args.steps:
@@ -913,7 +931,8 @@ def bench_large(args):
         runs[name] = s
         out[name] = {"setup_s": round(setup_s, 1), "ms_per_step": round(total_ms / args.steps, 4),
                      "gcell_steps_per_s": round(s.N * args.steps / (total_ms * 1e-3) / 1e9, 3),
-                     "tile": {"cfg": tile[0], "tx": tile[1], "ty": tile[2], "kc": tile[3]},
+                     "tile": ({"two_step_cfg": tile2[0], "tx": tile2[1], "ty": tile2[2]} if tile2
+                              else {"cfg": tile[0], "tx": tile[1], "ty": tile[2], "kc": tile[3]}),
                      "esz": np.dtype(dtype).itemsize}
 
     # FP32 against FP64, z-slab by z-slab (full x-y planes are contiguous)
@@ -938,23 +957,34 @@ def bench_large(args):
     finite = bool(np.isfinite(h64).all() and np.isfinite(h32).all())
 
     # average fused-kernel launch over a few more steps (after the comparison)
+    # (two-step launches: two cell-steps of 12 field words per cell each)
     for name, s in runs.items():
         nk = 20
+        c0 = s.ctx.launch_counts()
         _, k1, _ = s.ctx.step_timed(nk, s.use_conductivity, per_kernel=True)
+        c1 = s.ctx.launch_counts()
+        n_one, n_two = c1[0] - c0[0], c1[1] - c0[1]
+        two = n_two > 0 and n_one == 0
         esz = out[name]["esz"]
-        alg = 12 * esz * s.N
-        kms = k1 / nk
-        out[name]["roofline"] = {"bound": "hbm", "kernel": "fused_sweep",
+        alg = (2 if two else 1) * 12 * esz * s.N
+        kms = k1 / (n_two if two else nk)
+        out[name]["roofline"] = {"bound": "hbm", "kernel": "fused2_sweep" if two else "fused_sweep",
                                  "achieved": round(alg / (kms * 1e-3) / 1e9, 1),
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": round(alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                                  "alg_bytes_per_launch": alg, "avg_launch_ms": round(kms, 4)}
+        if two:
+            out[name]["roofline"]["steps_per_launch"] = 2
     N = r32.N
     f32 = out["fp32_mixed"]
     tile = f32["tile"]
-    traffic, tsrc = traffic_from_profiles(
-        f"fused_sweep<float, {tile['tx']}, {tile['ty']}, {tile['kc']}>", "f32", f"{n}x{n}x{n}",
-        tile["cfg"])
+    if "two_step_cfg" in tile:
+        traffic, tsrc = traffic_from_profiles(
+            f"fused2_sweep<float, {tile['tx']}, {tile['ty']}>", "f32", f"{n}x{n}x{n}", "f2")
+    else:
+        traffic, tsrc = traffic_from_profiles(
+            f"fused_sweep<float, {tile['tx']}, {tile['ty']}, {tile['kc']}>", "f32",
+            f"{n}x{n}x{n}", tile["cfg"])
     roof = dict(f32["roofline"], traffic=traffic,
                 traffic_source=f"profiles/{tsrc}" if tsrc else None)
     for s in runs.values():
@@ -1006,6 +1036,10 @@ def main():
     ap.add_argument("--ncav", type=int, default=64, help="batch: number of cavities")
     ap.add_argument("--nreq", type=int, default=24, help="serve: number of requests")
     ap.add_argument("--chunk", type=int, default=256, help="serve: steps per batch call")
+    ap.add_argument("--all-batched", action="store_true",
+                    help="serve: every request in the batched launch (no solves stepped alone)")
+    ap.add_argument("--served-only", action="store_true",
+                    help="serve: skip the one-at-a-time FP64 baseline (no speedup reported)")
     ap.add_argument("--one-step", action="store_true",
                     help="batch: one-step launches only (no two-step pairs)")
     ap.add_argument("--kc2", type=int, default=0,

@@ -252,13 +252,14 @@ int fit_set_kernel(fit_ctx *ctx, int32_t kernel);
  * for the grid); fit_get_tile reports the one in use and its tile sizes. */
 enum { FIT_NTILE = 12 };
 int fit_set_tile(fit_ctx *ctx, int32_t cfg);
-/* Addressing of the fused sweep's lean path (no reference counterpart: a
- * layout choice of this library).  0 (default): one buffer resource per field
- * while a field spans less than 2^32 bytes, else, in FP32, one per field
- * component based at each chunk (any grid fit_create accepts: fewer than
- * 2^31 cells); 1: per component
- * and chunk on every FP32 grid (the same results; lets small grids exercise
- * that path).  FP64 contexts ignore it. */
+/* Addressing of the fused sweeps (no reference counterpart: a layout choice
+ * of this library).  0 (default): one buffer resource per field while a
+ * field spans less than 2^32 bytes, else, in FP32, one per field component
+ * based at each chunk (any grid fit_create accepts: fewer than 2^31 cells) in
+ * the one-step sweep's lean path and in the two-step sweep, whose 64 x 8
+ * shape is then the only one; 1: per component and chunk on every FP32 grid
+ * (the same results; lets small grids exercise that path).  FP64 contexts
+ * ignore it. */
 int fit_set_chunk_rsrc(fit_ctx *ctx, int32_t on);
 int fit_get_tile(fit_ctx *ctx, int32_t *cfg, int32_t *tx, int32_t *ty, int32_t *kc);
 /* Autotuner: time every tile configuration of the fused sweep on this

@@ -181,3 +181,33 @@ def test_two_step_tuner_keeps_state():
     pb._advance(6)
     for f in "EH":
         assert rel(getattr(pa, f).toarray(), getattr(pb, f).toarray()) <= TOL_KERNEL, f
+
+
+@pytest.mark.parametrize("name,shape,bl,bh,solids,nonuniform",
+                         [c for c in CASES if c[0] in ("pml_conductive", "periodic_lean",
+                                                       "pmc_lean_nonuniform", "mixed_odd_nz")],
+                         ids=["pml_conductive", "periodic_lean", "pmc_lean_nonuniform",
+                              "mixed_odd_nz"])
+def test_two_step_per_component_resources(name, shape, bl, bh, solids, nonuniform):
+    """The per-component, chunk-based buffer resources of the two-step sweep
+    (what fields of 2^32 bytes or more take), forced on small grids with
+    fit_set_chunk_rsrc: against the FP32 oracle and the one-step sweep."""
+    from wakis_amd import _lib
+
+    nsteps = 7
+    print(f"[case] two-step per-component resources {name} {shape}", flush=True)
+    osv, p2 = _pair(shape, bl, bh, solids, nonuniform, "fused2")
+    _, p1 = _pair(shape, bl, bh, solids, nonuniform, "fused")
+    p2.ctx.set_chunk_rsrc(1)
+    assert p2.ctx.kernel_in_use() == _lib.KERNEL_FUSED2
+    assert p2.ctx.get_tile2()[0] == 0  # 64 x 8: the shape with per-component resources
+    for _ in range(nsteps):
+        osv.one_step()
+    p2._advance(nsteps)
+    p1._advance(nsteps)
+    assert p2.ctx.launch_counts() == (1, 3)
+    for f in "EHJ":
+        o = getattr(osv, f).toarray()
+        a2 = getattr(p2, f).toarray()
+        a1 = getattr(p1, f).toarray()
+        assert rel(a2, o) <= TOL_ORACLE and rel(a2, a1) <= TOL_KERNEL, (f, rel(a2, o), rel(a2, a1))

@@ -1263,6 +1263,7 @@ int fit_set_tile(fit_ctx *c, int32_t cfg) {
 int fit_set_chunk_rsrc(fit_ctx *c, int32_t on) {
   if (!c) return fail(FIT_EINVAL, "null context");
   c->chunk_rsrc = on != 0;
+  c->f2_def = -1;  // the two-step default shape depends on it
   return 0;
 }
 

@@ -89,7 +89,10 @@ __host__ __device__ inline void chunk2_span(const Fused2Args<T> &f, int c, int &
 // window cells' true planes) lies in 1 .. nz-2, so validity is constant and
 // the boundary masks reduce to per-cell 0/1 factors fixed over the march
 // (the same expressions, with fewer instructions per plane).
-template <typename T, int TX, int TY, bool NTS, bool LEAN>
+// PCR: fields addressed with one buffer resource per component, based at
+// the chunk's lowest plane (fields of 2^32 bytes or more), as the one-step
+// sweep's lean path does; else one resource per field.
+template <typename T, int TX, int TY, bool NTS, bool LEAN, bool PCR = false>
 __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int tile,
                                             Smem2<T, TX, TY> &sm, T *__restrict__ smt) {
   using W = Win2<TX, TY>;
@@ -164,16 +167,28 @@ __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int ti
   };
 
   // ---- loads: CDNA buffer instructions ----
-  // One resource per array (the fields span < 2^32 bytes: fused2_supported),
-  // a per-lane byte offset and the component offset c N sizeof(T) as the
-  // uniform one.  Planes are clamped to the grid, so every access is in
-  // range; values at clamped planes are never used (their u and w are 0).
+  // One resource per array (the fields span < 2^32 bytes), a per-lane byte
+  // offset and the component offset c N sizeof(T) as the uniform one; or
+  // (PCR) one resource per field component based at plane kbr, the lowest
+  // the chunk touches, with plane offsets (k - kbr) PB.  Planes are clamped
+  // to the grid, so every access is in range; values at clamped planes are
+  // never used (their u and w are 0).
   constexpr uint32_t SZ = sizeof(T);
-  const uint32_t NB = (uint32_t)a.N * SZ, NB2 = 2 * NB;
+  const uint32_t NB = (uint32_t)a.N * SZ;
   const uint32_t PB = (uint32_t)plane * SZ;
   const int64_t FB = 3 * a.N * (int64_t)SZ;
-  const Rsrc rE = make_rsrc(a.E, FB), rH = make_rsrc(a.H, FB);
-  const Rsrc rEn = make_rsrc(f.En, FB), rHn = make_rsrc(f.Hn, FB);
+  const int kbr = PCR ? max(k0 - 3, 0) : 0;
+  const int64_t ebr = (int64_t)kbr * plane;
+  const int64_t cspan = PCR ? ((int64_t)a.N - ebr) * SZ : FB;
+  const int64_t rspan = cspan < 0xFFFFFFFFll ? cspan : 0xFFFFFFFFll;
+  auto comp_rsrc = [&](const T *p, int c) __attribute__((always_inline)) -> Rsrc {
+    return PCR ? make_rsrc(p + (int64_t)c * a.N + ebr, rspan) : make_rsrc(p, FB);
+  };
+  const uint32_t co1 = PCR ? 0u : NB, co2 = PCR ? 0u : 2 * NB;
+  const Rsrc rE0 = comp_rsrc(a.E, 0), rE1 = comp_rsrc(a.E, 1), rE2 = comp_rsrc(a.E, 2);
+  const Rsrc rH0 = comp_rsrc(a.H, 0), rH1 = comp_rsrc(a.H, 1), rH2 = comp_rsrc(a.H, 2);
+  const Rsrc rEn0 = comp_rsrc(f.En, 0), rEn1 = comp_rsrc(f.En, 1), rEn2 = comp_rsrc(f.En, 2);
+  const Rsrc rHn0 = comp_rsrc(f.Hn, 0), rHn1 = comp_rsrc(f.Hn, 1), rHn2 = comp_rsrc(f.Hn, 2);
   const bool hasmat = a.mat != nullptr;
   const Rsrc rM = make_rsrc(hasmat ? (const void *)a.mat : (const void *)g.Lx, hasmat ? a.N : 1);
   const Rsrc rG = make_rsrc(g.Lx, a.geo_bytes);
@@ -202,18 +217,18 @@ __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int ti
   };
   auto load_S = [&](int k, SPk &p) __attribute__((always_inline)) {
     const int kr = clampk(k + cdk);
-    const uint32_t v = cb + (uint32_t)kr * PB;
-    p.e[0] = bload<T>(rE, v, 0);
-    p.e[1] = bload<T>(rE, v, NB);
-    p.e[2] = bload<T>(rE, v, NB2);
+    const uint32_t v = cb + (uint32_t)(kr - kbr) * PB;
+    p.e[0] = bload<T>(rE0, v, 0);
+    p.e[1] = bload<T>(rE1, v, co1);
+    p.e[2] = bload<T>(rE2, v, co2);
     p.lz = bload<T>(rG, oLz + (uint32_t)kr * SZ, 0);
   };
   auto load_C = [&](int k, CPk &p) __attribute__((always_inline)) {
     const int kr = clampk(k + cdk);
-    const uint32_t v = cb + (uint32_t)kr * PB;
-    p.h[0] = bload<T>(rH, v, 0);
-    p.h[1] = bload<T>(rH, v, NB);
-    p.h[2] = bload<T>(rH, v, NB2);
+    const uint32_t v = cb + (uint32_t)(kr - kbr) * PB;
+    p.h[0] = bload<T>(rH0, v, 0);
+    p.h[1] = bload<T>(rH1, v, co1);
+    p.h[2] = bload<T>(rH2, v, co2);
     p.m = hasmat ? bload_u8(rM, (uint32_t)ccol + (uint32_t)kr * (uint32_t)plane, 0) : 0;
     p.iax = bload<T>(rG, oiAx + (uint32_t)kr * rowx, 0);
     p.iay = bload<T>(rG, oiAy + (uint32_t)kr * rowy, 0);
@@ -349,10 +364,10 @@ __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int ti
         w[1][cl] = (vf * (fm1 * ctLy)) * h1;
         w[2][cl] = (vf * (fm2 * tlzp)) * h2;
         if (rout && km >= k0 && km < k1) {
-          const uint32_t v = cb + (uint32_t)km * PB;
-          bstore(rHn, v, 0, h0, NTS);
-          bstore(rHn, v, NB, h1, NTS);
-          bstore(rHn, v, NB2, h2, NTS);
+          const uint32_t v = cb + (uint32_t)(km - kbr) * PB;
+          bstore(rHn0, v, 0, h0, NTS);
+          bstore(rHn1, v, co1, h1, NTS);
+          bstore(rHn2, v, co2, h2, NTS);
         }
       }
     }
@@ -371,10 +386,10 @@ __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int ti
         const T jx = f.js2 * (mt[6] * E1p[0]);
         const T jy = f.js2 * (mt[7] * E1p[1]);
         const T jz = (cbeam && b2on) ? b2p : f.js2 * (mt[8] * E1p[2]);
-        const uint32_t v = cb + (uint32_t)km * PB;
-        bstore(rEn, v, 0, E1p[0] + mt[3] * (ax * curl_w(w, wm, 0) - jx), NTS);
-        bstore(rEn, v, NB, E1p[1] + mt[4] * (ay * curl_w(w, wm, 1) - jy), NTS);
-        bstore(rEn, v, NB2, E1p[2] + mt[5] * (az * curl_w(w, wm, 2) - jz), NTS);
+        const uint32_t v = cb + (uint32_t)(km - kbr) * PB;
+        bstore(rEn0, v, 0, E1p[0] + mt[3] * (ax * curl_w(w, wm, 0) - jx), NTS);
+        bstore(rEn1, v, co1, E1p[1] + mt[4] * (ay * curl_w(w, wm, 1) - jy), NTS);
+        bstore(rEn2, v, co2, E1p[2] + mt[5] * (az * curl_w(w, wm, 2) - jz), NTS);
       }
     }
 
@@ -424,7 +439,7 @@ __device__ __forceinline__ void fused2_tile(const Fused2Args<T> &f, const int ti
 // slot is its parity) and each thread carries its cell's H1, E1 and update
 // data of the two previous planes.  Same per-cell expressions as
 // fused2_tile.
-template <typename T, int TX, int TY, bool NTS>
+template <typename T, int TX, int TY, bool NTS, bool PCR = false>
 __device__ __forceinline__ void fused2_tile_lean2b(const Fused2Args<T> &f, const int tile,
                                                    Smem2<T, TX, TY> &sm, T *__restrict__ smt) {
   using W = Win2<TX, TY>;
@@ -479,11 +494,23 @@ __device__ __forceinline__ void fused2_tile_lean2b(const Fused2Args<T> &f, const
 
   // ---- loads (as fused2_tile; every plane read lies inside the grid) ----
   constexpr uint32_t SZ = sizeof(T);
-  const uint32_t NB = (uint32_t)a.N * SZ, NB2 = 2 * NB;
+  const uint32_t NB = (uint32_t)a.N * SZ;
   const uint32_t PB = (uint32_t)plane * SZ;
   const int64_t FB = 3 * a.N * (int64_t)SZ;
-  const Rsrc rE = make_rsrc(a.E, FB), rH = make_rsrc(a.H, FB);
-  const Rsrc rEn = make_rsrc(f.En, FB), rHn = make_rsrc(f.Hn, FB);
+  // fields: as fused2_tile (PCR: per-component resources based at plane
+  // kbr = k0 - 3, the lowest a lean chunk touches, which is >= 1)
+  const int kbr = PCR ? k0 - 3 : 0;
+  const int64_t ebr = (int64_t)kbr * plane;
+  const int64_t cspan = PCR ? ((int64_t)a.N - ebr) * SZ : FB;
+  const int64_t rspan = cspan < 0xFFFFFFFFll ? cspan : 0xFFFFFFFFll;
+  auto comp_rsrc = [&](const T *p, int c) __attribute__((always_inline)) -> Rsrc {
+    return PCR ? make_rsrc(p + (int64_t)c * a.N + ebr, rspan) : make_rsrc(p, FB);
+  };
+  const uint32_t co1 = PCR ? 0u : NB, co2 = PCR ? 0u : 2 * NB;
+  const Rsrc rE0 = comp_rsrc(a.E, 0), rE1 = comp_rsrc(a.E, 1), rE2 = comp_rsrc(a.E, 2);
+  const Rsrc rH0 = comp_rsrc(a.H, 0), rH1 = comp_rsrc(a.H, 1), rH2 = comp_rsrc(a.H, 2);
+  const Rsrc rEn0 = comp_rsrc(f.En, 0), rEn1 = comp_rsrc(f.En, 1), rEn2 = comp_rsrc(f.En, 2);
+  const Rsrc rHn0 = comp_rsrc(f.Hn, 0), rHn1 = comp_rsrc(f.Hn, 1), rHn2 = comp_rsrc(f.Hn, 2);
   const bool hasmat = a.mat != nullptr;
   const Rsrc rM = make_rsrc(hasmat ? (const void *)a.mat : (const void *)g.Lx, hasmat ? a.N : 1);
   const Rsrc rG = make_rsrc(g.Lx, a.geo_bytes);
@@ -494,7 +521,8 @@ __device__ __forceinline__ void fused2_tile_lean2b(const Fused2Args<T> &f, const
   const Rsrc rB1 = make_rsrc(b1on ? f.beam1 : g.Lz, (int64_t)nz * SZ);
   const Rsrc rB2 = make_rsrc(b2on ? f.beam2 : g.Lz, (int64_t)nz * SZ);
   const uint32_t cb = (uint32_t)ccol * SZ;
-  const uint32_t cbk = cb + (uint32_t)cdk * PB;  // plane offset of the carry
+  // plane offset of the carry, relative to the resources' base plane
+  const uint32_t cbk = cb + (uint32_t)(cdk - kbr) * PB;
   const uint32_t cmk = (uint32_t)ccol + (uint32_t)cdk * (uint32_t)plane;
   const uint32_t oLz = gofs(g.Lz) + (uint32_t)cdk * SZ, otLz = gofs(g.tLz) + (uint32_t)cdk * SZ;
   const uint32_t rowx = (uint32_t)ny * SZ, rowy = (uint32_t)nx * SZ;
@@ -514,16 +542,16 @@ __device__ __forceinline__ void fused2_tile_lean2b(const Fused2Args<T> &f, const
   // window plane k (true plane k + cdk, inside the grid for a lean chunk)
   auto load_S = [&](int k, SPk &p) __attribute__((always_inline)) {
     const uint32_t v = cbk + (uint32_t)k * PB;
-    p.e[0] = bload<T>(rE, v, 0);
-    p.e[1] = bload<T>(rE, v, NB);
-    p.e[2] = bload<T>(rE, v, NB2);
+    p.e[0] = bload<T>(rE0, v, 0);
+    p.e[1] = bload<T>(rE1, v, co1);
+    p.e[2] = bload<T>(rE2, v, co2);
     p.lz = bload<T>(rG, oLz + (uint32_t)k * SZ, 0);
   };
   auto load_C = [&](int k, CPk &p) __attribute__((always_inline)) {
     const uint32_t v = cbk + (uint32_t)k * PB;
-    p.h[0] = bload<T>(rH, v, 0);
-    p.h[1] = bload<T>(rH, v, NB);
-    p.h[2] = bload<T>(rH, v, NB2);
+    p.h[0] = bload<T>(rH0, v, 0);
+    p.h[1] = bload<T>(rH1, v, co1);
+    p.h[2] = bload<T>(rH2, v, co2);
     p.m = hasmat ? bload_u8(rM, cmk + (uint32_t)k * (uint32_t)plane, 0) : 0;
     p.iax = bload<T>(rG, oiAx + (uint32_t)k * rowx, 0);
     p.iay = bload<T>(rG, oiAy + (uint32_t)k * rowy, 0);
@@ -620,10 +648,10 @@ __device__ __forceinline__ void fused2_tile_lean2b(const Fused2Args<T> &f, const
         w[1][cl] = wyF * h1;
         w[2][cl] = (pm2 * Pq.tlz) * h2;
         if (rout && km >= k0 && km < k1) {
-          const uint32_t v = cb + (uint32_t)km * PB;
-          bstore(rHn, v, 0, h0, NTS);
-          bstore(rHn, v, NB, h1, NTS);
-          bstore(rHn, v, NB2, h2, NTS);
+          const uint32_t v = cb + (uint32_t)(km - kbr) * PB;
+          bstore(rHn0, v, 0, h0, NTS);
+          bstore(rHn1, v, co1, h1, NTS);
+          bstore(rHn2, v, co2, h2, NTS);
         }
       }
     }
@@ -662,10 +690,10 @@ __device__ __forceinline__ void fused2_tile_lean2b(const Fused2Args<T> &f, const
         const T jx = f.js2 * (mt[6] * Pq.E1[0]);
         const T jy = f.js2 * (mt[7] * Pq.E1[1]);
         const T jz = (cbeam && b2on) ? Pq.b2 : f.js2 * (mt[8] * Pq.E1[2]);
-        const uint32_t v = cb + (uint32_t)km * PB;
-        bstore(rEn, v, 0, Pq.E1[0] + mt[3] * (ax * curl_w(w, wm, 0) - jx), NTS);
-        bstore(rEn, v, NB, Pq.E1[1] + mt[4] * (ay * curl_w(w, wm, 1) - jy), NTS);
-        bstore(rEn, v, NB2, Pq.E1[2] + mt[5] * (itAz2 * curl_w(w, wm, 2) - jz), NTS);
+        const uint32_t v = cb + (uint32_t)(km - kbr) * PB;
+        bstore(rEn0, v, 0, Pq.E1[0] + mt[3] * (ax * curl_w(w, wm, 0) - jx), NTS);
+        bstore(rEn1, v, co1, Pq.E1[1] + mt[4] * (ay * curl_w(w, wm, 1) - jy), NTS);
+        bstore(rEn2, v, co2, Pq.E1[2] + mt[5] * (itAz2 * curl_w(w, wm, 2) - jz), NTS);
       }
     }
     put_u(sm.u0[PA], S.e, S.lz);  // plane k+2 (u0 of plane k is no longer read)
@@ -718,18 +746,18 @@ extern __shared__ double fit_dyn_lds[];
 // kernels, so that each is compiled for its own register need: block b takes
 // chunk b / (ntx nty) of the lean chunks (nremap blocks, under the XCD
 // remap), or of the others, which follow them in the chunk order.
-template <typename T, int TX, int TY, bool NTS>
+template <typename T, int TX, int TY, bool NTS, bool PCR = false>
 __global__ void __launch_bounds__((TX + 4) * (TY + 4)) fused2_sweep(Fused2Args<T> f) {
   __shared__ Smem2<T, TX, TY> sm;
   T *smt = (T *)fit_dyn_lds;
-  fused2_tile_lean2b<T, TX, TY, NTS>(f, xcd_tile(blockIdx.x, f.nremap), sm, smt);
+  fused2_tile_lean2b<T, TX, TY, NTS, PCR>(f, xcd_tile(blockIdx.x, f.nremap), sm, smt);
 }
 
-template <typename T, int TX, int TY, bool NTS>
+template <typename T, int TX, int TY, bool NTS, bool PCR = false>
 __global__ void __launch_bounds__((TX + 4) * (TY + 4)) fused2_sweep_faces(Fused2Args<T> f) {
   __shared__ Smem2<T, TX, TY> sm;
   T *smt = (T *)fit_dyn_lds;
-  fused2_tile<T, TX, TY, NTS, false>(f, f.nremap + blockIdx.x, sm, smt);
+  fused2_tile<T, TX, TY, NTS, false, PCR>(f, f.nremap + blockIdx.x, sm, smt);
 }
 
 // Tile of the two-step sweep: 64 x 8 columns, a (68 x 12)-cell window, one
@@ -929,18 +957,39 @@ inline void f2_dims(int cfg, int &tx, int &ty) {
 // arguments: FP32, nx >= TX and ny >= max(16, TY) (a window cell's flat
 // carry then stays within one row and one plane), fields of less than 2^32
 // bytes, J zero or sigma*E (no J buffer), the kernels spill-free.
+// Whether the two-step sweep addresses the fields with per-component
+// resources: fields of 2^32 bytes or more, or forced (fit_set_chunk_rsrc).
+template <typename T>
+inline bool fused2_pcr(const StepArgs<T> &a) {
+  return !a.buf32 || a.pcr;
+}
+
 template <typename T, int TX, int TY>
 inline bool fused2_supported_t(const StepArgs<T> &a) {
   if constexpr (!cfg2_fits<T, TX, TY>()) {
     (void)a;
     return false;
   } else {
-    return a.jmode != J_BUFFER && a.buf32 && a.nx >= TX && a.ny >= 16 && a.ny >= TY &&
-           a.nz >= 2 &&
-           kernel_spill_free((const void *)fused2_sweep<T, TX, TY, false>) &&
-           kernel_spill_free((const void *)fused2_sweep<T, TX, TY, true>) &&
-           kernel_spill_free((const void *)fused2_sweep_faces<T, TX, TY, false>) &&
-           kernel_spill_free((const void *)fused2_sweep_faces<T, TX, TY, true>);
+    if (a.jmode == J_BUFFER || a.nx < TX || a.ny < 16 || a.ny < TY || a.nz < 2) return false;
+    if (!fused2_pcr(a))
+      return kernel_spill_free((const void *)fused2_sweep<T, TX, TY, false>) &&
+             kernel_spill_free((const void *)fused2_sweep<T, TX, TY, true>) &&
+             kernel_spill_free((const void *)fused2_sweep_faces<T, TX, TY, false>) &&
+             kernel_spill_free((const void *)fused2_sweep_faces<T, TX, TY, true>);
+    // per-component resources: the 64 x 8 shape (the one large grids take),
+    // when the planes of the longest chunk (384) and its halo, the material
+    // map and the metric tables fit 32-bit offsets
+    if constexpr (TX == 64 && TY == 8) {
+      constexpr int64_t LIM = 0xFFFFFFFFll;
+      const int64_t pbytes = (int64_t)a.nx * a.ny * (int64_t)sizeof(T);
+      return a.geo_bytes < LIM && a.N < LIM && (int64_t)(384 + 16) * pbytes < LIM &&
+             kernel_spill_free((const void *)fused2_sweep<T, TX, TY, false, true>) &&
+             kernel_spill_free((const void *)fused2_sweep<T, TX, TY, true, true>) &&
+             kernel_spill_free((const void *)fused2_sweep_faces<T, TX, TY, false, true>) &&
+             kernel_spill_free((const void *)fused2_sweep_faces<T, TX, TY, true, true>);
+    } else {
+      return false;
+    }
   }
 }
 
@@ -1027,6 +1076,25 @@ inline bool launch_fused2_t(const StepArgs<T> &a, T *En, T *Hn, const T *beam1, 
     }
     f.nb = ncol * f.nchunk;
     const int nfaces = f.nb - f.nremap;
+    if constexpr (TX == 64 && TY == 8) {
+      if (fused2_pcr(a)) {
+        if (f.nremap > 0) {
+          if (a.nt)
+            hipLaunchKernelGGL((fused2_sweep<T, TX, TY, true, true>), dim3(f.nremap), dim3(NT),
+                               lds, s, f);
+          else
+            hipLaunchKernelGGL((fused2_sweep<T, TX, TY, false, true>), dim3(f.nremap), dim3(NT),
+                               lds, s, f);
+        }
+        if (a.nt)
+          hipLaunchKernelGGL((fused2_sweep_faces<T, TX, TY, true, true>), dim3(nfaces), dim3(NT),
+                             lds, s, f);
+        else
+          hipLaunchKernelGGL((fused2_sweep_faces<T, TX, TY, false, true>), dim3(nfaces),
+                             dim3(NT), lds, s, f);
+        return true;
+      }
+    }
     if (f.nremap > 0) {
       if (a.nt)
         hipLaunchKernelGGL((fused2_sweep<T, TX, TY, true>), dim3(f.nremap), dim3(NT), lds, s, f);
