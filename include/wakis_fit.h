@@ -283,7 +283,8 @@ enum {
   FIT_TILE2_32x16 = 1,
   FIT_TILE2_32x8 = 2,
   FIT_TILE2_16x16 = 3,
-  FIT_NTILE2 = 4
+  FIT_TILE2_60x12 = 4,
+  FIT_NTILE2 = 5
 };
 int fit_set_tile2(fit_ctx *ctx, int32_t cfg);
 int fit_get_tile2(fit_ctx *ctx, int32_t *cfg, int32_t *tx, int32_t *ty);

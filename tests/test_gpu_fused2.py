@@ -127,7 +127,8 @@ SHAPE_CASES = [
 ]
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3], ids=["64x8", "32x16", "32x8", "16x16"])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4],
+                         ids=["64x8", "32x16", "32x8", "16x16", "60x12"])
 @pytest.mark.parametrize("name,shape,bl,bh,solids,nonuniform", SHAPE_CASES,
                          ids=[c[0] for c in SHAPE_CASES])
 def test_two_step_tile_shapes(cfg, name, shape, bl, bh, solids, nonuniform):
@@ -141,7 +142,7 @@ def test_two_step_tile_shapes(cfg, name, shape, bl, bh, solids, nonuniform):
     osv, p2 = _pair(shape, bl, bh, solids, nonuniform, "fused2")
     _, p1 = _pair(shape, bl, bh, solids, nonuniform, "fused")
     p2.ctx.set_tile2(cfg)
-    tx = (64, 32, 32, 16)[cfg]
+    tx = (64, 32, 32, 16, 60)[cfg]
     if shape[0] >= tx:
         assert p2.ctx.get_tile2()[0] == cfg
     else:  # falls back to the modelled default
@@ -188,19 +189,23 @@ def test_two_step_tuner_keeps_state():
                                                        "pmc_lean_nonuniform", "mixed_odd_nz")],
                          ids=["pml_conductive", "periodic_lean", "pmc_lean_nonuniform",
                               "mixed_odd_nz"])
-def test_two_step_per_component_resources(name, shape, bl, bh, solids, nonuniform):
+@pytest.mark.parametrize("cfg", [-1, 4], ids=["default", "set_60x12"])
+def test_two_step_per_component_resources(cfg, name, shape, bl, bh, solids, nonuniform):
     """The per-component, chunk-based buffer resources of the two-step sweep
     (what fields of 2^32 bytes or more take), forced on small grids with
-    fit_set_chunk_rsrc: against the FP32 oracle and the one-step sweep."""
+    fit_set_chunk_rsrc: against the FP32 oracle and the one-step sweep.  The
+    64 x 8 shape is the one built with them; a shape set that is not falls
+    back to it."""
     from wakis_amd import _lib
 
     nsteps = 7
-    print(f"[case] two-step per-component resources {name} {shape}", flush=True)
+    print(f"[case] two-step per-component resources {cfg} {name} {shape}", flush=True)
     osv, p2 = _pair(shape, bl, bh, solids, nonuniform, "fused2")
     _, p1 = _pair(shape, bl, bh, solids, nonuniform, "fused")
     p2.ctx.set_chunk_rsrc(1)
+    p2.ctx.set_tile2(cfg)
     assert p2.ctx.kernel_in_use() == _lib.KERNEL_FUSED2
-    assert p2.ctx.get_tile2()[0] == 0  # 64 x 8: the shape with per-component resources
+    assert p2.ctx.get_tile2()[0] == 0
     for _ in range(nsteps):
         osv.one_step()
     p2._advance(nsteps)

@@ -524,13 +524,16 @@ __device__ __forceinline__ void fused2_tile_lean2b(const Fused2Args<T> &f, const
   // plane offset of the carry, relative to the resources' base plane
   const uint32_t cbk = cb + (uint32_t)(cdk - kbr) * PB;
   const uint32_t cmk = (uint32_t)ccol + (uint32_t)cdk * (uint32_t)plane;
-  const uint32_t oLz = gofs(g.Lz) + (uint32_t)cdk * SZ, otLz = gofs(g.tLz) + (uint32_t)cdk * SZ;
+  // metric tables and beam lines: one lane offset per table layout (its
+  // plane part added per plane), the table's offset in the geometry block
+  // in the wave-uniform SGPR operand
+  const uint32_t vZ = (uint32_t)cdk * SZ;
   const uint32_t rowx = (uint32_t)ny * SZ, rowy = (uint32_t)nx * SZ;
-  const uint32_t oiAx = gofs(g.iAx) + (uint32_t)cj * SZ + (uint32_t)cdk * rowx;
-  const uint32_t oitAx = gofs(g.itAx) + (uint32_t)cj * SZ + (uint32_t)cdk * rowx;
-  const uint32_t oiAy = gofs(g.iAy) + (uint32_t)ci * SZ + (uint32_t)cdk * rowy;
-  const uint32_t oitAy = gofs(g.itAy) + (uint32_t)ci * SZ + (uint32_t)cdk * rowy;
-  const uint32_t ob = (uint32_t)cdk * SZ;
+  const uint32_t vAx = (uint32_t)cj * SZ + (uint32_t)cdk * rowx;
+  const uint32_t vAy = (uint32_t)ci * SZ + (uint32_t)cdk * rowy;
+  const uint32_t oLz = gofs(g.Lz), otLz = gofs(g.tLz);
+  const uint32_t oiAx = gofs(g.iAx), oitAx = gofs(g.itAx);
+  const uint32_t oiAy = gofs(g.iAy), oitAy = gofs(g.itAy);
 
   struct SPk {
     T e[3], lz;
@@ -545,7 +548,7 @@ __device__ __forceinline__ void fused2_tile_lean2b(const Fused2Args<T> &f, const
     p.e[0] = bload<T>(rE0, v, 0);
     p.e[1] = bload<T>(rE1, v, co1);
     p.e[2] = bload<T>(rE2, v, co2);
-    p.lz = bload<T>(rG, oLz + (uint32_t)k * SZ, 0);
+    p.lz = bload<T>(rG, vZ + (uint32_t)k * SZ, oLz);
   };
   auto load_C = [&](int k, CPk &p) __attribute__((always_inline)) {
     const uint32_t v = cbk + (uint32_t)k * PB;
@@ -553,13 +556,15 @@ __device__ __forceinline__ void fused2_tile_lean2b(const Fused2Args<T> &f, const
     p.h[1] = bload<T>(rH1, v, co1);
     p.h[2] = bload<T>(rH2, v, co2);
     p.m = hasmat ? bload_u8(rM, cmk + (uint32_t)k * (uint32_t)plane, 0) : 0;
-    p.iax = bload<T>(rG, oiAx + (uint32_t)k * rowx, 0);
-    p.iay = bload<T>(rG, oiAy + (uint32_t)k * rowy, 0);
-    p.itax = bload<T>(rG, oitAx + (uint32_t)k * rowx, 0);
-    p.itay = bload<T>(rG, oitAy + (uint32_t)k * rowy, 0);
-    p.tlz = bload<T>(rG, otLz + (uint32_t)k * SZ, 0);
-    p.b1 = bload<T>(rB1, ob + (uint32_t)k * SZ, 0);
-    p.b2 = bload<T>(rB2, ob + (uint32_t)k * SZ, 0);
+    const uint32_t vax = vAx + (uint32_t)k * rowx, vay = vAy + (uint32_t)k * rowy;
+    const uint32_t vz = vZ + (uint32_t)k * SZ;
+    p.iax = bload<T>(rG, vax, oiAx);
+    p.iay = bload<T>(rG, vay, oiAy);
+    p.itax = bload<T>(rG, vax, oitAx);
+    p.itay = bload<T>(rG, vay, oitAy);
+    p.tlz = bload<T>(rG, vz, otLz);
+    p.b1 = bload<T>(rB1, vz, 0);
+    p.b2 = bload<T>(rB2, vz, 0);
   };
   auto curl_u = [&](const T (*u)[WC], const T (*un)[WC], int d)
       __attribute__((always_inline)) -> T {
@@ -747,7 +752,7 @@ extern __shared__ double fit_dyn_lds[];
 // chunk b / (ntx nty) of the lean chunks (nremap blocks, under the XCD
 // remap), or of the others, which follow them in the chunk order.
 template <typename T, int TX, int TY, bool NTS, bool PCR = false>
-__global__ void __launch_bounds__((TX + 4) * (TY + 4)) fused2_sweep(Fused2Args<T> f) {
+__global__ void __launch_bounds__((TX + 4) * (TY + 4), 4) fused2_sweep(Fused2Args<T> f) {
   __shared__ Smem2<T, TX, TY> sm;
   T *smt = (T *)fit_dyn_lds;
   fused2_tile_lean2b<T, TX, TY, NTS, PCR>(f, xcd_tile(blockIdx.x, f.nremap), sm, smt);
@@ -939,16 +944,26 @@ inline void chunk2_layout(Fused2Args<T> &f, int kc) {
 }
 
 // Single-grid tile shapes of the two-step sweep: 64 x 8 columns (a 68 x 12
-// window, 816 threads), 32 x 16 (36 x 20, 720), 32 x 8 (36 x 12, 432) or
-// 16 x 16 (20 x 20, 400).  64 x 8 has the longest contiguous rows and the
-// least time per window cell; the others leave less overhang on grids whose
-// width is not a multiple of 64, and the two small ones give more blocks
-// (and two resident per CU) on small grids.  (24 x 24 and 28 x 12 were
-// slower than one of these on every grid measured, 48^2 to 512^2 cells.)
-enum { F2_CFG_64x8 = 0, F2_CFG_32x16 = 1, F2_CFG_32x8 = 2, F2_CFG_16x16 = 3, F2_NCFG = 4 };
+// window, 816 threads), 32 x 16 (36 x 20, 720), 32 x 8 (36 x 12, 432),
+// 16 x 16 (20 x 20, 400) or 60 x 12 (64 x 16, 1024).  64 x 8 has the longest
+// contiguous rows and, with 60 x 12, the least time per window cell; 60 x 12
+// has the fewest window cells per column (one 64-cell window row per wave,
+// 16 waves, four per SIMD) but overhangs grids whose width is not near a
+// multiple of 60; 32-wide shapes leave less overhang on grids whose width is
+// not a multiple of 64, and the two small ones give more blocks (and two
+// resident per CU) on small grids.  (24 x 24, 28 x 12 and 60 x 8 were slower
+// than one of these on every grid measured, 48^2 to 512^2 cells.)
+enum {
+  F2_CFG_64x8 = 0,
+  F2_CFG_32x16 = 1,
+  F2_CFG_32x8 = 2,
+  F2_CFG_16x16 = 3,
+  F2_CFG_60x12 = 4,
+  F2_NCFG = 5
+};
 
 inline void f2_dims(int cfg, int &tx, int &ty) {
-  static const int d[F2_NCFG][2] = {{64, 8}, {32, 16}, {32, 8}, {16, 16}};
+  static const int d[F2_NCFG][2] = {{64, 8}, {32, 16}, {32, 8}, {16, 16}, {60, 12}};
   tx = d[cfg][0];
   ty = d[cfg][1];
 }
@@ -1000,6 +1015,7 @@ inline bool fused2_supported(const StepArgs<T> &a, int cfg) {
     case F2_CFG_32x16: return fused2_supported_t<T, 32, 16>(a);
     case F2_CFG_32x8: return fused2_supported_t<T, 32, 8>(a);
     case F2_CFG_16x16: return fused2_supported_t<T, 16, 16>(a);
+    case F2_CFG_60x12: return fused2_supported_t<T, 60, 12>(a);
     default: return false;
   }
 }
@@ -1118,6 +1134,7 @@ inline bool launch_fused2(const StepArgs<T> &a, int cfg, T *En, T *Hn, const T *
     case F2_CFG_32x16: return launch_fused2_t<T, 32, 16>(a, En, Hn, beam1, beam2, js1, js2, s);
     case F2_CFG_32x8: return launch_fused2_t<T, 32, 8>(a, En, Hn, beam1, beam2, js1, js2, s);
     case F2_CFG_16x16: return launch_fused2_t<T, 16, 16>(a, En, Hn, beam1, beam2, js1, js2, s);
+    case F2_CFG_60x12: return launch_fused2_t<T, 60, 12>(a, En, Hn, beam1, beam2, js1, js2, s);
     default: return false;
   }
 }
@@ -1160,6 +1177,7 @@ inline double fused2_time(const StepArgs<T> &a, int cfg, int dev) {
     case F2_CFG_32x16: return fused2_time_t<T, 32, 16>(a, dev, 1.17);
     case F2_CFG_32x8: return fused2_time_t<T, 32, 8>(a, dev, 1.08);
     case F2_CFG_16x16: return fused2_time_t<T, 16, 16>(a, dev, 1.27);
+    case F2_CFG_60x12: return fused2_time_t<T, 60, 12>(a, dev, 0.97);
     default: return 1e300;
   }
 }
