@@ -731,7 +731,8 @@ def bench_serve(args):
         print(f"[bench] req {r.rid:3d} N {r.cells:9d} Nt {r.steps:6d} arr {r.arrival:6.2f} "
               f"adm {r.admitted:6.2f} fin {r.finished:6.2f} prep {r.prep_s:5.2f}s "
               f"post {r.finish_s:5.2f}s fp64 {d:5.2f}s dZ {errs[-1]:.2e} "
-              f"finite {bool(np.isfinite(r.result['Z']).all())}/{bool(np.isfinite(ref['Z']).all())}",
+              f"finite {bool(np.isfinite(r.result['Z']).all())}/{bool(np.isfinite(ref['Z']).all())} "
+              f"tile2 {r.result.get('tile2')}",
               file=sys.stderr, flush=True)
     if not args.no_plain:
         # torch carries its own HIP runtime, which cannot start on a device

@@ -68,14 +68,16 @@ def test_serve_results_equal_alone(budget):
         assert np.array_equal(r.result["Z"], alone["Z"]), r.rid
 
 
-def test_serve_two_step_members_equal_alone():
+@pytest.mark.parametrize("tune2", [False, True], ids=["model", "tuned"])
+def test_serve_two_step_members_equal_alone(tune2):
     """Requests wide enough for the two-step sweep (nx >= 64) are stepped by
     their own fit_step calls beside the batch; each is bit for bit the same
-    as solved alone, and the context bytes stay within the model."""
+    as solved alone (with the two-step tile shape the server's tuner chose,
+    when it tunes), and the context bytes stay within the model."""
     from wakis_amd import _lib
     from wakis_amd.serve import Request, Server, serve_cavity, solve_alone
 
-    print("[case] serve two-step members", flush=True)
+    print(f"[case] serve two-step members tune2={tune2}", flush=True)
     specs = [("rect", 0.050, 0.060, 0.015, 8e5, None, 0.0),
              ("cyl", 0.050, 0.070, 0.020, 9e5, (3.0, 1e-3), 0.05)]
     reqs = []
@@ -91,13 +93,13 @@ def test_serve_two_step_members_equal_alone():
     tile = 2
     clock = VClock()
     srv = Server(dtype=np.float32, budget=288e9, chunk_steps=40, graph=True, group=8, tile=tile,
-                 clock=clock)
+                 clock=clock, tune2=tune2)
     clock.server = srv
     done = srv.run(reqs)
     assert max(e[8] for e in srv.log) >= 1 and max(e[1] for e in srv.log) >= 2
     for r in done:
         assert 0 < r.result["device_peak"] <= r.result["device_model"], r.rid
-        t, alone = solve_alone(r, dtype=np.float32, tile=tile)
+        t, alone = solve_alone(r, dtype=np.float32, tile=tile, tile2=r.result["tile2"])
         assert r.result["Nt"] == alone["Nt"]
         assert np.array_equal(r.result["WP"], alone["WP"]), r.rid
         assert np.array_equal(r.result["Z"], alone["Z"]), r.rid

@@ -72,6 +72,17 @@ class FakeContext:
         two = self.esz == 4 and self.nx >= 64 and self.ny >= 16
         return _lib.KERNEL_FUSED2 if two else _lib.KERNEL_FUSED
 
+    # the two-step tile shape: the tuner (no allocation; the second buffers
+    # are already held) picks shape 4 and is recorded with the steps taken
+    # before it
+    def tune2(self, reps):
+        self.tuned = getattr(self, "tuned", []) + [getattr(self, "steps", 0)]
+        self.tile2 = 4
+        return np.array([1.0, 2.0, 2.0, 2.0, 0.5])
+
+    def get_tile2(self):
+        return (getattr(self, "tile2", 0), 64, 8)
+
     def step(self, n, conductive=False):
         assert not self.closed
         self.second_buffers()
@@ -268,10 +279,13 @@ def test_model_terms():
 
 
 @pytest.mark.timeout(300)
-def test_two_step_members_step_alone_in_even_calls(fake_device):
+@pytest.mark.parametrize("tune2", [False, True], ids=["model", "tuned"])
+def test_two_step_members_step_alone_in_even_calls(fake_device, tune2):
     # requests wide enough for the two-step sweep are stepped by their own
     # fit_step calls of min(chunk, steps left): every call but the last is
-    # even, so pairs of steps start at even indices as in a solve alone
+    # even, so pairs of steps start at even indices as in a solve alone; with
+    # tune2 each is tuned once, before its first step, and its result records
+    # the shape
     reqs = serve.make_requests(3, seed=7, median_cells=8e5, sigma_ln=0.0, clip=(8e5, 8e5),
                                slab_prob=0.0, wakelength=0.05)
     small = _burst(2)
@@ -282,7 +296,8 @@ def test_two_step_members_step_alone_in_even_calls(fake_device):
         r.workload["solver"]["n_pml"] = 4
         r.arrival = 0.0
     clock = VClock()
-    srv = serve.Server(dtype=np.float32, budget=288e9, chunk_steps=50, group=8, clock=clock)
+    srv = serve.Server(dtype=np.float32, budget=288e9, chunk_steps=50, group=8, clock=clock,
+                       tune2=tune2)
     clock.server = srv
     made = []
     orig = FakeContext.__init__
@@ -301,9 +316,14 @@ def test_two_step_members_step_alone_in_even_calls(fake_device):
     assert len(solo) == 3 and len(made) == 5
     for c in solo:
         assert all(n == 50 for n in c.calls[:-1]) and 0 < c.calls[-1] <= 50
+        assert getattr(c, "tuned", []) == ([0] if tune2 else [])
     for c in made:
         if c not in solo:
             assert not hasattr(c, "calls")  # batched members are stepped by the batch
+            assert not hasattr(c, "tuned")
+    wide = [r for r in done if r.rid < 10]
+    assert all(r.result["tile2"] == (4 if tune2 else 0) for r in wide)
+    assert all(r.result["tile2"] is None for r in done if r.rid >= 10)
     assert max(e[8] for e in srv.log) >= 1
     assert sum(e[7] for e in srv.log) == sum(r.cells * r.steps for r in done)
     assert fake_device.peak <= srv.peak_admitted
