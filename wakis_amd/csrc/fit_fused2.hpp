@@ -1144,8 +1144,10 @@ inline bool launch_fused2(const StepArgs<T> &a, int cfg, T *En, T *Hn, const T *
 // blocks, a round lasting kc + 3 planes of one block, and a block's plane
 // time is its window cells x the shape's measured time per window cell
 // relative to 64 x 8 (launches on a 256 x 256 x 300 grid, no overhang:
-// 64 x 8 0.378 ms, 32 x 16 0.390, 32 x 8 0.432, 16 x 16 0.472) x its
-// resident blocks per CU (blocks sharing a CU share its throughput).  On
+// 64 x 8 0.378 ms, 32 x 16 0.390, 32 x 8 0.432, 16 x 16 0.472; 60 x 12
+// from grids it tiles well) x the blocks that share a CU in a round (the
+// resident blocks per CU, or fewer when a round holds fewer blocks than
+// there are CUs).  On
 // large grids this is the window cells over the covered (overhanging)
 // columns; on small ones the rounds and chunk lengths decide.
 template <typename T, int TX, int TY>
@@ -1166,7 +1168,9 @@ inline double fused2_time_t(const StepArgs<T> &a, int dev, double rel) {
   const long long blocks = (long long)ncol * ((a.nz + kc - 1) / kc);
   const int xslots = slots / 8 > 0 ? slots / 8 : 1;
   const long long rounds = ((blocks + 7) / 8 + xslots - 1) / xslots;
-  return (double)rounds * (kc + 3) * NT * rel * bpc;
+  const long long per_round = blocks < slots ? blocks : slots;
+  const long long share = std::min<long long>(bpc, (per_round + ncu - 1) / ncu);
+  return (double)rounds * (kc + 3) * NT * rel * (double)share;
   }
 }
 
