@@ -136,6 +136,54 @@ def traffic_from_profiles(kernel, dtype, grid, tile):
     return None, None
 
 
+def cpu_baseline_large(n_sample=128, steps=40):
+    """The CPU oracle on an n_sample^3 sub-box of the stratified box with the
+    same layering (16 z-layers, the central diagonal-tensor block, 12-cell
+    PML on six faces): the full 1024^3 grid is out of reach on the host (its
+    two CSC operators alone would take ~200 GB), so the rate per cell-step is
+    measured on the sub-box and reported as extrapolated."""
+    from scipy.constants import epsilon_0 as eps_0
+
+    from oracle.fit_oracle import OGrid, OSolver
+    from wakis_amd.workloads import stratified_box
+
+    n = n_sample
+    wl = stratified_box(n=n, half=0.256 * n / 1024)
+    g, sv = wl["grid"], wl["solver"]
+    iso = [(r, m) for r, m in g["solids"] if not isinstance(m, dict)]
+    tensors = [(r, m) for r, m in g["solids"] if isinstance(m, dict)]
+
+    def mask(region):
+        m = np.zeros((n, n, n), bool)
+        m[region] = True
+        return m
+
+    osv = OSolver(OGrid(x=g["x"], y=g["y"], z=g["z"]), cfln=sv["cfln"], bc_low=sv["bc_low"],
+                  bc_high=sv["bc_high"], solids=[(mask(r), m) for r, m in iso], bg=sv["bg"],
+                  dtype=np.float64, n_pml=sv["n_pml"], use_sibc=sv["use_sibc"])
+    for region, mat in tensors:  # the reference's route: ieps edits + update_tensors
+        for d, e in zip("xyz", mat["eps"]):
+            osv.ieps[region + (d,)] = np.float64(1.0) / (e * eps_0)
+    osv.update_tensors("ieps")
+    osv.step_0 = True
+    osv.one_step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        osv.one_step()
+    dt = time.perf_counter() - t0
+    return {
+        "value": osv.N * steps / dt / 1e9,
+        "unit": "Gcell-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"extrapolated: oracle FP64 scipy.sparse one_step on a {n}^3 sub-box of the "
+                  f"stratified box (same 16 layers, tensor block and 12-cell PML), {steps} steps, "
+                  f"{dt:.2f} s on 1 core of {cpu_model()}; the rate per cell-step applies to the "
+                  "1024^3 grid, whose operators do not fit the host",
+        **cpu_info(),
+    }
+
+
 def cpu_baseline(nz_sample=32, steps=150):
     """The CPU oracle (scipy.sparse FP64, single-threaded csc/csr mat-vec, the
     reference's arithmetic path) on a z-slab of the same pillbox grid."""
@@ -990,6 +1038,11 @@ def bench_large(args):
                 traffic_source=f"profiles/{tsrc}" if tsrc else None)
     for s in runs.values():
         s.ctx.close()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_large()
+        print(f"[bench] large cpu baseline {cpu['value']:.4f} Gcell-steps/s ({cpu['sample']})",
+              file=sys.stderr, flush=True)
     return {
         "metric": "Gcell-steps/s",
         "value": f32["gcell_steps_per_s"],
@@ -1009,6 +1062,7 @@ def bench_large(args):
                                f"{args.steps} steps, FP32 (mixed) vs FP64",
                    "runs": out, "fp32_vs_fp64": errs, "finite": finite},
         "roofline": roof,
+        "cpu_baseline": cpu,
     }
 
 
