@@ -124,6 +124,9 @@ SHAPE_CASES = [
     ("mixed_narrow_lean", (50, 24, 41), ["periodic", "pec", "pec"],
      ["periodic", "pmc", "pml"], True, True),
     ("pml_conductive", (64, 32, 44), ["pec", "pec", "pml"], ["pec", "pec", "pml"], True, False),
+    # 61 wide: 60 x 12 overhangs by one column, 64 x 8 does not apply
+    ("overhang_61", (61, 26, 38), ["pec", "periodic", "pec"], ["pmc", "periodic", "pml"], True,
+     True),
 ]
 
 
