@@ -777,6 +777,11 @@ constexpr bool cfg2_fits() {
          sizeof(Smem2<T, TX, TY>) + MAX_MATERIALS * MAT_STRIDE * sizeof(T) <= 160 * 1024;
 }
 
+// Planes the z chunks of a two-step launch split: with 4-plane chunks at
+// the z faces (when at least one 16-plane chunk fits between them) the
+// planes between, else all nz planes.
+inline int f2_span(int nz) { return nz - 8 >= 16 ? nz - 8 : nz; }
+
 // Planes per chunk: blocks run in rounds of `slots` (an eighth per XCD); a
 // round lasts about a chunk plus its 3 planes of pipeline fill; the least
 // modelled time rounds x (kc + 3) among a few lengths.
@@ -1073,7 +1078,7 @@ inline bool launch_fused2_t(const StepArgs<T> &a, T *En, T *Hn, const T *beam1, 
     const size_t lds = fused_mt_bytes<T>(a.nmat);
     const int slots = resident_slots(dev, (const void *)fused2_sweep<T, TX, TY, false>, NT, lds);
     const int ncol = f.ntx * f.nty;
-    f.kc = pick_kc2(ncol, a.nz, slots);
+    f.kc = pick_kc2(ncol, f2_span(a.nz), slots);
     // 4-plane chunks at the z faces take the general path (fused2_sweep_faces),
     // the chunks between (all lean: every plane their outputs depend on lies
     // in 1 .. nz-2) the two-barrier lean path (fused2_sweep); a grid too short
@@ -1164,8 +1169,9 @@ inline double fused2_time_t(const StepArgs<T> &a, int dev, double rel) {
                                    fused_mt_bytes<T>(a.nmat));
   const int bpc = slots / ncu > 0 ? slots / ncu : 1;
   const int ncol = ((a.nx + TX - 1) / TX) * ((a.ny + TY - 1) / TY);
-  const int kc = pick_kc2(ncol, a.nz, slots);
-  const long long blocks = (long long)ncol * ((a.nz + kc - 1) / kc);
+  const int span = f2_span(a.nz);
+  const int kc = pick_kc2(ncol, span, slots);
+  const long long blocks = (long long)ncol * ((span + kc - 1) / kc);
   const int xslots = slots / 8 > 0 ? slots / 8 : 1;
   const long long rounds = ((blocks + 7) / 8 + xslots - 1) / xslots;
   const long long per_round = blocks < slots ? blocks : slots;
