@@ -784,22 +784,31 @@ inline int f2_span(int nz) { return nz - 8 >= 16 ? nz - 8 : nz; }
 
 // Planes per chunk: blocks run in rounds of `slots` (an eighth per XCD); a
 // round lasts about a chunk plus its 3 planes of pipeline fill; the least
-// modelled time rounds x (kc + 3) among a few lengths.
+// modelled time rounds x (kc + 3) among a few lengths and the equal splits
+// of the span (m chunks of ceil(nz / m) planes, so that no short last chunk
+// leaves its round's other blocks running alone), 16 to 384 planes; ties go
+// to the longer chunk.
 inline int pick_kc2(int ncol, int nz, int slots) {
   static const int cand[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384};
-  int best = 64;
+  int best = 16;
   double bt = 1e300;
   const int xslots = slots / 8 > 0 ? slots / 8 : 1;
-  for (int kc : cand) {
-    if (kc > nz && kc != cand[0]) break;
+  auto consider = [&](int kc) {
     const long long blocks = (long long)ncol * ((nz + kc - 1) / kc);
     const long long per_xcd = (blocks + 7) / 8;
     const long long rounds = (per_xcd + xslots - 1) / xslots;
     const double t = (double)rounds * (kc + 3);
-    if (t < bt) {
+    if (t < bt || (t == bt && kc > best)) {
       bt = t;
       best = kc;
     }
+  };
+  for (int kc : cand)
+    if (kc <= nz || kc == cand[0]) consider(kc);
+  for (int m = 1; m <= 64; ++m) {
+    const int kc = (nz + m - 1) / m;
+    if (kc < 16) break;
+    if (kc <= 384) consider(kc);
   }
   return best;
 }
