@@ -765,6 +765,20 @@ __global__ void __launch_bounds__((TX + 4) * (TY + 4)) fused2_sweep_faces(Fused2
   fused2_tile<T, TX, TY, NTS, false, PCR>(f, f.nremap + blockIdx.x, sm, smt);
 }
 
+// Both kinds of chunk in one launch: blocks below nremap run the lean
+// chunks (XCD-remapped), the rest the z-face chunks, so that face blocks
+// take the CUs the lean blocks leave idle instead of waiting for a second
+// launch.  Used where it compiles spill-free (else the two kernels above).
+template <typename T, int TX, int TY, bool NTS, bool PCR = false>
+__global__ void __launch_bounds__((TX + 4) * (TY + 4), 4) fused2_sweep_all(Fused2Args<T> f) {
+  __shared__ Smem2<T, TX, TY> sm;
+  T *smt = (T *)fit_dyn_lds;
+  if ((int)blockIdx.x < f.nremap)
+    fused2_tile_lean2b<T, TX, TY, NTS, PCR>(f, xcd_tile(blockIdx.x, f.nremap), sm, smt);
+  else
+    fused2_tile<T, TX, TY, NTS, false, PCR>(f, blockIdx.x, sm, smt);
+}
+
 // Tile of the two-step sweep: 64 x 8 columns, a (68 x 12)-cell window, one
 // thread per window cell (816 threads).  (64 x 7 balances the waves over the
 // SIMDs, 12 against 13, and ran 10 % faster per plane, but 8 % slower per
@@ -1124,6 +1138,14 @@ inline bool launch_fused2_t(const StepArgs<T> &a, T *En, T *Hn, const T *beam1, 
                              dim3(NT), lds, s, f);
         return true;
       }
+    }
+    if (kernel_spill_free((const void *)fused2_sweep_all<T, TX, TY, true>) &&
+        kernel_spill_free((const void *)fused2_sweep_all<T, TX, TY, false>)) {
+      if (a.nt)
+        hipLaunchKernelGGL((fused2_sweep_all<T, TX, TY, true>), dim3(f.nb), dim3(NT), lds, s, f);
+      else
+        hipLaunchKernelGGL((fused2_sweep_all<T, TX, TY, false>), dim3(f.nb), dim3(NT), lds, s, f);
+      return true;
     }
     if (f.nremap > 0) {
       if (a.nt)
