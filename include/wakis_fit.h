@@ -202,8 +202,9 @@ int fit_monitor_read(fit_ctx *ctx, int32_t f, int32_t comp, double *host);
  *   H -= dt*M_H E;  E += dt*(M_E H - ieps*J);  J = sigma*E if conductive.
  * conductive != 0 selects J := sigma*E after each step.  Asynchronous.
  * With FIT_KERNEL_AUTO or FIT_KERNEL_FUSED2, pairs of steps run as ONE
- * two-step sweep (E and H read and written once per two steps) on FP32
- * contexts with nx >= 16 and ny >= 16 (fields below 2^32 bytes), when J is
+ * two-step sweep (E and H read and written once per two steps) on
+ * contexts with nx >= 16 and ny >= 16 (FP32; FP64 with fields below 2^32
+ * bytes), when J is
  * 0 or sigma*E
  * (not FIT_J_BUFFER) and no field-monitor update is staged for the pair;
  * every other step runs the one-step sweep.  Each step's beam value, J and
@@ -254,12 +255,13 @@ enum { FIT_NTILE = 12 };
 int fit_set_tile(fit_ctx *ctx, int32_t cfg);
 /* Addressing of the fused sweeps (no reference counterpart: a layout choice
  * of this library).  0 (default): one buffer resource per field while a
- * field spans less than 2^32 bytes, else, in FP32, one per field component
- * based at each chunk (any grid fit_create accepts: fewer than 2^31 cells) in
- * the one-step sweep's lean path and in the two-step sweep, whose 64 x 8
- * shape is then the only one; 1: per component and chunk on every FP32 grid
- * (the same results; lets small grids exercise that path).  FP64 contexts
- * ignore it. */
+ * field spans less than 2^32 bytes, else one per field component based at
+ * each chunk (any grid fit_create accepts: fewer than 2^31 cells): in FP32
+ * in the one-step sweep's lean path and in the two-step sweep, whose 64 x 8
+ * shape is then the only one; in FP64 in the two-step sweep, whose 32 x 8
+ * shape is then the only one (the FP64 one-step sweep keeps 64-bit
+ * addresses); 1: per component and chunk on every grid (the same results;
+ * lets small grids exercise that path). */
 int fit_set_chunk_rsrc(fit_ctx *ctx, int32_t on);
 int fit_get_tile(fit_ctx *ctx, int32_t *cfg, int32_t *tx, int32_t *ty, int32_t *kc);
 /* Autotuner: time every tile configuration of the fused sweep on this
@@ -271,8 +273,9 @@ int fit_tune(fit_ctx *ctx, int32_t reps, double *ms);
  * default (the shape of least modelled cost: window cells per tile cell,
  * tile overhang over the grid and measured time per window cell).  A shape
  * the grid cannot take (nx < tile width, ny < max(16, tile height)) falls
- * back to the default.  Every shape evaluates the same per-cell
- * expressions.  fit_get_tile2
+ * back to the default, and so does one not built for the context's
+ * precision (FP64 contexts: 32 x 8 and 16 x 16 only).  Every shape
+ * evaluates the same per-cell expressions.  fit_get_tile2
  * reports the shape the next two-step sweep will use (-1: none, 0 x 0);
  * fit_tune2 times every shape that applies on this context's grid and data
  * (reps two-step launches each; the state is not changed), selects the

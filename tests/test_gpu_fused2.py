@@ -1,8 +1,10 @@
-"""The two-step fused sweep (two leapfrog steps per launch, FP32; used by
-fit_step for pairs of steps on grids with nx >= 24 and ny >= 16, in tiles of
-64 x 8, 32 x 16 or 24 x 24 columns) against the FP32 oracle (relative L2
-<= 1e-4, the FP32 bar of the other parity tests) and against the one-step fused sweep on the same context setup
-(<= 1e-5: the two kernels evaluate the same per-cell expressions).  Cases:
+"""The two-step fused sweep (two leapfrog steps per launch; used by fit_step
+for pairs of steps on grids with nx >= 16 and ny >= 16, in tiles of 64 x 8,
+32 x 16, 32 x 8, 16 x 16 or 60 x 12 columns in FP32, 32 x 8 or 16 x 16 in
+FP64) against the oracle (relative L2 <= 1e-4 in FP32, <= 1e-10 in FP64:
+the bars of the other parity tests) and against the one-step fused sweep on
+the same context setup (<= 1e-5 in FP32, <= 1e-12 in FP64: the two kernels
+evaluate the same per-cell expressions).  Cases:
 every boundary kind, the helical x/y wrap across tile and grid edges, odd
 step counts (a single step after the pairs), odd plane counts, non-uniform
 spacing, conductive solids with PML (J = sigma*E in both steps), and the
@@ -45,13 +47,13 @@ def _solids(shape):
     ]
 
 
-def _pair(shape, bl, bh, solids, nonuniform, kernel):
-    osv, psv = make_pair(*shape, bl, bh, dtype=np.float32, solids=_solids(shape) if solids else (),
+def _pair(shape, bl, bh, solids, nonuniform, kernel, dtype=np.float32):
+    osv, psv = make_pair(*shape, bl, bh, dtype=dtype, solids=_solids(shape) if solids else (),
                          bg="pec" if solids else (1.0, 1.0), n_pml=3, nonuniform=nonuniform,
                          init=False, kernel=kernel, seed=3)
     rng = np.random.default_rng(17)
     for name in ("E", "H"):  # J stays 0 (J mode zero / sigma*E)
-        a = rng.standard_normal(3 * osv.N).astype(np.float32)
+        a = rng.standard_normal(3 * osv.N).astype(dtype)
         getattr(osv, name).fromarray(a)
         getattr(psv, name).fromarray(a)
     return osv, psv
@@ -219,3 +221,129 @@ def test_two_step_per_component_resources(cfg, name, shape, bl, bh, solids, nonu
         a2 = getattr(p2, f).toarray()
         a1 = getattr(p1, f).toarray()
         assert rel(a2, o) <= TOL_ORACLE and rel(a2, a1) <= TOL_KERNEL, (f, rel(a2, o), rel(a2, a1))
+
+
+# ---------------- FP64 ----------------
+
+TOL_ORACLE64 = 1e-10
+TOL_KERNEL64 = 1e-12
+
+
+@pytest.mark.parametrize("name,shape,bl,bh,solids,nonuniform", CASES, ids=[c[0] for c in CASES])
+def test_two_step_fp64_matches_oracle_and_one_step(name, shape, bl, bh, solids, nonuniform):
+    """The FP64 two-step sweep (default shape) against the FP64 oracle and the
+    FP64 one-step sweep: three two-step launches and one single step."""
+    from wakis_amd import _lib
+
+    nsteps = 7
+    print(f"[case] two-step fp64 {name} {shape}", flush=True)
+    osv, p2 = _pair(shape, bl, bh, solids, nonuniform, "fused2", np.float64)
+    _, p1 = _pair(shape, bl, bh, solids, nonuniform, "fused", np.float64)
+    assert p2.ctx.kernel_in_use() == _lib.KERNEL_FUSED2
+    assert p2.ctx.get_tile2()[0] in (2, 3)
+    for _ in range(nsteps):
+        osv.one_step()
+    p2._advance(nsteps)
+    p1._advance(nsteps)
+    assert p2.ctx.launch_counts() == (1, 3), p2.ctx.launch_counts()
+    assert p1.ctx.launch_counts() == (nsteps, 0)
+    errs = {f: (rel(getattr(p2, f).toarray(), getattr(osv, f).toarray()),
+                rel(getattr(p2, f).toarray(), getattr(p1, f).toarray())) for f in "EHJ"}
+    print(errs, flush=True)
+    for f, (eo, ek) in errs.items():
+        assert eo <= TOL_ORACLE64 and ek <= TOL_KERNEL64, (f, eo, ek)
+
+
+@pytest.mark.parametrize("cfg", [0, 2, 3], ids=["64x8_falls_back", "32x8", "16x16"])
+@pytest.mark.parametrize("name,shape,bl,bh,solids,nonuniform", SHAPE_CASES,
+                         ids=[c[0] for c in SHAPE_CASES])
+def test_two_step_fp64_tile_shapes(cfg, name, shape, bl, bh, solids, nonuniform):
+    """The FP64 shapes (32 x 8, 16 x 16) against the FP64 oracle and the
+    one-step sweep; an FP32-only shape set on an FP64 context falls back to
+    the modelled default."""
+    from wakis_amd import _lib
+
+    nsteps = 9
+    print(f"[case] two-step fp64 shape {cfg} {name} {shape}", flush=True)
+    osv, p2 = _pair(shape, bl, bh, solids, nonuniform, "fused2", np.float64)
+    _, p1 = _pair(shape, bl, bh, solids, nonuniform, "fused", np.float64)
+    p2.ctx.set_tile2(cfg)
+    got = p2.ctx.get_tile2()[0]
+    assert got == cfg if cfg in (2, 3) else got in (2, 3), got
+    assert p2.ctx.kernel_in_use() == _lib.KERNEL_FUSED2
+    for _ in range(nsteps):
+        osv.one_step()
+    p2._advance(nsteps)
+    p1._advance(nsteps)
+    assert p2.ctx.launch_counts() == (1, 4)
+    errs = {f: (rel(getattr(p2, f).toarray(), getattr(osv, f).toarray()),
+                rel(getattr(p2, f).toarray(), getattr(p1, f).toarray())) for f in "EHJ"}
+    print(errs, flush=True)
+    for f, (eo, ek) in errs.items():
+        assert eo <= TOL_ORACLE64 and ek <= TOL_KERNEL64, (f, eo, ek)
+
+
+def test_two_step_fp64_wakesolve_matches_oracle():
+    """The FP64 wakesolve of a conductive cavity with z PML, beam on the axis
+    and the 3 x 3 E_z probe on the two-step path: fields, WP and Z against
+    the FP64 oracle and the one-step path."""
+    import wakis_amd as wa  # noqa: F401
+
+    shape = (48, 20, 48)
+    Nx, Ny, Nz = shape
+    wall = ~block_mask(shape, (4, 3, 0), (Nx - 4, Ny - 3, Nz))
+    kw = dict(wakelength=0.12, q=1e-9, sigmaz=0.01, beta=1.0)
+    res = {}
+    for kernel in ("fused2", "fused"):
+        osv, psv = make_pair(*shape, ["pec", "pec", "pml"], ["pec", "pec", "pml"],
+                             dtype=np.float64, solids=[(wall, [1.0, 1.0, 1e3])], n_pml=5,
+                             init=False, kernel=kernel, wake_kw=kw)
+        print(f"[case] two-step fp64 wakesolve kernel={kernel}", flush=True)
+        psv.wakesolve(compute_plane="longitudinal")
+        res[kernel] = (psv, osv)
+    p2, osv = res["fused2"]
+    p1, _ = res["fused"]
+    assert p2.ctx.launch_counts()[1] > p2.Nt // 4
+    osv.wakesolve(compute_plane="longitudinal")
+    errs = {"E": rel(p2.E.toarray(), osv.E.toarray()), "H": rel(p2.H.toarray(), osv.H.toarray()),
+            "WP": rel(p2.wake.WP, osv.wake.WP), "Z": rel(p2.wake.Z, osv.wake.Z),
+            "E_one_step": rel(p2.E.toarray(), p1.E.toarray()),
+            "WP_one_step": rel(p2.wake.WP, p1.wake.WP)}
+    print(errs, flush=True)
+    assert max(errs["E"], errs["H"], errs["WP"], errs["Z"]) <= TOL_ORACLE64, errs
+    assert max(errs["E_one_step"], errs["WP_one_step"]) <= TOL_KERNEL64, errs
+
+
+@pytest.mark.parametrize("name,shape,bl,bh,solids,nonuniform",
+                         [c for c in CASES if c[0] in ("pml_conductive", "periodic_lean",
+                                                       "pmc_lean_nonuniform", "mixed_odd_nz")],
+                         ids=["pml_conductive", "periodic_lean", "pmc_lean_nonuniform",
+                              "mixed_odd_nz"])
+@pytest.mark.parametrize("cfg", [-1, 3], ids=["default", "set_16x16"])
+def test_two_step_fp64_per_component_resources(cfg, name, shape, bl, bh, solids, nonuniform):
+    """The FP64 two-step sweep with per-component, chunk-based buffer
+    resources (what FP64 fields of 2^32 bytes or more take), forced on small
+    grids: against the FP64 oracle and the one-step sweep.  The 32 x 8 shape
+    is the FP64 one built with them; a shape set that is not falls back to
+    it."""
+    from wakis_amd import _lib
+
+    nsteps = 7
+    print(f"[case] two-step fp64 per-component resources {cfg} {name} {shape}", flush=True)
+    osv, p2 = _pair(shape, bl, bh, solids, nonuniform, "fused2", np.float64)
+    _, p1 = _pair(shape, bl, bh, solids, nonuniform, "fused", np.float64)
+    p2.ctx.set_chunk_rsrc(1)
+    p2.ctx.set_tile2(cfg)
+    assert p2.ctx.kernel_in_use() == _lib.KERNEL_FUSED2
+    assert p2.ctx.get_tile2()[0] == 2
+    for _ in range(nsteps):
+        osv.one_step()
+    p2._advance(nsteps)
+    p1._advance(nsteps)
+    assert p2.ctx.launch_counts() == (1, 3)
+    for f in "EHJ":
+        o = getattr(osv, f).toarray()
+        a2 = getattr(p2, f).toarray()
+        a1 = getattr(p1, f).toarray()
+        assert rel(a2, o) <= TOL_ORACLE64 and rel(a2, a1) <= TOL_KERNEL64, \
+            (f, rel(a2, o), rel(a2, a1))

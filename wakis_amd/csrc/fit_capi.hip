@@ -1299,9 +1299,10 @@ int fit_set_tile2(fit_ctx *c, int32_t cfg) {
 int fit_get_tile2(fit_ctx *c, int32_t *cfg, int32_t *tx, int32_t *ty) {
   if (!c || !cfg) return fail(FIT_EINVAL, "null argument");
   *cfg = -1;
-  if (c->dtype == FIT_F32 && c->geo_set && c->mat_set && c->dt_set) {
+  if (c->geo_set && c->mat_set && c->dt_set) {
     if (int rc = set_device(c)) return rc;
-    *cfg = f2_cfg_of<float>(c, make_args<float>(c));
+    *cfg = c->dtype == FIT_F32 ? f2_cfg_of<float>(c, make_args<float>(c))
+                               : f2_cfg_of<double>(c, make_args<double>(c));
   }
   int a = 0, b = 0;
   if (*cfg >= 0) fit::f2_dims(*cfg, a, b);
@@ -1313,12 +1314,11 @@ int fit_get_tile2(fit_ctx *c, int32_t *cfg, int32_t *tx, int32_t *ty) {
 int fit_tune2(fit_ctx *c, int32_t reps, double *ms) {
   if (int rc = check_ready(c)) return rc;
   if (reps < 1) return fail(FIT_EINVAL, "reps < 1");
-  if (c->dtype != FIT_F32) return fail(FIT_ESTATE, "the two-step sweep is FP32 only");
   bool fused = false;
   if (int rc = want_fused(c, &fused)) return rc;
   if (!fused) return fail(FIT_ESTATE, "the fused sweeps are not in use");
   if (int rc = set_device(c)) return rc;
-  return tune2_impl<float>(c, reps, ms);
+  return c->dtype == FIT_F32 ? tune2_impl<float>(c, reps, ms) : tune2_impl<double>(c, reps, ms);
 }
 
 int fit_kernel_in_use(fit_ctx *c, int32_t *kernel) {
@@ -1327,9 +1327,11 @@ int fit_kernel_in_use(fit_ctx *c, int32_t *kernel) {
   if (int rc = want_fused(c, &fused)) return rc;
   *kernel = fused ? FIT_KERNEL_FUSED : FIT_KERNEL_SPLIT;
   if (fused && (c->kernel == FIT_KERNEL_AUTO || c->kernel == FIT_KERNEL_FUSED2) &&
-      c->dtype == FIT_F32 && c->geo_set && c->mat_set && c->dt_set) {
+      c->geo_set && c->mat_set && c->dt_set) {
     if (int rc = set_device(c)) return rc;
-    if (fit::fused2_supported<float>(make_args<float>(c))) *kernel = FIT_KERNEL_FUSED2;
+    const bool two = c->dtype == FIT_F32 ? fit::fused2_supported<float>(make_args<float>(c))
+                                         : fit::fused2_supported<double>(make_args<double>(c));
+    if (two) *kernel = FIT_KERNEL_FUSED2;
   }
   return 0;
 }

@@ -53,6 +53,10 @@ struct Smem2 {
   T w2[2][3][W::WC];  // w of H2, planes k-1 and k-2
 };
 
+// Waves per SIMD the lean kernels are compiled for: 4 in FP32 (at most 128
+// VGPRs), 2 in FP64 (at most 256; a block of at most 8 waves).
+#define F2_WAVES(T) (sizeof(T) == 4 ? 4 : 2)
+
 template <typename T>
 struct Fused2Args {
   StepArgs<T> a;           // a.E, a.H: buffers read; a.step: index of the first step
@@ -752,7 +756,7 @@ extern __shared__ double fit_dyn_lds[];
 // chunk b / (ntx nty) of the lean chunks (nremap blocks, under the XCD
 // remap), or of the others, which follow them in the chunk order.
 template <typename T, int TX, int TY, bool NTS, bool PCR = false>
-__global__ void __launch_bounds__((TX + 4) * (TY + 4), 4) fused2_sweep(Fused2Args<T> f) {
+__global__ void __launch_bounds__((TX + 4) * (TY + 4), F2_WAVES(T)) fused2_sweep(Fused2Args<T> f) {
   __shared__ Smem2<T, TX, TY> sm;
   T *smt = (T *)fit_dyn_lds;
   fused2_tile_lean2b<T, TX, TY, NTS, PCR>(f, xcd_tile(blockIdx.x, f.nremap), sm, smt);
@@ -770,7 +774,7 @@ __global__ void __launch_bounds__((TX + 4) * (TY + 4)) fused2_sweep_faces(Fused2
 // take the CUs the lean blocks leave idle instead of waiting for a second
 // launch.  Used where it compiles spill-free (else the two kernels above).
 template <typename T, int TX, int TY, bool NTS, bool PCR = false>
-__global__ void __launch_bounds__((TX + 4) * (TY + 4), 4) fused2_sweep_all(Fused2Args<T> f) {
+__global__ void __launch_bounds__((TX + 4) * (TY + 4), F2_WAVES(T)) fused2_sweep_all(Fused2Args<T> f) {
   __shared__ Smem2<T, TX, TY> sm;
   T *smt = (T *)fit_dyn_lds;
   if ((int)blockIdx.x < f.nremap)
@@ -782,12 +786,16 @@ __global__ void __launch_bounds__((TX + 4) * (TY + 4), 4) fused2_sweep_all(Fused
 // Tile of the two-step sweep: 64 x 8 columns, a (68 x 12)-cell window, one
 // thread per window cell (816 threads).  (64 x 7 balances the waves over the
 // SIMDs, 12 against 13, and ran 10 % faster per plane, but 8 % slower per
-// cell.)  FP32 only: an FP64 window does not fit the LDS.
+// cell.)
 constexpr int F2_TX = 64, F2_TY = 8;
 
+// Shapes built per precision: every shape in FP32; in FP64 those of at most
+// 512 window cells (32 x 8, 16 x 16), whose block of at most 8 waves runs at
+// two waves per SIMD with up to 256 VGPRs (the FP64 body holds 189-210) and
+// whose rings (77-83 KB) leave room for the material table in the LDS.
 template <typename T, int TX, int TY>
 constexpr bool cfg2_fits() {
-  return sizeof(T) == 4 &&
+  return (sizeof(T) == 4 || (sizeof(T) == 8 && (TX + 4) * (TY + 4) <= 512)) &&
          sizeof(Smem2<T, TX, TY>) + MAX_MATERIALS * MAT_STRIDE * sizeof(T) <= 160 * 1024;
 }
 
@@ -1002,6 +1010,13 @@ inline void f2_dims(int cfg, int &tx, int &ty) {
 // bytes, J zero or sigma*E (no J buffer), the kernels spill-free.
 // Whether the two-step sweep addresses the fields with per-component
 // resources: fields of 2^32 bytes or more, or forced (fit_set_chunk_rsrc).
+// The shape built with per-component resources: 64 x 8 in FP32, 32 x 8 in
+// FP64 (each precision's fastest shape on large grids).
+template <typename T, int TX, int TY>
+constexpr bool f2_pcr_shape() {
+  return sizeof(T) == 4 ? (TX == 64 && TY == 8) : (TX == 32 && TY == 8);
+}
+
 template <typename T>
 inline bool fused2_pcr(const StepArgs<T> &a) {
   return !a.buf32 || a.pcr;
@@ -1019,10 +1034,10 @@ inline bool fused2_supported_t(const StepArgs<T> &a) {
              kernel_spill_free((const void *)fused2_sweep<T, TX, TY, true>) &&
              kernel_spill_free((const void *)fused2_sweep_faces<T, TX, TY, false>) &&
              kernel_spill_free((const void *)fused2_sweep_faces<T, TX, TY, true>);
-    // per-component resources: the 64 x 8 shape (the one large grids take),
-    // when the planes of the longest chunk (384) and its halo, the material
-    // map and the metric tables fit 32-bit offsets
-    if constexpr (TX == 64 && TY == 8) {
+    // per-component resources: one shape per precision (the one large grids
+    // take), when the planes of the longest chunk (384) and its halo, the
+    // material map and the metric tables fit 32-bit offsets
+    if constexpr (f2_pcr_shape<T, TX, TY>()) {
       constexpr int64_t LIM = 0xFFFFFFFFll;
       const int64_t pbytes = (int64_t)a.nx * a.ny * (int64_t)sizeof(T);
       return a.geo_bytes < LIM && a.N < LIM && (int64_t)(384 + 16) * pbytes < LIM &&
@@ -1120,7 +1135,7 @@ inline bool launch_fused2_t(const StepArgs<T> &a, T *En, T *Hn, const T *beam1, 
     }
     f.nb = ncol * f.nchunk;
     const int nfaces = f.nb - f.nremap;
-    if constexpr (TX == 64 && TY == 8) {
+    if constexpr (f2_pcr_shape<T, TX, TY>()) {
       if (fused2_pcr(a)) {
         if (f.nremap > 0) {
           if (a.nt)
