@@ -1079,7 +1079,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--tile", type=int, default=None, help="fused tile configuration id")
     ap.add_argument("--tile2", type=int, default=None,
-                    help="two-step tile shape (0: 64x8, 1: 32x16, 2: 32x8, 3: 16x16, 4: 60x12)")
+                    help="two-step tile shape (0: 64x8, 1: 32x16, 2: 32x8, 3: 16x16, 4: 60x12, "
+                         "5: 28x12; FP32 0-4, FP64 2, 3, 5)")
     ap.add_argument("--wakelength", type=float, default=None, help="precision: wakelength (m)")
     ap.add_argument("--workload", default="pillbox",
                     choices=["pillbox", "batch", "serve", "precision", "large", "plain", "cubic",

@@ -259,7 +259,7 @@ int fit_set_tile(fit_ctx *ctx, int32_t cfg);
  * each chunk (any grid fit_create accepts: fewer than 2^31 cells): in FP32
  * in the one-step sweep's lean path and in the two-step sweep, whose 64 x 8
  * shape is then the only one; in FP64 in the two-step sweep, whose 32 x 8
- * shape is then the only one (the FP64 one-step sweep keeps 64-bit
+ * and 28 x 12 shapes are then the only ones (the FP64 one-step sweep keeps 64-bit
  * addresses); 1: per component and chunk on every grid (the same results;
  * lets small grids exercise that path). */
 int fit_set_chunk_rsrc(fit_ctx *ctx, int32_t on);
@@ -274,7 +274,8 @@ int fit_tune(fit_ctx *ctx, int32_t reps, double *ms);
  * tile overhang over the grid and measured time per window cell).  A shape
  * the grid cannot take (nx < tile width, ny < max(16, tile height)) falls
  * back to the default, and so does one not built for the context's
- * precision (FP64 contexts: 32 x 8 and 16 x 16 only).  Every shape
+ * precision (FP32: 64 x 8, 32 x 16, 32 x 8, 16 x 16, 60 x 12; FP64: 32 x 8,
+ * 16 x 16, 28 x 12).  Every shape
  * evaluates the same per-cell expressions.  fit_get_tile2
  * reports the shape the next two-step sweep will use (-1: none, 0 x 0);
  * fit_tune2 times every shape that applies on this context's grid and data
@@ -287,7 +288,8 @@ enum {
   FIT_TILE2_32x8 = 2,
   FIT_TILE2_16x16 = 3,
   FIT_TILE2_60x12 = 4,
-  FIT_NTILE2 = 5
+  FIT_TILE2_28x12 = 5,
+  FIT_NTILE2 = 6
 };
 int fit_set_tile2(fit_ctx *ctx, int32_t cfg);
 int fit_get_tile2(fit_ctx *ctx, int32_t *cfg, int32_t *tx, int32_t *ty);

@@ -20,7 +20,7 @@ J_ZERO, J_SIGMA_E, J_BUFFER = 0, 1, 2
 BEAM_OFF, BEAM_VALUES, BEAM_GAUSS = 0, 1, 2
 KERNEL_AUTO, KERNEL_SPLIT, KERNEL_FUSED, KERNEL_FUSED2 = 0, 1, 2, 3
 NTILE = 12
-NTILE2 = 5  # FIT_NTILE2: two-step tile shapes 64x8, 32x16, 32x8, 16x16, 60x12
+NTILE2 = 6  # FIT_NTILE2: two-step tile shapes 64x8, 32x16, 32x8, 16x16, 60x12, 28x12
 
 
 class FitError(RuntimeError):
@@ -368,7 +368,8 @@ class Context:
 
     def set_tile2(self, cfg: int):
         """Tile shape of the two-step sweep (0: 64x8, 1: 32x16, 2: 32x8,
-        3: 16x16, 4: 60x12, -1: the modelled default)."""
+        3: 16x16, 4: 60x12, 5: 28x12, -1: the modelled default; FP32 builds
+        0-4, FP64 2, 3 and 5)."""
         check(self.lib.fit_set_tile2(self.h, int(cfg)), "fit_set_tile2")
 
     def get_tile2(self):

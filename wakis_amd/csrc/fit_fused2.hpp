@@ -789,13 +789,16 @@ __global__ void __launch_bounds__((TX + 4) * (TY + 4), F2_WAVES(T)) fused2_sweep
 // cell.)
 constexpr int F2_TX = 64, F2_TY = 8;
 
-// Shapes built per precision: every shape in FP32; in FP64 those of at most
-// 512 window cells (32 x 8, 16 x 16), whose block of at most 8 waves runs at
-// two waves per SIMD with up to 256 VGPRs (the FP64 body holds 189-210) and
-// whose rings (77-83 KB) leave room for the material table in the LDS.
+// Shapes built per precision: in FP32 all but 28 x 12 (slower than one of
+// the others on most grids); in FP64 those of at most 512 window cells
+// (32 x 8, 16 x 16, 28 x 12), whose block of at most 8 waves runs at two
+// waves per SIMD with up to 256 VGPRs (the FP64 body holds 189-214) and
+// whose rings (77-98 KB) leave room for the material table in the LDS.
 template <typename T, int TX, int TY>
 constexpr bool cfg2_fits() {
-  return (sizeof(T) == 4 || (sizeof(T) == 8 && (TX + 4) * (TY + 4) <= 512)) &&
+  constexpr bool shape = sizeof(T) == 4 ? !(TX == 28 && TY == 12)
+                                        : sizeof(T) == 8 && (TX + 4) * (TY + 4) <= 512;
+  return shape &&
          sizeof(Smem2<T, TX, TY>) + MAX_MATERIALS * MAT_STRIDE * sizeof(T) <= 160 * 1024;
 }
 
@@ -981,25 +984,31 @@ inline void chunk2_layout(Fused2Args<T> &f, int kc) {
 
 // Single-grid tile shapes of the two-step sweep: 64 x 8 columns (a 68 x 12
 // window, 816 threads), 32 x 16 (36 x 20, 720), 32 x 8 (36 x 12, 432),
-// 16 x 16 (20 x 20, 400) or 60 x 12 (64 x 16, 1024).  64 x 8 has the longest
+// 16 x 16 (20 x 20, 400), 60 x 12 (64 x 16, 1024) or, in FP64, 28 x 12
+// (32 x 16, 512).  64 x 8 has the longest
 // contiguous rows and, with 60 x 12, the least time per window cell; 60 x 12
 // has the fewest window cells per column (one 64-cell window row per wave,
 // 16 waves, four per SIMD) but overhangs grids whose width is not near a
 // multiple of 60; 32-wide shapes leave less overhang on grids whose width is
 // not a multiple of 64, and the two small ones give more blocks (and two
-// resident per CU) on small grids.  (24 x 24, 28 x 12 and 60 x 8 were slower
-// than one of these on every grid measured, 48^2 to 512^2 cells.)
+// resident per CU) on small grids.  (In FP32, 24 x 24, 28 x 12 and 60 x 8
+// were slower than one of these on most grids measured, 48^2 to 512^2
+// cells.)  FP64 builds the shapes of at most 512 window cells: 32 x 8,
+// 16 x 16 and 28 x 12 (32 x 16 window, 512 threads), 28 x 12 having the
+// fewest window cells per tile cell of them (1.52 against 1.69 for 32 x 8)
+// and 32 x 8 the least overhang on widths that are multiples of 32.
 enum {
   F2_CFG_64x8 = 0,
   F2_CFG_32x16 = 1,
   F2_CFG_32x8 = 2,
   F2_CFG_16x16 = 3,
   F2_CFG_60x12 = 4,
-  F2_NCFG = 5
+  F2_CFG_28x12 = 5,
+  F2_NCFG = 6
 };
 
 inline void f2_dims(int cfg, int &tx, int &ty) {
-  static const int d[F2_NCFG][2] = {{64, 8}, {32, 16}, {32, 8}, {16, 16}, {60, 12}};
+  static const int d[F2_NCFG][2] = {{64, 8}, {32, 16}, {32, 8}, {16, 16}, {60, 12}, {28, 12}};
   tx = d[cfg][0];
   ty = d[cfg][1];
 }
@@ -1010,11 +1019,12 @@ inline void f2_dims(int cfg, int &tx, int &ty) {
 // bytes, J zero or sigma*E (no J buffer), the kernels spill-free.
 // Whether the two-step sweep addresses the fields with per-component
 // resources: fields of 2^32 bytes or more, or forced (fit_set_chunk_rsrc).
-// The shape built with per-component resources: 64 x 8 in FP32, 32 x 8 in
-// FP64 (each precision's fastest shape on large grids).
+// The shapes built with per-component resources: 64 x 8 in FP32, 32 x 8 and
+// 28 x 12 in FP64 (each precision's fastest shapes on large grids).
 template <typename T, int TX, int TY>
 constexpr bool f2_pcr_shape() {
-  return sizeof(T) == 4 ? (TX == 64 && TY == 8) : (TX == 32 && TY == 8);
+  return sizeof(T) == 4 ? (TX == 64 && TY == 8)
+                        : (TX == 32 && TY == 8) || (TX == 28 && TY == 12);
 }
 
 template <typename T>
@@ -1059,6 +1069,7 @@ inline bool fused2_supported(const StepArgs<T> &a, int cfg) {
     case F2_CFG_32x8: return fused2_supported_t<T, 32, 8>(a);
     case F2_CFG_16x16: return fused2_supported_t<T, 16, 16>(a);
     case F2_CFG_60x12: return fused2_supported_t<T, 60, 12>(a);
+    case F2_CFG_28x12: return fused2_supported_t<T, 28, 12>(a);
     default: return false;
   }
 }
@@ -1186,6 +1197,7 @@ inline bool launch_fused2(const StepArgs<T> &a, int cfg, T *En, T *Hn, const T *
     case F2_CFG_32x8: return launch_fused2_t<T, 32, 8>(a, En, Hn, beam1, beam2, js1, js2, s);
     case F2_CFG_16x16: return launch_fused2_t<T, 16, 16>(a, En, Hn, beam1, beam2, js1, js2, s);
     case F2_CFG_60x12: return launch_fused2_t<T, 60, 12>(a, En, Hn, beam1, beam2, js1, js2, s);
+    case F2_CFG_28x12: return launch_fused2_t<T, 28, 12>(a, En, Hn, beam1, beam2, js1, js2, s);
     default: return false;
   }
 }
@@ -1196,7 +1208,9 @@ inline bool launch_fused2(const StepArgs<T> &a, int cfg, T *En, T *Hn, const T *
 // time is its window cells x the shape's measured time per window cell
 // relative to 64 x 8 (launches on a 256 x 256 x 300 grid, no overhang:
 // 64 x 8 0.378 ms, 32 x 16 0.390, 32 x 8 0.432, 16 x 16 0.472; 60 x 12
-// from grids it tiles well) x the blocks that share a CU in a round (the
+// from grids it tiles well; FP64, relative to 32 x 8, fitted over nine
+// grids from 48 x 48 x 300 to 256 x 256 x 768: 16 x 16 1.105, 28 x 12
+// 0.948) x the blocks that share a CU in a round (the
 // resident blocks per CU, or fewer when a round holds fewer blocks than
 // there are CUs).  On
 // large grids this is the window cells over the covered (overhanging)
@@ -1228,12 +1242,14 @@ inline double fused2_time_t(const StepArgs<T> &a, int dev, double rel) {
 
 template <typename T>
 inline double fused2_time(const StepArgs<T> &a, int cfg, int dev) {
+  constexpr bool fp64 = sizeof(T) == 8;
   switch (cfg) {
     case F2_CFG_64x8: return fused2_time_t<T, 64, 8>(a, dev, 1.0);
     case F2_CFG_32x16: return fused2_time_t<T, 32, 16>(a, dev, 1.17);
-    case F2_CFG_32x8: return fused2_time_t<T, 32, 8>(a, dev, 1.08);
-    case F2_CFG_16x16: return fused2_time_t<T, 16, 16>(a, dev, 1.27);
+    case F2_CFG_32x8: return fused2_time_t<T, 32, 8>(a, dev, fp64 ? 1.0 : 1.08);
+    case F2_CFG_16x16: return fused2_time_t<T, 16, 16>(a, dev, fp64 ? 1.105 : 1.27);
     case F2_CFG_60x12: return fused2_time_t<T, 60, 12>(a, dev, 0.97);
+    case F2_CFG_28x12: return fused2_time_t<T, 28, 12>(a, dev, 0.948);
     default: return 1e300;
   }
 }

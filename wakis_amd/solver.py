@@ -295,8 +295,8 @@ class SolverFIT3D(RoutinesMixin):
     def tune2(self, reps=3):
         """Autotune the two-step sweep's tile shape on this grid (the fields
         are not changed); returns {shape id: ms per two-step launch} for the
-        shapes that apply (0: 64x8, 1: 32x16, 2: 32x8, 3: 16x16, 4: 60x12;
-        FP64 builds 2 and 3 only)."""
+        shapes that apply (0: 64x8, 1: 32x16, 2: 32x8, 3: 16x16, 4: 60x12,
+        5: 28x12; FP32 builds 0-4, FP64 2, 3 and 5)."""
         self._require_device()
         ms = self.ctx.tune2(reps)
         return {i: float(t) for i, t in enumerate(ms) if t >= 0}
