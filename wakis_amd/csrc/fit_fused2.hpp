@@ -617,18 +617,25 @@ __device__ __forceinline__ void fused2_tile_lean2b(const Fused2Args<T> &f, const
   for (int d = 0; d < 3; ++d) E0b[d] = S.e[d];
   lzb = S.lz;
   load_S(kb + 2, S);
-  CPk CA, CB;
+  // update data (H, material, metric, beam) is loaded CD planes ahead: two
+  // in the FP64 shapes of 7 waves per CU (32 x 8, 16 x 16), which keep fewer
+  // loads in flight than FP32's 13 (pillbox, 32 x 8: 1.837 against 1.925 ms
+  // per launch); one elsewhere (FP64 28 x 12, at 256 VGPRs with two planes
+  // ahead, was 3 % slower on 512 x 512 x 200)
+  constexpr int CD = (sizeof(T) == 8 && TX * TY <= 256) ? 2 : 1;
+  CPk CA, CB, CC;
   load_C(kb, CA);
+  if constexpr (CD == 2) load_C(kb + 1, CB);
 
-  // One plane k.  Cc: update data of plane k (loaded one iteration earlier),
-  // Cn receives plane k+1's.  PH: 1 = E1(k), 2 = H2 and E2 of plane k-2 (the
+  // One plane k.  Cc: update data of plane k (loaded CD iterations
+  // earlier), Cn receives plane k+CD's.  PH: 1 = E1(k), 2 = H2 and E2 of plane k-2 (the
   // pipeline-fill iterations before the loop run fewer).  PA: ring slot of
   // plane k, (k - kb) & 1, a compile-time constant in every unrolled copy.
   auto iter = [&](const int k, const CPk &Cc, CPk &Cn, auto ph, auto pa)
       __attribute__((always_inline)) {
     constexpr int PH = decltype(ph)::value;
     constexpr int PA = decltype(pa)::value, PB1 = PA ^ 1;
-    load_C(k + 1, Cn);
+    load_C(k + CD, Cn);
     // P. H1(k) -> w1[PA]; H2(k-2) -> w2[PA]
     T H1n[3] = {T(0), T(0), T(0)};
     if (rh1) {
@@ -737,15 +744,33 @@ __device__ __forceinline__ void fused2_tile_lean2b(const Fused2Args<T> &f, const
   using P3 = std::integral_constant<int, 3>;
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
-  iter(kb, CA, CB, P0{}, S0{});      // H1(k0-2)
-  iter(kb + 1, CB, CA, P1{}, S1{});  // H1, E1 of k0-1
-  // iterations k0 .. k1+1 (H2/E2 of plane k-2 from k0-2 on; only planes in
-  // [k0, k1) are stored), plus one more when their number is odd, so that
-  // the loop body, written out twice, has no conditional part
-  const int kend = ((k1 - k0) & 1) ? k1 + 3 : k1 + 2;
-  for (int k = k0; k < kend; k += 2) {
-    iter(k, CA, CB, P3{}, S0{});
-    iter(k + 1, CB, CA, P3{}, S1{});
+  if constexpr (CD == 1) {
+    iter(kb, CA, CB, P0{}, S0{});      // H1(k0-2)
+    iter(kb + 1, CB, CA, P1{}, S1{});  // H1, E1 of k0-1
+    // iterations k0 .. k1+1 (H2/E2 of plane k-2 from k0-2 on; only planes
+    // in [k0, k1) are stored), plus one more when their number is odd, so
+    // that the loop body, written out twice, has no conditional part
+    const int kend = ((k1 - k0) & 1) ? k1 + 3 : k1 + 2;
+    for (int k = k0; k < kend; k += 2) {
+      iter(k, CA, CB, P3{}, S0{});
+      iter(k + 1, CB, CA, P3{}, S1{});
+    }
+  } else {
+    // plane p's update data in packet (p - kb) mod 3 (CA, CB, CC); the
+    // packet loaded in iteration k (plane k+2) is plane k-1's, consumed;
+    // iterations k0 .. k1+1 rounded up to a multiple of 6 (ring parity x
+    // packet), the extra ones storing nothing
+    iter(kb, CA, CC, P0{}, S0{});
+    iter(kb + 1, CB, CA, P1{}, S1{});
+    const int kend = k0 + 6 * ((k1 + 2 - k0 + 5) / 6);
+    for (int k = k0; k < kend; k += 6) {
+      iter(k, CC, CB, P3{}, S0{});
+      iter(k + 1, CA, CC, P3{}, S1{});
+      iter(k + 2, CB, CA, P3{}, S0{});
+      iter(k + 3, CC, CB, P3{}, S1{});
+      iter(k + 4, CA, CC, P3{}, S0{});
+      iter(k + 5, CB, CA, P3{}, S1{});
+    }
   }
 }
 
