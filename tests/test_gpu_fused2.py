@@ -348,3 +348,35 @@ def test_two_step_fp64_per_component_resources(cfg, name, shape, bl, bh, solids,
         a1 = getattr(p1, f).toarray()
         assert rel(a2, o) <= TOL_ORACLE64 and rel(a2, a1) <= TOL_KERNEL64, \
             (f, rel(a2, o), rel(a2, a1))
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, TOL_KERNEL64), (np.float32, TOL_KERNEL)],
+                         ids=["fp64", "fp32"])
+def test_full_size_pillbox_two_step_matches_one_step(dtype, tol):
+    """The lossy pillbox at its full size (256 x 256 x 768: lossy walls, 10-cell
+    cubic z PML, J = sigma*E), 41 steps from the same random fields: the
+    two-step sweep (20 launches of the default shape and one single step)
+    against the one-step sweep, relative L2 over the whole grid (the kernels
+    evaluate the same per-cell expressions; the oracle is checked on the
+    smaller grids above)."""
+    from wakis_amd.workloads import build, lossy_pillbox
+
+    nsteps = 41
+    wl = lossy_pillbox(256, 256, 768)
+    out = {}
+    for kernel in ("fused2", "fused"):
+        print(f"[case] full-size pillbox {np.dtype(dtype).name} kernel={kernel}", flush=True)
+        _, _, s = build(wl, dtype=dtype, kernel=kernel)
+        rng = np.random.default_rng(11)
+        for name in ("E", "H"):
+            getattr(s, name).fromarray(rng.standard_normal(3 * s.N).astype(dtype))
+        s._advance(nsteps)
+        out[kernel] = (s.E.toarray(), s.H.toarray(), s.ctx.launch_counts())
+        s.ctx.close()
+    e2, h2, c2 = out["fused2"]
+    e1, h1, c1 = out["fused"]
+    assert c2 == (1, 20) and c1 == (nsteps, 0), (c2, c1)
+    assert np.isfinite(e2).all() and np.isfinite(h2).all()
+    errs = (rel(e2, e1), rel(h2, h1))
+    print(errs, flush=True)
+    assert max(errs) <= tol, errs
