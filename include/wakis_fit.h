@@ -294,9 +294,10 @@ enum {
 int fit_set_tile2(fit_ctx *ctx, int32_t cfg);
 int fit_get_tile2(fit_ctx *ctx, int32_t *cfg, int32_t *tx, int32_t *ty);
 int fit_tune2(fit_ctx *ctx, int32_t reps, double *ms);
-/* The step kernel the next fit_step will use (FIT_KERNEL_SPLIT or
- * FIT_KERNEL_FUSED; AUTO resolves to FUSED when its second field buffers
- * fit in device memory). */
+/* The step kernel the next fit_step will use: FIT_KERNEL_SPLIT,
+ * FIT_KERNEL_FUSED, or FIT_KERNEL_FUSED2 when its pairs of steps run the
+ * two-step sweep (FP32 or FP64; see fit_step).  AUTO resolves to a fused
+ * sweep when its second field buffers fit in device memory. */
 int fit_kernel_in_use(fit_ctx *ctx, int32_t *kernel);
 int fit_sync(fit_ctx *ctx);
 
