@@ -546,7 +546,8 @@ def bench_cubic(args):
 def bench_batch(args):
     """64 uneven rectangular PEC cavities (box 60-140 mm, pipe 10-25 mm,
     length 60-160 mm, dx = 1.5 mm, z-end PML, beta in {0.8, 0.9, 1.0}) in
-    FP32, stepped by one launch per step (a block per tile) inside hipGraph launch
+    FP32 (or FP64 with --dtype f64), stepped by one launch per pair of steps
+    (two-step sweep) or per step, a block per tile, inside hipGraph launch
     groups (the wakesolve inner loop of every member: device beam + E_z
     probe + step), against the same cavities stepped one context at a time."""
     import wakis_amd as wa
@@ -554,7 +555,9 @@ def bench_batch(args):
 
     wls = cavity_batch(n=args.ncav, seed=3)
     t0 = time.perf_counter()
-    solvers = [build(w, dtype=np.float32)[2] for w in wls]
+    dtype = np.float32 if args.dtype == "f32" else np.float64
+    esz = np.dtype(dtype).itemsize
+    solvers = [build(w, dtype=dtype)[2] for w in wls]
     setup_s = time.perf_counter() - t0
     runs = [s._wake_setup(None, None, "longitudinal") for s in solvers]
     for s, r in zip(solvers, runs):
@@ -595,19 +598,19 @@ def bench_batch(args):
         # one two-step launch pair (lean chunks, then face chunks of every
         # member) per two steps: 12 field words per cell read and written
         kname = "fused2_batch"
-        alg = 2 * 12 * 4 * cells
+        alg = 2 * 12 * esz * cells
         launch_ms = 2 * ms_step
-        if len(solvers) == 64:
+        if len(solvers) == 64 and args.dtype == "f32":
             tx2, ty2 = ((24, 24) if cfg2 == 1 else (32, 16))
             traffic, tsrc = traffic_from_profiles(f"fused2_batch<float, {tx2}, {ty2}>", "f32",
                                                   "batch64", f"f2b-kc{kc2}")
     else:
         kname = "fused_batch"
-        alg = 12 * 4 * cells
+        alg = 12 * esz * cells
         launch_ms = ms_step
         dims = {0: (320, 3, 64), 1: (128, 4, 32), 2: (64, 8, 32), 3: (256, 4, 32), 4: (64, 4, 32),
                 5: (256, 4, 64)}.get(cfg)
-        if dims and len(solvers) == 64:
+        if dims and len(solvers) == 64 and args.dtype == "f32":
             traffic, tsrc = traffic_from_profiles(
                 f"fused_batch<float, {dims[0]}, {dims[1]}, {dims[2]}>", "f32", "batch64", cfg)
     achieved = alg / (launch_ms * 1e-3) / 1e9
@@ -624,7 +627,7 @@ def bench_batch(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": args.dtype,
         "data": "synthetic",
         "config": {
             "workload": f"batch of {len(solvers)} rectangular PEC cavities (seed 3), "
@@ -632,7 +635,7 @@ def bench_batch(args):
                         + ("two-step sweeps: one launch pair per two steps"
                            if pairs else "one launch per step")
                         + " in hipGraph groups of 16 steps; wakesolve inner loop",
-            "two_step": pairs, "two_step_tile": ({0: "32x16", 1: "24x24"}.get(cfg2)
+            "two_step": pairs, "two_step_tile": ({0: "32x16", 1: "24x24", 2: "28x12"}.get(cfg2)
                                                  if pairs else None),
             "kc": kc2 if pairs else None,
             "tile": None if pairs else cfg, "blocks": None if pairs else nblk,

@@ -383,10 +383,11 @@ int fit_batch_tune(fit_batch *batch, int32_t reps, double *ms);
  * replayed. */
 int fit_batch_set_graph(fit_batch *batch, int32_t on, int32_t group);
 /* Two-step pairs on (default) or off for this batch, their tile
- * configuration (FIT_BATCH2_32x16, FIT_BATCH2_24x24, or -1) and planes per z
- * chunk (8 .. 4096, or 0).  -1 and 0 pick by the launch model: the least
- * rounds x (kc + 8) x waves per block over all members' blocks. */
-enum { FIT_BATCH2_32x16 = 0, FIT_BATCH2_24x24 = 1, FIT_BATCH2_NTUNE = 6 };
+ * configuration (FIT_BATCH2_32x16 or FIT_BATCH2_24x24 in FP32,
+ * FIT_BATCH2_28x12 in FP64, or -1) and planes per z chunk (8 .. 4096, or
+ * 0).  -1 and 0 pick by the launch model: the least rounds x (kc + 8) x
+ * waves per block over all members' blocks. */
+enum { FIT_BATCH2_32x16 = 0, FIT_BATCH2_24x24 = 1, FIT_BATCH2_28x12 = 2, FIT_BATCH2_NTUNE = 9 };
 int fit_batch_set_two_step(fit_batch *batch, int32_t on, int32_t cfg, int32_t kc);
 /* Whether the next fit_batch_step steps pairs with the two-step sweep (*on),
  * with which tile configuration (*cfg, -1 if not) and planes per chunk. */

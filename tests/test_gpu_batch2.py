@@ -1,5 +1,6 @@
 """The batched two-step sweep (FP32 batches whose members are all at least 32
-cells wide and 16 deep: pairs of steps in two launches over every member):
+cells wide and 16 deep, FP64 ones at least 28 wide: pairs of steps in two
+launches over every member):
 each member against the FP32 oracle (relative L2 <= 1e-4) and against the
 one-step sweep alone (<= 1e-5: the same per-cell expressions), on uneven
 members with lean and face chunks, a conductive member with PML and an odd
@@ -34,16 +35,16 @@ def _solids(shape):
     ]
 
 
-def _members(kernel, seed=17):
+def _members(kernel, seed=17, dtype=np.float32):
     out = []
     rng = np.random.default_rng(seed)
     for shape, bl, bh, solids, nonuniform in MEMBERS:
-        osv, psv = make_pair(*shape, bl, bh, dtype=np.float32,
+        osv, psv = make_pair(*shape, bl, bh, dtype=dtype,
                              solids=_solids(shape) if solids else (),
                              bg="pec" if solids else (1.0, 1.0), n_pml=3, nonuniform=nonuniform,
                              init=False, kernel=kernel, seed=3)
         for name in ("E", "H"):
-            a = rng.standard_normal(3 * osv.N).astype(np.float32)
+            a = rng.standard_normal(3 * osv.N).astype(dtype)
             getattr(osv, name).fromarray(a)
             getattr(psv, name).fromarray(a)
         out.append((osv, psv))
@@ -140,3 +141,45 @@ def test_two_step_batch_wakesolve_matches_alone():
                 "E": rel(sb.E.toarray(), sa.E.toarray()), "H": rel(sb.H.toarray(), sa.H.toarray())}
         print(sb.Nx, sb.Nz, sb.Nt, errs, flush=True)
         assert max(errs.values()) <= TOL_KERNEL, errs
+
+
+def test_two_step_batch_fp64_matches_oracle_and_one_step():
+    """The FP64 two-step batch (28 x 12 tiles; the FP64 two-step shape of at
+    most 512 window cells the batch builds) on the same uneven members:
+    against the FP64 oracle (1e-10) and the FP64 one-step sweep alone
+    (1e-12); graph replays of 2-pair groups equal eager launch groups bit
+    for bit."""
+    import wakis_amd as wa
+
+    nsteps = 7
+    print("[case] two-step batch fp64 vs oracle", flush=True)
+    batch = _members("auto", dtype=np.float64)
+    alone = _members("fused", dtype=np.float64)
+    st = wa.BatchStepper([p for _, p in batch], graph=False)
+    on, cfg, kc = st.two_step()
+    assert on and cfg == 2 and kc >= 16, (on, cfg, kc)
+    st.step(nsteps)
+    for (osv, pb), (_, pa) in zip(batch, alone):
+        for _ in range(nsteps):
+            osv.one_step()
+        pa._advance(nsteps)
+        assert pb.ctx.launch_counts()[1] == 3, pb.ctx.launch_counts()
+        errs = {}
+        for f in "EHJ":
+            o = getattr(osv, f).toarray()
+            b = getattr(pb, f).toarray()
+            a = getattr(pa, f).toarray()
+            errs[f] = (rel(b, o), rel(b, a))
+        print(pb.Nx, pb.Ny, pb.Nz, errs, flush=True)
+        for f, (eo, ek) in errs.items():
+            assert eo <= 1e-10 and ek <= 1e-12, (pb.Nx, f, eo, ek)
+    ga_m = _members("auto", seed=5, dtype=np.float64)
+    gb_m = _members("auto", seed=5, dtype=np.float64)
+    ga = wa.BatchStepper([p for _, p in ga_m], graph=True, group=4)
+    gb = wa.BatchStepper([p for _, p in gb_m], graph=False, group=4)
+    for n in (5, 6, 1):
+        ga.step(n)
+        gb.step(n)
+        for (_, pa), (_, pb) in zip(ga_m, gb_m):
+            for f in "EH":
+                assert np.array_equal(getattr(pa, f).toarray(), getattr(pb, f).toarray()), (n, f)

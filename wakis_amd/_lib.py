@@ -517,7 +517,7 @@ class Batch:
     def tune_two_step(self, reps: int = 3):
         """(tile configuration, planes per chunk, {(cfg, kc): ms per launch})."""
         cfg, kc = C.c_int32(0), C.c_int32(0)
-        ms = np.zeros(6, dtype=np.float64)
+        ms = np.zeros(9, dtype=np.float64)  # FIT_BATCH2_NTUNE
         check(self.lib.fit_batch_tune_two_step(self.h, int(reps), C.byref(cfg), C.byref(kc),
                                                dptr(ms)), "fit_batch_tune_two_step")
         times = {(i // 3, (48, 64, 96)[i % 3]): float(v) for i, v in enumerate(ms) if v > 0}

@@ -24,11 +24,12 @@ class BatchStepper:
 
     def __init__(self, solvers, graph: bool = True, group: int = 16, tile=None,
                  two_step: bool = True, kc2: int = 0, cfg2: int = -1):
-        """two_step: FP32 batches whose members all suit the two-step sweep
-        (nx >= 32, ny >= 16) advance pairs of steps with it (one read and
-        write of E and H per two steps); False keeps the one-step launch.
-        kc2, cfg2: planes per z chunk and tile configuration (0: 32 x 16,
-        1: 24 x 24) of the two-step sweep (0, -1: by its launch model)."""
+        """two_step: batches whose members all suit the two-step sweep
+        (FP32: nx >= 32 and ny >= 16, or nx, ny >= 24; FP64: nx >= 28, ny >= 16) advance pairs of
+        steps with it (one read and write of E and H per two steps); False
+        keeps the one-step launch.  kc2, cfg2: planes per z chunk and tile
+        configuration (FP32 0: 32 x 16, 1: 24 x 24; FP64 2: 28 x 12) of the
+        two-step sweep (0, -1: by its launch model)."""
         self.solvers = list(solvers)
         for s in self.solvers:
             s._require_device()

@@ -885,12 +885,14 @@ struct Batch2Member {
 // threads) or 24 x 24 (28 x 28, 784 threads).  Against the 64 x 8 tile of the
 // single-grid launch both leave less overhang on the narrow grids of a batch
 // and have fewer window cells per tile cell (1.41 and 1.36 against 1.59);
-// 24 x 24 needs ny >= 24.
-enum { F2B_CFG_32x16 = 0, F2B_CFG_24x24 = 1, F2B_NCFG = 2 };
+// 24 x 24 needs ny >= 24.  Both are FP32 only (12-13 waves per block); FP64
+// batches take 28 x 12 (32 x 16 window, 512 threads, 8 waves: two per SIMD,
+// as the single-grid FP64 shapes).
+enum { F2B_CFG_32x16 = 0, F2B_CFG_24x24 = 1, F2B_CFG_28x12 = 2, F2B_NCFG = 3 };
 
 inline void f2b_dims(int cfg, int &tx, int &ty) {
-  tx = cfg == F2B_CFG_24x24 ? 24 : 32;
-  ty = cfg == F2B_CFG_24x24 ? 24 : 16;
+  tx = cfg == F2B_CFG_24x24 ? 24 : cfg == F2B_CFG_28x12 ? 28 : 32;
+  ty = cfg == F2B_CFG_24x24 ? 24 : cfg == F2B_CFG_28x12 ? 12 : 16;
 }
 
 template <typename T, int TX, int TY>
@@ -1122,8 +1124,12 @@ inline bool fused2_batch_supported_t(const StepArgs<T> &a) {
 
 template <typename T>
 inline bool fused2_batch_supported(const StepArgs<T> &a, int cfg) {
-  return cfg == F2B_CFG_24x24 ? fused2_batch_supported_t<T, 24, 24>(a)
-                              : fused2_batch_supported_t<T, 32, 16>(a);
+  switch (cfg) {
+    case F2B_CFG_32x16: return fused2_batch_supported_t<T, 32, 16>(a);
+    case F2B_CFG_24x24: return fused2_batch_supported_t<T, 24, 24>(a);
+    case F2B_CFG_28x12: return fused2_batch_supported_t<T, 28, 12>(a);
+    default: return false;
+  }
 }
 
 // Launches two steps with tiles TX x TY; false (nothing launched) when not
