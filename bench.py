@@ -600,10 +600,11 @@ def bench_batch(args):
         kname = "fused2_batch"
         alg = 2 * 12 * esz * cells
         launch_ms = 2 * ms_step
-        if len(solvers) == 64 and args.dtype == "f32":
-            tx2, ty2 = ((24, 24) if cfg2 == 1 else (32, 16))
-            traffic, tsrc = traffic_from_profiles(f"fused2_batch<float, {tx2}, {ty2}>", "f32",
-                                                  "batch64", f"f2b-kc{kc2}")
+        if len(solvers) == 64:
+            tx2, ty2 = {0: (32, 16), 1: (24, 24), 2: (28, 12)}[cfg2]
+            ctype = "float" if args.dtype == "f32" else "double"
+            traffic, tsrc = traffic_from_profiles(f"fused2_batch<{ctype}, {tx2}, {ty2}>",
+                                                  args.dtype, "batch64", f"f2b-kc{kc2}")
     else:
         kname = "fused_batch"
         alg = 12 * esz * cells
